@@ -1,0 +1,267 @@
+"""Benchmark of the geometric-query path on one MI355X.
+
+Headline workload (one "step" = one pass of the hot path over one batch):
+  rays_scene -- procedural terrain 708x708 (999,698 tris) + 64 icospheres L5
+  (1,310,720 tris) = 2,310,418 FP32 triangles; 1,048,576 incoherent rays traced for the
+  closest hit (tmin 1e-5, tmax 1e7) and again for any hit (tmax U[1,50]).  value = rays
+  per second over both passes, inputs resident in HBM, BVH built before timing.
+
+The same JSON line carries:
+  roofline      achieved algorithmic GB/s of the closest-hit traversal kernel (bytes per
+                ray = 32 in + 16 out + 128 per wide node visited + 36 per triangle tested,
+                visits counted by the kernel's counting variant) over its mean duration
+                (HIP events on the launch stream), against the 8 TB/s HBM3E peak.
+  cpu_baseline  the oracle's CPU LBVH (C, pthreads on every host core) on a bounded
+                sample of the same rays.
+  extra         the other workloads: closest-point queries/s (5,120 / 5,242,880 tris),
+                forward and forward+backward on the differentiable 1.3M-tri workload,
+                build times.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-rt_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "closest-pt queries/s + rays/s, 1M-tri, fwd & fwd+bwd, batched vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = {2: 64, 4: 128, 8: 256}
+TRI_BYTES = 36  # three float3 vertices per triangle tested
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup, stream=None):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps  # ms
+
+
+def ray_bench(args, torch, gpurt, W, O, dev, ctx):
+    t0 = time.time()
+    w = W.rays_scene()
+    log(f"[bench] rays_scene generated in {time.time() - t0:.1f}s: {w['meta']}")
+    V, F = w["verts"], w["tris"]
+    tv = torch.from_numpy(V).to(dev)
+    tf = torch.from_numpy(F).to(dev)
+    rays = torch.from_numpy(w["rays"]).to(dev)
+    rays_any = torch.from_numpy(w["rays_any"]).to(dev)
+    n = rays.shape[0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    info = bvh.info()
+    log(f"[bench] GPU build {build_ms:.1f} ms, {info}")
+    prim = torch.empty(n, dtype=torch.int32, device=dev)
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    uv = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    occ = torch.empty(n, dtype=torch.uint8, device=dev)
+
+    def step():
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack)
+        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack)
+
+    ms = timed(step, args.steps, args.warmup)
+    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack),
+                       args.steps, 1)
+    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack), args.steps, 1)
+    st = torch.zeros(2, dtype=torch.int64, device=dev)
+    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, stats=st)
+    torch.cuda.synchronize()
+    nodes, tris = st.cpu().tolist()
+    bytes_closest = n * (32 + 16) + nodes * NODE_BYTES[args.width] + tris * TRI_BYTES
+    st.zero_()
+    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, stats=st)
+    torch.cuda.synchronize()
+    nodes_a, tris_a = st.cpu().tolist()
+    hit_rate = float((prim >= 0).float().mean().item())
+    occ_rate = float(occ.float().mean().item())
+
+    # parity spot check against the oracle's CPU LBVH on a sample (not timed)
+    sample = np.arange(0, n, n // 4096)
+    cpu_bvh_t0 = time.perf_counter()
+    cbvh = O.CpuBvh(V, F, 4)
+    cpu_build_s = time.perf_counter() - cpu_bvh_t0
+    ep, et, _ = cbvh.trace_closest(w["rays"][sample])
+    parity = bool(np.array_equal(prim.cpu().numpy()[sample], ep) and
+                  np.array_equal(t.cpu().numpy()[sample], et))
+
+    # CPU baseline: the oracle LBVH over a bounded sample of the same rays, all host cores
+    threads = O.num_threads()
+    m = args.cpu_rays
+    R1, R2 = w["rays"][:m], w["rays_any"][:m]
+    cbvh.trace_closest(R1[:1024])
+    t0 = time.perf_counter()
+    cbvh.trace_closest(R1)
+    cbvh.trace_any(R2)
+    cpu_s = time.perf_counter() - t0
+    cpu_rate = 2 * m / cpu_s
+    roof = {
+        "kernel": "k_ray<W=%d,closest>" % args.width,
+        "bound": "hbm",
+        "achieved": bytes_closest / (ms_closest * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "traffic": None,
+        "bytes_per_launch": bytes_closest,
+        "nodes_per_ray": nodes / n,
+        "tris_per_ray": tris / n,
+        "kernel_ms": ms_closest,
+    }
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    res = {
+        "value": 2 * n / (ms * 1e-3),
+        "ms_per_step": ms,
+        "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
+                   "passes": ["closest_hit", "any_hit"], "node_width": args.width,
+                   "leaf_size": 4, "stack_depth": args.stack},
+        "roofline": roof,
+        "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
+                         "sample": f"{m} closest-hit + {m} any-hit rays of rays_scene on the "
+                                   f"oracle CPU LBVH (build {cpu_build_s:.1f}s not included)",
+                         "seconds": cpu_s},
+        "extra_rays": {"closest_rays_per_s": n / (ms_closest * 1e-3),
+                       "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
+                       "hit_rate": hit_rate, "occluded_rate": occ_rate,
+                       "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
+                       "bvh": info, "parity_vs_cpu_lbvh_sample": parity},
+    }
+    del bvh
+    return res
+
+
+def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    tv = torch.from_numpy(V).to(dev)
+    tf = torch.from_numpy(F).to(dev)
+    q = torch.from_numpy(Q).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    n = q.shape[0]
+    out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
+           torch.empty(n, device=dev), torch.empty((n, 3), device=dev))
+    ms = timed(lambda: bvh.closest_point(q, out=out, stack_depth=args.stack), args.steps, 2)
+    st = torch.zeros(2, dtype=torch.int64, device=dev)
+    bvh.closest_point(q, out=out, stack_depth=args.stack, stats=st)
+    torch.cuda.synchronize()
+    nodes, tris = st.cpu().tolist()
+    byts = n * (12 + 20 + 12) + nodes * NODE_BYTES[args.width] + tris * TRI_BYTES
+    del bvh
+    return {"workload": name, "n_tris": int(F.shape[0]), "n_queries": n,
+            "queries_per_s": n / (ms * 1e-3), "ms": ms, "gpu_build_ms": build_ms,
+            "nodes_per_query": nodes / n, "tris_per_query": tris / n,
+            "algorithmic_GBps": byts / (ms * 1e-3) / 1e9}
+
+
+def diff_bench(args, torch, gpurt, W, dev, ctx):
+    w = W.diff()
+    V, F = w["verts"], w["tris"]
+    tv = torch.from_numpy(V).to(dev)
+    tf = torch.from_numpy(F).to(dev)
+    q0 = torch.from_numpy(w["queries"]).to(dev)
+    o0 = torch.from_numpy(w["ray_o"]).to(dev)
+    d0 = torch.from_numpy(w["ray_d"]).to(dev)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    nq, nr = q0.shape[0], o0.shape[0]
+
+    def fwd():
+        with torch.no_grad():
+            dist, _ = gpurt.closest_point_distance(bvh, q0, tv)
+            t, _ = gpurt.ray_hit_t(bvh, o0, d0, tv)
+        return dist, t
+
+    def fwd_bwd():
+        q = q0.detach().requires_grad_(True)
+        o = o0.detach().requires_grad_(True)
+        v = tv.detach().requires_grad_(True)
+        dist, _ = gpurt.closest_point_distance(bvh, q, v)
+        t, _ = gpurt.ray_hit_t(bvh, o, d0, v)
+        loss = dist.sum() + torch.where(torch.isfinite(t), t, torch.zeros_like(t)).sum()
+        loss.backward()
+        return loss
+
+    ms_f = timed(fwd, args.steps, 2)
+    ms_fb = timed(fwd_bwd, args.steps, 2)
+    del bvh
+    return {"workload": "diff", "n_tris": int(F.shape[0]), "n_queries": nq, "n_rays": nr,
+            "fwd_units_per_s": (nq + nr) / (ms_f * 1e-3), "fwd_ms": ms_f,
+            "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
+    ap.add_argument("--stack", type=int, default=32, choices=[16, 32, 64])
+    ap.add_argument("--cpu-rays", type=int, default=131072)
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box "
+              "here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gpurt
+    from gpurt import workloads as W
+    from oracle import oracle as O
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    ctx = gpurt.Context(0)
+    r = ray_bench(args, torch, gpurt, W, O, dev, ctx)
+    extra = {"rays": r.pop("extra_rays")}
+    if not args.no_extra:
+        log("[bench] cpq_small")
+        extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx)
+        log("[bench] diff fwd / fwd+bwd")
+        extra["diff"] = diff_bench(args, torch, gpurt, W, dev, ctx)
+        if args.large:
+            log("[bench] cpq_large")
+            extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx)
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,304 @@
+// abi.hip -- the extern "C" entry points of include/gpurt.h: argument checks, device
+// selection, ownership, and dispatch to the builder / traversal / backward launchers.
+#include <cstring>
+#include <string>
+
+#include "gpurt_internal.h"
+
+namespace gpurt_impl {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return GPURT_E_OOM;
+    return GPURT_E_DEVICE;
+}
+
+// Makes `dev` current for the scope and restores the caller's device afterwards.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+__global__ void k_compact_verts(uint32_t n, const float* __restrict__ v, uint64_t stride,
+                                float* __restrict__ out) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float* p = v + (size_t)i * stride;
+    out[3 * (size_t)i] = p[0];
+    out[3 * (size_t)i + 1] = p[1];
+    out[3 * (size_t)i + 2] = p[2];
+}
+
+int compact_verts(const float* verts, size_t stride_floats, size_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_verts, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, verts,
+                       (uint64_t)stride_floats, out);
+    GPURT_HIP(hipGetLastError());
+    return GPURT_OK;
+}
+
+}  // namespace gpurt_impl
+
+using namespace gpurt_impl;
+
+extern "C" {
+
+const char* gpurt_last_error(void) { return g_last_error.c_str(); }
+
+int gpurt_ctx_create(int device, gpurt_ctx* out) {
+    if (!out) return set_error(GPURT_E_INVALID, "out is null");
+    *out = nullptr;
+    int n_dev = 0;
+    GPURT_HIP(hipGetDeviceCount(&n_dev));
+    if (device < 0 || device >= n_dev) return set_error(GPURT_E_INVALID, "device out of range");
+    DeviceGuard g(device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    auto* c = new gpurt_ctx_s();
+    c->device = device;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
+        c->n_cu = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) ==
+            hipSuccess &&
+        v > 0)
+        c->lds_per_cu = v;
+    if (hipMalloc((void**)&c->err, 256) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return set_error(GPURT_E_OOM, "context allocation failed");
+    }
+    if (hipMemset(c->err, 0, 256) != hipSuccess) {
+        (void)hipFree(c->err);
+        delete c;
+        return set_error(GPURT_E_DEVICE, "context init failed");
+    }
+    // keep freed stream-ordered allocations (traversal spill stacks) in the pool
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t thr = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    (void)hipGetLastError();
+    *out = c;
+    return GPURT_OK;
+}
+
+void gpurt_ctx_destroy(gpurt_ctx ctx) {
+    if (!ctx) return;
+    DeviceGuard g(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->err) (void)hipFree(ctx->err);
+    delete ctx;
+}
+
+static int check_mesh(const float* verts, size_t vstride_bytes, size_t n_verts,
+                      const uint32_t* tri_idx, size_t n_tris) {
+    if (!verts || !tri_idx) return set_error(GPURT_E_INVALID, "null vertex or index pointer");
+    if (vstride_bytes < 12 || vstride_bytes % 4 != 0)
+        return set_error(GPURT_E_INVALID, "vstride_bytes must be a multiple of 4 and >= 12");
+    if (n_verts == 0 || n_verts > 0xFFFFFFFFull)
+        return set_error(GPURT_E_INVALID, "n_verts out of range");
+    if (n_tris == 0 || n_tris > (size_t)kMaxLeafFirst - 8)
+        return set_error(GPURT_E_INVALID, "n_tris out of range (1 .. 2^28-9)");
+    return GPURT_OK;
+}
+
+static int check_build_opts(const gpurt_build_opts* o, int& W, int& leaf, int& tsize, int& trounds) {
+    W = 4;
+    leaf = 4;
+    tsize = 0;
+    trounds = 0;
+    if (o) {
+        if (o->node_width) W = o->node_width;
+        if (o->leaf_size) leaf = o->leaf_size;
+        tsize = o->treelet_size;
+        trounds = o->treelet_rounds;
+    }
+    if (W != 2 && W != 4 && W != 8) return set_error(GPURT_E_UNSUPPORTED, "node_width must be 2, 4 or 8");
+    if (leaf < 1 || leaf > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
+    if (tsize != 0 && (tsize < 5 || tsize > 7))
+        return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
+    if (trounds < 0 || trounds > 8) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0..8");
+    if (tsize && trounds == 0) trounds = 1;
+    return GPURT_OK;
+}
+
+int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, size_t n_verts,
+                    const uint32_t* tri_idx, size_t n_tris, const gpurt_build_opts* opts,
+                    void* stream, gpurt_bvh* out) {
+    if (!ctx || !out) return set_error(GPURT_E_INVALID, "null context or output handle");
+    *out = nullptr;
+    int rc = check_mesh(verts, vstride_bytes, n_verts, tri_idx, n_tris);
+    if (rc) return rc;
+    int W, leaf, tsize, trounds;
+    if ((rc = check_build_opts(opts, W, leaf, tsize, trounds))) return rc;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    hipStream_t s = (hipStream_t)stream;
+
+    SegDesc sd{};
+    sd.verts = verts;
+    sd.vstride = vstride_bytes / 4;
+    sd.n_verts = (uint32_t)n_verts;
+    sd.n_tris = (uint32_t)n_tris;
+    sd.idx = tri_idx;
+    sd.tri_offset = 0;
+    auto* b = new gpurt_bvh_s();
+    b->ctx = ctx;
+    b->n_verts = (int64_t)n_verts;
+    rc = build_segments(ctx, &sd, 1, W, leaf, tsize, trounds, s, true, b);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    // keep the index buffer and compact positions for the backward passes and refit
+    if (hipMalloc((void**)&b->tri_idx, sizeof(uint32_t) * 3 * n_tris) != hipSuccess ||
+        hipMalloc((void**)&b->verts, sizeof(float) * 3 * n_verts) != hipSuccess) {
+        (void)hipGetLastError();
+        gpurt_bvh_destroy(b);
+        return set_error(GPURT_E_OOM, "hipMalloc of mesh copy failed");
+    }
+    if (hipMemcpyAsync(b->tri_idx, tri_idx, sizeof(uint32_t) * 3 * n_tris, hipMemcpyDeviceToDevice, s) !=
+        hipSuccess) {
+        rc = hip_error(hipGetLastError(), "copying indices");
+        gpurt_bvh_destroy(b);
+        return rc;
+    }
+    if ((rc = compact_verts(verts, vstride_bytes / 4, n_verts, b->verts, s)) != GPURT_OK) {
+        gpurt_bvh_destroy(b);
+        return rc;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        rc = hip_error(hipGetLastError(), "build");
+        gpurt_bvh_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return GPURT_OK;
+}
+
+int gpurt_bvh_refit(gpurt_bvh bvh, const float* verts, size_t vstride_bytes, void* stream) {
+    if (!bvh || !verts) return set_error(GPURT_E_INVALID, "null argument");
+    if (vstride_bytes < 12 || vstride_bytes % 4 != 0)
+        return set_error(GPURT_E_INVALID, "vstride_bytes must be a multiple of 4 and >= 12");
+    DeviceGuard g(bvh->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    int rc = launch_refit(bvh, verts, vstride_bytes / 4, (hipStream_t)stream);
+    if (rc) return rc;
+    GPURT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return GPURT_OK;
+}
+
+void gpurt_bvh_destroy(gpurt_bvh b) {
+    if (!b) return;
+    DeviceGuard g(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (b->nodes) (void)hipFree(b->nodes);
+    if (b->tris) (void)hipFree(b->tris);
+    if (b->tri_idx) (void)hipFree(b->tri_idx);
+    if (b->verts) (void)hipFree(b->verts);
+    if (b->leaf_order) (void)hipFree(b->leaf_order);
+    delete b;
+}
+
+int gpurt_bvh_get_info(gpurt_bvh b, gpurt_bvh_info* out) {
+    if (!b || !out) return set_error(GPURT_E_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->n_tris = b->n_tris;
+    out->n_verts = b->n_verts;
+    out->n_nodes = b->n_nodes;
+    out->node_bytes = (int64_t)node_bytes(b->width);
+    out->tri_bytes = 48;
+    out->depth = b->depth;
+    out->node_width = b->width;
+    out->leaf_size = b->leaf_size;
+    out->sah_cost = b->sah_cost;
+    return GPURT_OK;
+}
+
+static int check_opts(const gpurt_query_opts* o) {
+    if (!o) return GPURT_OK;
+    if (o->stack_depth != 0 && o->stack_depth != 16 && o->stack_depth != 32 && o->stack_depth != 64)
+        return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 16, 32 or 64");
+    if (o->waves_per_cu < 0 || o->waves_per_cu > 64)
+        return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
+    return GPURT_OK;
+}
+
+int gpurt_closest_point(gpurt_bvh b, const float* q, size_t n, int32_t* prim, float* pt,
+                        float* dist, float* bary, const gpurt_query_opts* opts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!q || !prim || !pt || !dist) return set_error(GPURT_E_INVALID, "null query/output pointer");
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_cpq(b, q, n, prim, pt, dist, bary, opts, (hipStream_t)stream);
+}
+
+int gpurt_trace_closest(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
+                        float* uv, const gpurt_query_opts* opts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!rays || !prim || !t) return set_error(GPURT_E_INVALID, "null ray/output pointer");
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_ray(b, rays, n, prim, t, uv, nullptr, false, opts, (hipStream_t)stream);
+}
+
+int gpurt_trace_any(gpurt_bvh b, const gpurt_ray* rays, size_t n, uint8_t* occluded,
+                    const gpurt_query_opts* opts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!rays || !occluded) return set_error(GPURT_E_INVALID, "null ray/output pointer");
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_ray(b, rays, n, nullptr, nullptr, nullptr, occluded, true, opts, (hipStream_t)stream);
+}
+
+int gpurt_closest_point_bwd(gpurt_bvh b, const float* q, size_t n, const int32_t* prim,
+                            const float* bary, const float* pt, const float* dist,
+                            const float* g_dist, float* g_q, float* g_verts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!q || !prim || !pt || !dist || !g_dist)
+        return set_error(GPURT_E_INVALID, "null input pointer");
+    if (g_verts && !bary) return set_error(GPURT_E_INVALID, "g_verts needs the saved barycentrics");
+    if (n > 0xFFFFFFFFull) return set_error(GPURT_E_INVALID, "too many queries in one call");
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_cpq_bwd(b, q, n, prim, bary, pt, dist, g_dist, g_q, g_verts, (hipStream_t)stream);
+}
+
+int gpurt_trace_bwd(gpurt_bvh b, const gpurt_ray* rays, size_t n, const int32_t* prim,
+                    const float* t, const float* uv, const float* g_t, float* g_origin,
+                    float* g_dir, float* g_verts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!rays || !prim || !t || !g_t) return set_error(GPURT_E_INVALID, "null input pointer");
+    if (g_verts && !uv) return set_error(GPURT_E_INVALID, "g_verts needs the saved (u,v)");
+    if (n > 0xFFFFFFFFull) return set_error(GPURT_E_INVALID, "too many rays in one call");
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_ray_bwd(b, rays, n, prim, t, uv, g_t, g_origin, g_dir, g_verts, (hipStream_t)stream);
+}
+
+}  // extern "C"

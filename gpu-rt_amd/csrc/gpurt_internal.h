@@ -1,0 +1,134 @@
+// gpurt_internal.h -- shared types of the gfx950 BVH builder and traversal kernels.
+//
+// Device layout (all 128-byte aligned allocations from hipMalloc):
+//   wide nodes   WNode<W>: per child slot j: box lo/hi as six W-float arrays (SoA inside
+//                the node, so one node is 6*W floats of boxes + W child words + W aux
+//                words = 32*W bytes: 64 B for W=2, 128 B (one L2 line) for W=4, 256 B
+//                for W=8).  Boxes are inflated by `pad` so culls stay conservative.
+//   triangles    3 float4 per triangle in leaf order: (a.xyz, id), (b.xyz, 0),
+//                (c.xyz, 0) with id = the triangle's index in its mesh (float bits).
+//   child word   internal child: wide-node index (< 2^31);
+//                leaf: bit31 | (count-1) << 28 | first packed triangle (count 1..8);
+//                empty slot: 0xFFFFFFFF.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gpurt.h"
+
+namespace gpurt_impl {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMaxLeafFirst = 0x0FFFFFFFu;  // 2^28 - 1 packed triangles max
+constexpr int kMaxLevels = 128;                  // wide-node levels the collapse can emit
+constexpr int kBlock = 256;
+
+__host__ __device__ inline uint32_t make_leaf(uint32_t first, uint32_t count) {
+    return kLeafBit | ((count - 1u) << 28) | first;
+}
+__host__ __device__ inline bool child_is_leaf(uint32_t c) { return (c & kLeafBit) != 0u; }
+__host__ __device__ inline uint32_t leaf_first(uint32_t c) { return c & kMaxLeafFirst; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t c) { return ((c >> 28) & 7u) + 1u; }
+
+template <int W>
+struct alignas(32 * W) WNode {
+    float lox[W], loy[W], loz[W];
+    float hix[W], hiy[W], hiz[W];
+    uint32_t child[W];
+    uint32_t aux[W];  // triangles under the child (for cost estimates)
+};
+static_assert(sizeof(WNode<2>) == 64, "node size");
+static_assert(sizeof(WNode<4>) == 128, "node size");
+static_assert(sizeof(WNode<8>) == 256, "node size");
+
+inline size_t node_bytes(int W) { return (size_t)32 * (size_t)W; }
+
+// One mesh of a (possibly single-mesh) build.
+struct SegDesc {
+    const float* verts;
+    uint64_t vstride;  // in floats
+    uint32_t n_verts;
+    uint32_t n_tris;
+    const uint32_t* idx;
+    uint32_t tri_offset;  // first triangle of this mesh in the concatenated order
+    uint32_t pad_;
+};
+
+// Build products that the traversal kernels read.
+struct DeviceBvh {
+    const void* nodes;   // WNode<W>[]
+    const float4* tris;  // 3 per triangle
+    uint32_t root;       // wide-node index of the root
+    uint32_t n_tris;
+};
+
+}  // namespace gpurt_impl
+
+struct gpurt_ctx_s {
+    int device = 0;
+    int n_cu = 256;
+    int lds_per_cu = 160 * 1024;
+    uint32_t* err = nullptr;  // sticky device error bits (bit 0: traversal stack overflow)
+};
+
+struct gpurt_bvh_s {
+    gpurt_ctx ctx = nullptr;
+    int width = 4;
+    int leaf_size = 4;
+    int depth = 0;
+    int64_t n_tris = 0;
+    int64_t n_verts = 0;
+    int64_t n_nodes = 0;
+    int64_t node_capacity = 0;
+    float scale = 0.f;
+    float pad = 0.f;
+    float sah_cost = 0.f;
+    void* nodes = nullptr;        // WNode<W>[node_capacity]
+    float4* tris = nullptr;       // 3 * n_tris
+    uint32_t* tri_idx = nullptr;  // 3 * n_tris, original order (for backward / refit)
+    float* verts = nullptr;       // 3 * n_verts compact positions, original order
+    uint32_t* leaf_order = nullptr;  // packed position -> original triangle (refit)
+};
+
+namespace gpurt_impl {
+
+// error plumbing (abi.hip)
+int set_error(int code, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+
+#define GPURT_HIP(call)                                            \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return ::gpurt_impl::hip_error(e_, #call); \
+    } while (0)
+
+// builder (build.hip)
+struct BuildScratch;
+int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width, int leaf_size,
+                   int treelet_size, int treelet_rounds, hipStream_t stream, bool sync_host,
+                   gpurt_bvh_s* out_single);
+
+// traversal launchers (query.hip)
+int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,
+               float* bary, const gpurt_query_opts* opts, hipStream_t s);
+int launch_ray(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
+               float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, hipStream_t s);
+
+// backward (backward.hip)
+int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim,
+                   const float* bary, const float* pt, const float* dist, const float* g,
+                   float* g_q, float* g_verts, hipStream_t s);
+int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_t* prim,
+                   const float* t, const float* uv, const float* g, float* g_o, float* g_d,
+                   float* g_verts, hipStream_t s);
+
+int launch_refit(gpurt_bvh_s* b, const float* verts, size_t vstride_floats, hipStream_t s);
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace gpurt_impl

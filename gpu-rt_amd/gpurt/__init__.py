@@ -1,0 +1,286 @@
+"""gpurt -- MI355X geometric queries (BVH build, closest point, closest/any-hit rays).
+
+Python host layer over libgpurt.so (include/gpurt.h).  Torch tensors supply device
+memory and the current HIP stream; every computation runs in the library's HIP kernels.
+
+    ctx = gpurt.Context(0)
+    bvh = gpurt.Bvh(ctx, verts, tris)             # VK::Accel(const Mesh&) equivalent
+    prim, pt, dist, bary = bvh.closest_point(q)
+    prim, t, uv = bvh.trace_closest(rays)          # rays: (n, 8) = o.xyz, tmin, d.xyz, tmax
+    occ = bvh.trace_any(rays)
+    d = gpurt.closest_point_distance(bvh, q, verts)   # differentiable
+    t = gpurt.ray_hit_t(bvh, rays_o, rays_d, verts)   # differentiable
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import BuildOpts, BvhInfo, GpurtError, QueryOpts
+
+__all__ = ["Context", "Bvh", "GpurtError", "closest_point_distance", "ray_hit_t", "make_rays",
+           "default_context"]
+
+EPS = 1e-5          # rtcommon.glsl:7 -- ray tmin of trace_ray / visibility
+LARGE_DIST = 1e7    # rtcommon.glsl:6 -- ray tmax of trace_ray
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _need(t, dtype, shape_tail, name, device):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    if t.device != device:
+        raise ValueError(f"{name} must be on {device}, got {t.device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if tuple(t.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f"{name} must have shape (n, {', '.join(map(str, shape_tail))}), "
+                         f"got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+class Context:
+    """Library context on one HIP device (gpurt_ctx_create)."""
+
+    def __init__(self, device=0):
+        lib = _lib.load()
+        if isinstance(device, torch.device):
+            device = device.index or 0
+        self.device = torch.device("cuda", int(device))
+        h = ctypes.c_void_p()
+        _lib.check(lib.gpurt_ctx_create(int(device), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            torch.cuda.synchronize(self.device)
+            _lib.load().gpurt_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def make_rays(o, d, tmin=EPS, tmax=LARGE_DIST):
+    """Pack origins/directions (n,3) and scalar or (n,) bounds into the (n,8) ray layout."""
+    n = o.shape[0]
+    r = torch.empty((n, 8), dtype=torch.float32, device=o.device)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+class Bvh:
+    """A BVH over one triangle mesh (gpurt_bvh_build); mirrors VK::Accel(const Mesh&).
+
+    verts: (n_verts, k) float32 device tensor with positions in columns 0..2 (k = 3, 4 or
+    12 -- the reference's 48-byte Mesh::Vertex); tris: (n_tris, 3) int32/uint32 indices.
+    """
+
+    def __init__(self, ctx, verts, tris, node_width=4, leaf_size=4, treelet_size=0,
+                 treelet_rounds=0):
+        lib = _lib.load()
+        self.ctx = ctx
+        dev = ctx.device
+        if verts.dim() != 2 or verts.shape[1] < 3:
+            raise ValueError("verts must be (n, k>=3)")
+        _need(verts, torch.float32, verts.shape[1:], "verts", dev)
+        if tris.dtype not in (torch.int32, torch.uint32):
+            raise TypeError("tris must be int32 or uint32")
+        _need(tris, tris.dtype, (3,), "tris", dev)
+        self.n_verts = verts.shape[0]
+        self.n_tris = tris.shape[0]
+        self._tris = tris  # indices are needed by no kernel after the build; kept for users
+        opts = BuildOpts(node_width, leaf_size, treelet_size, treelet_rounds)
+        h = ctypes.c_void_p()
+        _lib.check(lib.gpurt_bvh_build(ctx.handle, _ptr(verts), verts.shape[1] * 4,
+                                       verts.shape[0], _ptr(tris), tris.shape[0],
+                                       ctypes.byref(opts), _stream(dev), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device(self):
+        return self.ctx.device
+
+    def info(self):
+        inf = BvhInfo()
+        _lib.check(_lib.load().gpurt_bvh_get_info(self._h, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in BvhInfo._fields_}
+
+    def refit(self, verts):
+        _need(verts, torch.float32, verts.shape[1:], "verts", self.device)
+        if verts.shape[0] != self.n_verts:
+            raise ValueError("refit needs the same vertex count")
+        _lib.check(_lib.load().gpurt_bvh_refit(self._h, _ptr(verts), verts.shape[1] * 4,
+                                               _stream(self.device)))
+
+    def _qopts(self, stack_depth, waves_per_cu, stats):
+        """stats: optional int64 device tensor of 2 counters (nodes visited, triangles
+        tested), accumulated by the counting kernel variant."""
+        if stats is not None:
+            if (stats.dtype != torch.int64 or stats.numel() < 2 or stats.device != self.device
+                    or not stats.is_contiguous()):
+                raise ValueError("stats must be a contiguous int64 tensor of >= 2 elements "
+                                 "on the BVH's device")
+        return QueryOpts(stack_depth, waves_per_cu,
+                         ctypes.c_void_p(stats.data_ptr()) if stats is not None else None)
+
+    def closest_point(self, q, with_bary=True, stack_depth=0, waves_per_cu=0, stats=None,
+                      out=None):
+        dev = self.device
+        _need(q, torch.float32, (3,), "q", dev)
+        n = q.shape[0]
+        if out is None:
+            prim = torch.empty(n, dtype=torch.int32, device=dev)
+            pt = torch.empty((n, 3), dtype=torch.float32, device=dev)
+            dist = torch.empty(n, dtype=torch.float32, device=dev)
+            bary = torch.empty((n, 3), dtype=torch.float32, device=dev) if with_bary else None
+        else:
+            prim, pt, dist, bary = out
+        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        _lib.check(_lib.load().gpurt_closest_point(self._h, _ptr(q), n, _ptr(prim), _ptr(pt),
+                                                   _ptr(dist), _ptr(bary), ctypes.byref(opts),
+                                                   _stream(dev)))
+        return prim, pt, dist, bary
+
+    def trace_closest(self, rays, with_uv=True, stack_depth=0, waves_per_cu=0, stats=None,
+                      out=None):
+        dev = self.device
+        _need(rays, torch.float32, (8,), "rays", dev)
+        n = rays.shape[0]
+        if out is None:
+            prim = torch.empty(n, dtype=torch.int32, device=dev)
+            t = torch.empty(n, dtype=torch.float32, device=dev)
+            uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
+        else:
+            prim, t, uv = out
+        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        _lib.check(_lib.load().gpurt_trace_closest(self._h, _ptr(rays), n, _ptr(prim), _ptr(t),
+                                                   _ptr(uv), ctypes.byref(opts), _stream(dev)))
+        return prim, t, uv
+
+    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None):
+        dev = self.device
+        _need(rays, torch.float32, (8,), "rays", dev)
+        n = rays.shape[0]
+        occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
+        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        _lib.check(_lib.load().gpurt_trace_any(self._h, _ptr(rays), n, _ptr(occ),
+                                               ctypes.byref(opts), _stream(dev)))
+        return occ
+
+    def closest_point_bwd(self, q, prim, bary, pt, dist, g_dist, g_q=None, g_verts=None):
+        _lib.check(_lib.load().gpurt_closest_point_bwd(
+            self._h, _ptr(q), q.shape[0], _ptr(prim), _ptr(bary), _ptr(pt), _ptr(dist),
+            _ptr(g_dist), _ptr(g_q), _ptr(g_verts), _stream(self.device)))
+
+    def trace_bwd(self, rays, prim, t, uv, g_t, g_origin=None, g_dir=None, g_verts=None):
+        _lib.check(_lib.load().gpurt_trace_bwd(
+            self._h, _ptr(rays), rays.shape[0], _ptr(prim), _ptr(t), _ptr(uv), _ptr(g_t),
+            _ptr(g_origin), _ptr(g_dir), _ptr(g_verts), _stream(self.device)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().gpurt_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ClosestPointDistance(torch.autograd.Function):
+    """d_i = min over triangles |q_i - p| ; backward via gpurt_closest_point_bwd."""
+
+    @staticmethod
+    def forward(ctx, bvh, q, verts):
+        q = q.contiguous()
+        prim, pt, dist, bary = bvh.closest_point(q)
+        ctx.bvh = bvh
+        ctx.n_verts = verts.shape[0]
+        ctx.save_for_backward(q, prim, pt, dist, bary)
+        ctx.mark_non_differentiable(prim)
+        return dist, prim
+
+    @staticmethod
+    def backward(ctx, g_dist, _g_prim):
+        q, prim, pt, dist, bary = ctx.saved_tensors
+        bvh = ctx.bvh
+        need_q, need_v = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+        g_dist = g_dist.contiguous().float()
+        g_q = torch.empty_like(q) if need_q else None
+        g_v = torch.zeros((ctx.n_verts, 3), dtype=torch.float32, device=q.device) if need_v else None
+        bvh.closest_point_bwd(q, prim, bary, pt, dist, g_dist, g_q, g_v)
+        return None, g_q, g_v
+
+
+class _RayHitT(torch.autograd.Function):
+    """t_i = closest-hit distance ; backward via gpurt_trace_bwd (misses give zero)."""
+
+    @staticmethod
+    def forward(ctx, bvh, o, d, verts, tmin, tmax):
+        rays = make_rays(o, d, tmin, tmax)
+        prim, t, uv = bvh.trace_closest(rays)
+        ctx.bvh = bvh
+        ctx.n_verts = verts.shape[0]
+        ctx.save_for_backward(rays, prim, t, uv)
+        ctx.mark_non_differentiable(prim)
+        return t, prim
+
+    @staticmethod
+    def backward(ctx, g_t, _g_prim):
+        rays, prim, t, uv = ctx.saved_tensors
+        bvh = ctx.bvh
+        need_o, need_d, need_v = ctx.needs_input_grad[1:4]
+        n = rays.shape[0]
+        g_t = g_t.contiguous().float()
+        g_o = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if need_o else None
+        g_d = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if need_d else None
+        g_v = torch.zeros((ctx.n_verts, 3), dtype=torch.float32, device=rays.device) if need_v else None
+        bvh.trace_bwd(rays, prim, t, uv, g_t, g_o, g_d, g_v)
+        return None, g_o, g_d, g_v, None, None
+
+
+def closest_point_distance(bvh, q, verts):
+    """Unsigned distance from each q to the mesh of `bvh` (built on `verts`), with
+    gradients w.r.t. q and verts.  Returns (dist, prim)."""
+    return _ClosestPointDistance.apply(bvh, q, verts)
+
+
+def ray_hit_t(bvh, o, d, verts, tmin=EPS, tmax=LARGE_DIST):
+    """Closest-hit distance of rays (o, d) against the mesh of `bvh` (built on `verts`),
+    with gradients w.r.t. o, d and verts.  Returns (t, prim); t = +inf on a miss."""
+    return _RayHitT.apply(bvh, o, d, verts, tmin, tmax)

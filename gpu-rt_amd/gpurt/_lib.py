@@ -1,0 +1,114 @@
+"""ctypes binding of libgpurt.so (the C ABI declared in include/gpurt.h).
+
+The library is loaded from this directory.  There is no fallback: if the HIP library
+is missing or cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpurt.so")
+
+GPURT_OK = 0
+GPURT_E_INVALID = -1
+GPURT_E_OOM = -2
+GPURT_E_STACK_OVERFLOW = -3
+GPURT_E_DEVICE = -4
+GPURT_E_UNSUPPORTED = -5
+
+_ERR_NAMES = {
+    GPURT_E_INVALID: "GPURT_E_INVALID",
+    GPURT_E_OOM: "GPURT_E_OOM",
+    GPURT_E_STACK_OVERFLOW: "GPURT_E_STACK_OVERFLOW",
+    GPURT_E_DEVICE: "GPURT_E_DEVICE",
+    GPURT_E_UNSUPPORTED: "GPURT_E_UNSUPPORTED",
+}
+
+# Every symbol include/gpurt.h declares.
+EXPORTS = (
+    "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_bvh_build", "gpurt_bvh_refit",
+    "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_closest_point", "gpurt_trace_closest",
+    "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd", "gpurt_batch_create",
+    "gpurt_batch_run", "gpurt_batch_destroy", "gpurt_last_error",
+)
+
+
+class GpurtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("node_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
+                ("treelet_size", ctypes.c_int32), ("treelet_rounds", ctypes.c_int32)]
+
+
+class QueryOpts(ctypes.Structure):
+    _fields_ = [("stack_depth", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("stats", ctypes.c_void_p)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("n_tris", ctypes.c_int64), ("n_verts", ctypes.c_int64),
+                ("n_nodes", ctypes.c_int64), ("node_bytes", ctypes.c_int64),
+                ("tri_bytes", ctypes.c_int64), ("depth", ctypes.c_int32),
+                ("node_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
+                ("sah_cost", ctypes.c_float)]
+
+
+class BatchMesh(ctypes.Structure):
+    _fields_ = [("verts", ctypes.c_void_p), ("vstride_bytes", ctypes.c_size_t),
+                ("n_verts", ctypes.c_size_t), ("tri_idx", ctypes.c_void_p),
+                ("n_tris", ctypes.c_size_t), ("q", ctypes.c_void_p), ("n_q", ctypes.c_size_t),
+                ("q_prim", ctypes.c_void_p), ("q_pt", ctypes.c_void_p),
+                ("q_dist", ctypes.c_void_p), ("q_bary", ctypes.c_void_p),
+                ("rays", ctypes.c_void_p), ("n_r", ctypes.c_size_t),
+                ("r_prim", ctypes.c_void_p), ("r_t", ctypes.c_void_p),
+                ("r_uv", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load libgpurt.so once; raises if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C gpu-rt_amd` "
+                          "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, S, V = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, None
+    sig = {
+        "gpurt_ctx_create": (I, [I, ctypes.POINTER(P)]),
+        "gpurt_ctx_destroy": (V, [P]),
+        "gpurt_bvh_build": (I, [P, P, S, S, P, S, ctypes.POINTER(BuildOpts), P, ctypes.POINTER(P)]),
+        "gpurt_bvh_refit": (I, [P, P, S, P]),
+        "gpurt_bvh_destroy": (V, [P]),
+        "gpurt_bvh_get_info": (I, [P, ctypes.POINTER(BvhInfo)]),
+        "gpurt_closest_point": (I, [P, P, S, P, P, P, P, ctypes.POINTER(QueryOpts), P]),
+        "gpurt_trace_closest": (I, [P, P, S, P, P, P, ctypes.POINTER(QueryOpts), P]),
+        "gpurt_trace_any": (I, [P, P, S, P, ctypes.POINTER(QueryOpts), P]),
+        "gpurt_closest_point_bwd": (I, [P, P, S, P, P, P, P, P, P, P, P]),
+        "gpurt_trace_bwd": (I, [P, P, S, P, P, P, P, P, P, P, P]),
+        "gpurt_batch_create": (I, [P, ctypes.POINTER(BatchMesh), S, ctypes.POINTER(BuildOpts),
+                                   ctypes.POINTER(QueryOpts), ctypes.POINTER(P)]),
+        "gpurt_batch_run": (I, [P, P]),
+        "gpurt_batch_destroy": (V, [P]),
+        "gpurt_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != GPURT_OK:
+        msg = load().gpurt_last_error()
+        raise GpurtError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,211 @@
+/*
+ * gpurt.h -- C ABI of the MI355X (gfx950) geometric-query library.
+ *
+ * Drop-in for the BVH build and traversal path of TheNumbat/GPU-RT:
+ *   - BVH build            replaces VK::Accel(const Mesh&) / Accel::recreate(const Mesh&)
+ *                          (src/vk/vulkan.h:256-284, src/vk/vulkan.cpp:881-936), which
+ *                          calls vkCmdBuildAccelerationStructuresKHR (vulkan.cpp:877).
+ *   - closest-hit rays     replaces trace_ray() -> traceRayEXT + rt.rchit / rt.rmiss
+ *                          (src/shaders/rt/rt.rgen:257-270, rt.rchit:11-16, rt.rmiss:9-11).
+ *   - any-hit rays         replaces visibility() (rt.rgen:272-291).
+ *   - closest-point query  the FCPW-style query the reference's README (README.md:6-8)
+ *                          points to; same BVH handle.
+ *   - backward passes      gradients of unsigned distance and hit distance t.
+ *
+ * Conventions
+ *   - Every array pointer is a caller-owned DEVICE pointer (hipMalloc or torch tensor
+ *     storage on the same device as the context).  BVH memory is library-owned.
+ *   - Calls are asynchronous on `stream` (a hipStream_t; NULL = the legacy default
+ *     stream) except gpurt_bvh_build / gpurt_bvh_refit, which wait for the build like
+ *     the reference's fence wait (vulkan.cpp:1205-1206) and return a complete handle.
+ *   - Status: 0 = OK, negative = error (see GPURT_E_*); nothing calls exit().  The
+ *     message of the last error on the calling thread is gpurt_last_error().
+ *   - A built BVH is immutable: concurrent queries from several threads/streams are
+ *     safe; a refit or destroy must not overlap queries on the same handle.
+ *   - Triangles are indexed uint32 triples; vertex positions are 3 floats at the start
+ *     of each vertex record of `vstride_bytes` bytes (12, 16, or 48 for the reference's
+ *     Mesh::Vertex {Vec4 pos, norm, tang}, src/vk/mesh.h:16-22).
+ *   - Primitive ids are the triangle index within its mesh (gl_PrimitiveID,
+ *     rt.rchit:15).  Ties (equal distance or equal t) go to the LOWEST id.
+ *   - Barycentrics of a ray hit are (u,v) with b = (1-u-v, u, v) (rt.rchit:13); the
+ *     closest-point query writes all three (b0,b1,b2) of the closest point.
+ */
+#ifndef GPURT_H
+#define GPURT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define GPURT_API __attribute__((visibility("default")))
+#else
+#define GPURT_API
+#endif
+
+#define GPURT_OK 0
+#define GPURT_E_INVALID (-1)        /* bad argument: null pointer, bad size, bad index   */
+#define GPURT_E_OOM (-2)            /* device allocation failed                          */
+#define GPURT_E_STACK_OVERFLOW (-3) /* traversal stack exhausted (results incomplete)    */
+#define GPURT_E_DEVICE (-4)         /* HIP runtime error                                 */
+#define GPURT_E_UNSUPPORTED (-5)    /* option value not supported                        */
+
+typedef struct gpurt_ctx_s* gpurt_ctx;
+typedef struct gpurt_bvh_s* gpurt_bvh;
+typedef struct gpurt_batch_s* gpurt_batch;
+
+/* One ray, 32 bytes: origin, tmin, direction, tmax.  A hit must have t in [tmin, tmax].
+ * The reference's rays use tmin = EPS = 1e-5 and tmax = LARGE_DIST = 1e7 for closest
+ * hits (rtcommon.glsl:6-7, rt.rgen:257-270) and tmax = |b-a| - EPS for visibility
+ * (rt.rgen:272-291). */
+typedef struct gpurt_ray {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} gpurt_ray;
+
+typedef struct gpurt_build_opts {
+    int32_t node_width;     /* children per node: 2, 4 or 8 (0 = 4)                  */
+    int32_t leaf_size;      /* max triangles per leaf, 1..8 (0 = 4)                   */
+    int32_t treelet_size;   /* leaves per treelet for SAH refinement, 0 = off, 5..7  */
+    int32_t treelet_rounds; /* refinement rounds (0 = 1 when treelet_size > 0)        */
+} gpurt_build_opts;
+
+typedef struct gpurt_query_opts {
+    int32_t stack_depth;  /* per-lane LDS stack entries: 16, 32 or 64 (0 = 32); deeper
+                             trees spill the rest to a device buffer                  */
+    int32_t waves_per_cu; /* resident waves per CU for the launch, 0 = library choice  */
+    uint64_t* stats;      /* optional DEVICE uint64[2], accumulated: nodes visited,
+                             triangles tested (slower counting kernel variant)        */
+} gpurt_query_opts;
+
+typedef struct gpurt_bvh_info {
+    int64_t n_tris;
+    int64_t n_verts;
+    int64_t n_nodes;    /* wide nodes                                                  */
+    int64_t node_bytes; /* bytes per wide node                                         */
+    int64_t tri_bytes;  /* bytes per packed triangle                                   */
+    int32_t depth;      /* wide-node levels                                            */
+    int32_t node_width;
+    int32_t leaf_size;
+    float sah_cost;     /* SAH cost of the wide tree, relative to the root area        */
+} gpurt_bvh_info;
+
+/* ---- context --------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  Replaces the ray-tracing part of
+ * VK::Manager::init / init_rt (src/vk/vulkan.cpp:1136-1160, 1326-1385). */
+GPURT_API int gpurt_ctx_create(int device, gpurt_ctx* out);
+GPURT_API void gpurt_ctx_destroy(gpurt_ctx ctx);
+
+/* ---- build ----------------------------------------------------------------------- */
+
+/* Build a BVH over one triangle mesh: 30-bit Morton LBVH (Karras 2012) -> optional SAH
+ * treelet refinement -> collapse to `node_width`-wide nodes -> packed node and triangle
+ * arrays.  Replaces VK::Accel::Accel(const Mesh&) / Accel::recreate(const Mesh&)
+ * (src/vk/vulkan.h:258,266; src/vk/vulkan.cpp:881-936: R32G32B32_SFLOAT at
+ * vertexStride, UINT32 indices, primitiveCount = n_idx/3, opaque geometry). */
+GPURT_API int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, size_t n_verts,
+                    const uint32_t* tri_idx, size_t n_tris, const gpurt_build_opts* opts,
+                    void* stream, gpurt_bvh* out);
+
+/* Update the boxes of a built BVH for moved vertices (same count, same topology).  The
+ * reference only rebuilds (mode BUILD, vulkan.cpp:832,915); this is the cheaper update
+ * for optimisation loops that move vertices every step. */
+GPURT_API int gpurt_bvh_refit(gpurt_bvh bvh, const float* verts, size_t vstride_bytes, void* stream);
+
+/* Free a BVH (waits for work queued on its build stream).  Replaces Accel::destroy
+ * (vulkan.cpp:768-776). */
+GPURT_API void gpurt_bvh_destroy(gpurt_bvh bvh);
+
+GPURT_API int gpurt_bvh_get_info(gpurt_bvh bvh, gpurt_bvh_info* out);
+
+/* ---- queries ---------------------------------------------------------------------- */
+
+/* Closest point on the mesh for n query points q (n x 3 floats).  Outputs: prim (int32,
+ * -1 only for an empty mesh), pt (n x 3 closest point), dist (unsigned distance), bary
+ * (n x 3 barycentrics of pt, nullable).  No counterpart in the mounted reference (its
+ * closest-point work is on a branch not included, README.md:6-8); it uses the same
+ * handle as the ray queries. */
+GPURT_API int gpurt_closest_point(gpurt_bvh bvh, const float* q, size_t n, int32_t* prim, float* pt,
+                        float* dist, float* bary, const gpurt_query_opts* opts, void* stream);
+
+/* Closest hit for n rays.  prim = hit triangle (-1 on miss, rt.rmiss:9-11), t = hit
+ * distance (+inf on miss), uv = (u,v) (n x 2, nullable; 0 on miss).  Replaces
+ * trace_ray()'s traceRayEXT(TLAS, gl_RayFlagsOpaqueEXT, 0xFF, ...) with the rt.rchit
+ * payload (rt.rgen:257-270, rt.rchit:11-16, rtcommon.glsl:55-60). */
+GPURT_API int gpurt_trace_closest(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, int32_t* prim,
+                        float* t, float* uv, const gpurt_query_opts* opts, void* stream);
+
+/* Any hit (occlusion) for n rays: occluded[i] = 1 iff some triangle is hit with t in
+ * [tmin, tmax].  Replaces visibility()'s traceRayEXT with TerminateOnFirstHit |
+ * Opaque | SkipClosestHitShader and the payload.hit preset/clear
+ * (rt.rgen:272-291, rt.rmiss:9-11). */
+GPURT_API int gpurt_trace_any(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, uint8_t* occluded,
+                    const gpurt_query_opts* opts, void* stream);
+
+/* ---- backward --------------------------------------------------------------------- */
+
+/* Gradient of sum_i g_dist[i] * dist_i.  Uses the forward pass's saved prim and bary
+ * (no re-traversal): g_q[i] = g_dist[i] (q_i - pt_i)/dist_i; for the three vertices v_j
+ * of prim_i, g_verts[v_j] -= b_j g_q[i]  (FP32 atomic scatter-add: g_verts is
+ * ACCUMULATED, n_verts x 3, zero it first).  g_q and g_verts are each nullable. */
+GPURT_API int gpurt_closest_point_bwd(gpurt_bvh bvh, const float* q, size_t n, const int32_t* prim,
+                            const float* bary, const float* pt, const float* dist,
+                            const float* g_dist, float* g_q, float* g_verts, void* stream);
+
+/* Gradient of sum_i g_t[i] * t_i for closest hits.  With n = (v1-v0) x (v2-v0):
+ * dt/do = -n/(n.d), dt/dd = -t n/(n.d), dt/dv_j = b_j n/(n.d), b = (1-u-v,u,v).
+ * g_origin / g_dir are n x 3 (written, nullable); g_verts is ACCUMULATED by atomic
+ * scatter-add (nullable).  Misses contribute zero. */
+GPURT_API int gpurt_trace_bwd(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, const int32_t* prim,
+                    const float* t, const float* uv, const float* g_t, float* g_origin,
+                    float* g_dir, float* g_verts, void* stream);
+
+/* ---- batched ---------------------------------------------------------------------- */
+
+/* One mesh of a batch, with its own query points and rays (all device pointers). */
+typedef struct gpurt_batch_mesh {
+    const float* verts;
+    size_t vstride_bytes;
+    size_t n_verts;
+    const uint32_t* tri_idx;
+    size_t n_tris;
+    const float* q;       /* n_q x 3 closest-point queries (nullable if n_q == 0) */
+    size_t n_q;
+    int32_t* q_prim;      /* outputs of the closest-point query                   */
+    float* q_pt;
+    float* q_dist;
+    float* q_bary;        /* nullable                                             */
+    const gpurt_ray* rays; /* n_r rays, closest hit                               */
+    size_t n_r;
+    int32_t* r_prim;
+    float* r_t;
+    float* r_uv;          /* nullable                                             */
+} gpurt_batch_mesh;
+
+/* Prepare a batch over n_meshes meshes: validates, allocates every buffer the build and
+ * query stages need, and records them (BVH builds of all meshes in shared launches, then
+ * one persistent closest-point + closest-hit launch fed by a device work queue) into a
+ * HIP graph.  The mesh and query arrays must stay valid while the batch is used. */
+GPURT_API int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_meshes,
+                       const gpurt_build_opts* bopts, const gpurt_query_opts* qopts,
+                       gpurt_batch* out);
+/* Run the batch (graph launch on `stream`): rebuild every BVH and answer every query. */
+GPURT_API int gpurt_batch_run(gpurt_batch batch, void* stream);
+GPURT_API void gpurt_batch_destroy(gpurt_batch batch);
+
+/* ---- errors ----------------------------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none). */
+GPURT_API const char* gpurt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPURT_H */

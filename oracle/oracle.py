@@ -1,0 +1,237 @@
+"""numpy/ctypes wrapper of liboracle.so (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker or the timed CPU port -- never by the product
+package.  See oracle.c for what each function restates and which reference lines it
+follows.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    I64, I, P = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
+    opt = lambda t: ctypes.c_void_p  # nullable arrays are passed as raw pointers
+    lib.orc_set_threads.argtypes = [I]
+    lib.orc_num_threads.restype = I
+    lib.orc_cp_tri_f32.argtypes = [_f32p] * 4 + [_f32p, _f32p]
+    lib.orc_cp_tri_f32.restype = ctypes.c_float
+    lib.orc_mt_f32.argtypes = [_f32p] * 5 + [ctypes.POINTER(ctypes.c_float)] * 3
+    lib.orc_mt_f32.restype = I
+    lib.orc_cp_tri_f64.argtypes = [_f64p] * 4 + [_f64p, _f64p]
+    lib.orc_cp_tri_f64.restype = ctypes.c_double
+    lib.orc_ray_tri_f64.argtypes = [_f64p] * 5 + [ctypes.POINTER(ctypes.c_double)] * 3
+    lib.orc_ray_tri_f64.restype = I
+    lib.orc_cpq_brute_f32.argtypes = [_f32p, _u32p, I64, _f32p, I64, _i32p, P, P, P]
+    lib.orc_cpq_brute_f64.argtypes = [_f64p, _u32p, I64, _f64p, I64, _i32p, P, P, P]
+    lib.orc_ray_closest_brute_f32.argtypes = [_f32p, _u32p, I64, _f32p, I64, _i32p, _f32p, P]
+    lib.orc_ray_any_brute_f32.argtypes = [_f32p, _u32p, I64, _f32p, I64, _u8p]
+    lib.orc_ray_closest_brute_f64.argtypes = [_f64p, _u32p, I64, _f64p, I64, _i32p, _f64p, P]
+    lib.orc_cpq_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P]
+    lib.orc_ray_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P, P]
+    lib.orc_morton_codes.argtypes = [_f32p, I64, _f32p, _f32p, _u32p]
+    lib.orc_lbvh_build.argtypes = [_f32p, I64, _u32p, I64, I]
+    lib.orc_lbvh_build.restype = P
+    lib.orc_lbvh_free.argtypes = [P]
+    lib.orc_lbvh_info.argtypes = [P, _i64p]
+    lib.orc_lbvh_order.argtypes = [P, _i32p]
+    lib.orc_lbvh_cpq.argtypes = [P, _f32p, I64, _i32p, P, P, P, P]
+    lib.orc_lbvh_ray_closest.argtypes = [P, _f32p, I64, _i32p, _f32p, P, P]
+    lib.orc_lbvh_ray_any.argtypes = [P, _f32p, I64, _u8p, P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def set_threads(n):
+    load().orc_set_threads(int(n))
+
+
+def num_threads():
+    return load().orc_num_threads()
+
+
+# ---- per-primitive -------------------------------------------------------------------
+
+def cp_tri_f32(p, a, b, c):
+    cp = np.zeros(3, np.float32); br = np.zeros(3, np.float32)
+    d2 = load().orc_cp_tri_f32(_c(p, np.float32), _c(a, np.float32), _c(b, np.float32),
+                               _c(c, np.float32), cp, br)
+    return np.float32(d2), cp, br
+
+
+def cp_tri_f64(p, a, b, c):
+    cp = np.zeros(3); br = np.zeros(3)
+    d2 = load().orc_cp_tri_f64(_c(p, np.float64), _c(a, np.float64), _c(b, np.float64),
+                               _c(c, np.float64), cp, br)
+    return d2, cp, br
+
+
+def mt_f32(o, d, a, b, c):
+    t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    ok = load().orc_mt_f32(_c(o, np.float32), _c(d, np.float32), _c(a, np.float32),
+                           _c(b, np.float32), _c(c, np.float32), ctypes.byref(t),
+                           ctypes.byref(u), ctypes.byref(v))
+    return bool(ok), np.float32(t.value), np.float32(u.value), np.float32(v.value)
+
+
+def ray_tri_f64(o, d, a, b, c):
+    t, u, v = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    ok = load().orc_ray_tri_f64(_c(o, np.float64), _c(d, np.float64), _c(a, np.float64),
+                                _c(b, np.float64), _c(c, np.float64), ctypes.byref(t),
+                                ctypes.byref(u), ctypes.byref(v))
+    return bool(ok), t.value, u.value, v.value
+
+
+# ---- brute force -----------------------------------------------------------------------
+
+def cpq_brute_f32(V, F, Q):
+    V, F, Q = _c(V, np.float32), _c(F, np.uint32), _c(Q, np.float32)
+    n = Q.shape[0]
+    prim = np.empty(n, np.int32); pt = np.empty((n, 3), np.float32)
+    dist = np.empty(n, np.float32); bary = np.empty((n, 3), np.float32)
+    load().orc_cpq_brute_f32(V, F, F.shape[0], Q, n, prim, _p(pt), _p(dist), _p(bary))
+    return prim, pt, dist, bary
+
+
+def cpq_brute_f64(V, F, Q):
+    V, F, Q = _c(V, np.float64), _c(F, np.uint32), _c(Q, np.float64)
+    n = Q.shape[0]
+    prim = np.empty(n, np.int32); pt = np.empty((n, 3)); dist = np.empty(n)
+    bary = np.empty((n, 3))
+    load().orc_cpq_brute_f64(V, F, F.shape[0], Q, n, prim, _p(pt), _p(dist), _p(bary))
+    return prim, pt, dist, bary
+
+
+def ray_closest_brute_f32(V, F, R):
+    V, F, R = _c(V, np.float32), _c(F, np.uint32), _c(R, np.float32)
+    n = R.shape[0]
+    prim = np.empty(n, np.int32); t = np.empty(n, np.float32); uv = np.empty((n, 2), np.float32)
+    load().orc_ray_closest_brute_f32(V, F, F.shape[0], R, n, prim, t, _p(uv))
+    return prim, t, uv
+
+
+def ray_any_brute_f32(V, F, R):
+    V, F, R = _c(V, np.float32), _c(F, np.uint32), _c(R, np.float32)
+    occ = np.empty(R.shape[0], np.uint8)
+    load().orc_ray_any_brute_f32(V, F, F.shape[0], R, R.shape[0], occ)
+    return occ
+
+
+def ray_closest_brute_f64(V, F, R):
+    V, F, R = _c(V, np.float64), _c(F, np.uint32), _c(R, np.float64)
+    n = R.shape[0]
+    prim = np.empty(n, np.int32); t = np.empty(n); uv = np.empty((n, 2))
+    load().orc_ray_closest_brute_f64(V, F, F.shape[0], R, n, prim, t, _p(uv))
+    return prim, t, uv
+
+
+# ---- gradients (FP64 closed forms) ---------------------------------------------------
+
+def cpq_grad_f64(V, F, Q, prim, pt, bary, g=None):
+    V, F, Q = _c(V, np.float64), _c(F, np.uint32), _c(Q, np.float64)
+    prim = _c(prim, np.int32); pt = _c(pt, np.float64); bary = _c(bary, np.float64)
+    g = _c(g, np.float64) if g is not None else None
+    n = Q.shape[0]
+    gq = np.zeros((n, 3)); gv = np.zeros((V.shape[0], 3))
+    load().orc_cpq_grad_f64(V, F, Q, n, prim, pt, bary, _p(g), _p(gq), _p(gv))
+    return gq, gv
+
+
+def ray_grad_f64(V, F, R, prim, t, uv, g=None):
+    V, F, R = _c(V, np.float64), _c(F, np.uint32), _c(R, np.float64)
+    prim = _c(prim, np.int32); t = _c(t, np.float64); uv = _c(uv, np.float64)
+    g = _c(g, np.float64) if g is not None else None
+    n = R.shape[0]
+    go = np.zeros((n, 3)); gd = np.zeros((n, 3)); gv = np.zeros((V.shape[0], 3))
+    load().orc_ray_grad_f64(V, F, R, n, prim, t, uv, _p(g), _p(go), _p(gd), _p(gv))
+    return go, gd, gv
+
+
+# ---- CPU LBVH ----------------------------------------------------------------------------
+
+def morton_codes(centroids, lo, hi):
+    c = _c(centroids, np.float32)
+    out = np.empty(c.shape[0], np.uint32)
+    load().orc_morton_codes(c, c.shape[0], _c(lo, np.float32), _c(hi, np.float32), out)
+    return out
+
+
+class CpuBvh:
+    """The oracle's CPU LBVH (Karras 2012, leaves <= leaf_size) -- the CPU baseline."""
+
+    def __init__(self, V, F, leaf_size=4):
+        self.V = _c(V, np.float32)
+        self.F = _c(F, np.uint32)
+        self.h = load().orc_lbvh_build(self.V, self.V.shape[0], self.F, self.F.shape[0],
+                                       int(leaf_size))
+        if not self.h:
+            raise ValueError("oracle LBVH build failed (bad input)")
+
+    def info(self):
+        out = np.zeros(5, np.int64)
+        load().orc_lbvh_info(self.h, out)
+        return {"n_nodes": int(out[0]), "depth": int(out[1]), "max_leaf": int(out[2]),
+                "n_leaves": int(out[3]), "bad": int(out[4])}
+
+    def order(self):
+        out = np.empty(self.F.shape[0], np.int32)
+        load().orc_lbvh_order(self.h, out)
+        return out
+
+    def closest_point(self, Q, counters=None):
+        Q = _c(Q, np.float32)
+        n = Q.shape[0]
+        prim = np.empty(n, np.int32); pt = np.empty((n, 3), np.float32)
+        dist = np.empty(n, np.float32); bary = np.empty((n, 3), np.float32)
+        load().orc_lbvh_cpq(self.h, Q, n, prim, _p(pt), _p(dist), _p(bary), _p(counters))
+        return prim, pt, dist, bary
+
+    def trace_closest(self, R, counters=None):
+        R = _c(R, np.float32)
+        n = R.shape[0]
+        prim = np.empty(n, np.int32); t = np.empty(n, np.float32); uv = np.empty((n, 2), np.float32)
+        load().orc_lbvh_ray_closest(self.h, R, n, prim, t, _p(uv), _p(counters))
+        return prim, t, uv
+
+    def trace_any(self, R, counters=None):
+        R = _c(R, np.float32)
+        occ = np.empty(R.shape[0], np.uint8)
+        load().orc_lbvh_ray_any(self.h, R, R.shape[0], occ, _p(counters))
+        return occ
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_lbvh_free(self.h)
+            self.h = None

@@ -1,0 +1,204 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle on the same inputs.
+
+Integer outputs (primitive ids, occlusion) and every FP32 output of the forward queries
+are compared BIT-EXACTLY with the oracle's FP32 brute force: the kernels and the oracle
+perform the same rounded operations, and the traversal never culls a subtree that may
+hold an equal-or-better primitive, so the answer cannot depend on the tree.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gpurt
+from gpurt import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _np(*ts):
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() if t is not None else None for t in ts]
+
+
+def _cpq_check(ctx, dev, V, F, Q, **kw):
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **{k: v for k, v in kw.items() if k in
+                                                      ("node_width", "leaf_size")})
+    qkw = {k: v for k, v in kw.items() if k in ("stack_depth", "waves_per_cu")}
+    prim, pt, dist, bary = _np(*bvh.closest_point(_t(Q, dev), **qkw))
+    ep, ept, ed, eb = O.cpq_brute_f32(V, F, Q)
+    assert np.array_equal(prim, ep), f"{(prim != ep).sum()} ids differ"
+    assert np.array_equal(dist, ed)
+    assert np.array_equal(pt, ept)
+    assert np.array_equal(bary, eb)
+    return bvh
+
+
+def test_cpq_small_full_workload_bitexact(ctx, dev):
+    w = W.cpq_small()
+    print("cpq_small", w["meta"], flush=True)
+    bvh = _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"])
+    info = bvh.info()
+    assert info["n_tris"] == 5120 and info["leaf_size"] == 4 and info["node_width"] == 4
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+@pytest.mark.parametrize("stack", [16, 32, 64])
+def test_cpq_widths_and_stacks(ctx, dev, width, stack):
+    w = W.cpq_small(n_queries=4096)
+    print("cpq width", width, "stack", stack, flush=True)
+    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], node_width=width, stack_depth=stack)
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 8])
+def test_cpq_leaf_sizes(ctx, dev, leaf):
+    w = W.cpq_small(n_queries=4096)
+    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], leaf_size=leaf)
+
+
+def _ray_check(ctx, dev, V, F, R, RA=None, **kw):
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **{k: v for k, v in kw.items() if k in
+                                                      ("node_width", "leaf_size")})
+    qkw = {k: v for k, v in kw.items() if k in ("stack_depth",)}
+    prim, t, uv = _np(*bvh.trace_closest(_t(R, dev), **qkw))
+    ep, et, euv = O.ray_closest_brute_f32(V, F, R)
+    assert np.array_equal(prim, ep), f"{(prim != ep).sum()} ids differ"
+    assert np.array_equal(t, et)
+    assert np.array_equal(uv, euv)
+    RA = R if RA is None else RA
+    occ, = _np(bvh.trace_any(_t(RA, dev), **qkw))
+    assert np.array_equal(occ, O.ray_any_brute_f32(V, F, RA))
+    return prim
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_rays_small_scene_bitexact(ctx, dev, width):
+    w = W.rays_scene(n_rays=8192, grid=101, n_spheres=6)
+    print("rays", w["meta"], "width", width, flush=True)
+    prim = _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], node_width=width)
+    assert 0.2 < (prim >= 0).mean() < 0.9
+
+
+@pytest.mark.parametrize("stride", [3, 4, 12])
+def test_vertex_strides(ctx, dev, stride):
+    """12-, 16- and 48-byte vertex records (48 = the reference's Mesh::Vertex)."""
+    w = W.cpq_small(n_queries=2048)
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    Vs = np.zeros((V.shape[0], stride), np.float32)
+    Vs[:, :3] = V
+    Vs[:, 3:] = 7.0  # normals/tangents/uv in the other columns must be ignored
+    bvh = gpurt.Bvh(ctx, _t(Vs, dev), _t(F, dev))
+    prim, dist = _np(*bvh.closest_point(_t(Q, dev))[:3:2])
+    ep, _, ed, _ = O.cpq_brute_f32(V, F, Q)
+    assert np.array_equal(prim, ep) and np.array_equal(dist, ed)
+
+
+@pytest.mark.parametrize("n_tris", [1, 2, 3, 5, 17])
+def test_tiny_meshes(ctx, dev, n_tris):
+    rng = np.random.default_rng(n_tris)
+    V = rng.normal(size=(3 * n_tris, 3)).astype(np.float32)
+    F = np.arange(3 * n_tris, dtype=np.int32).reshape(-1, 3)
+    Q = rng.normal(size=(777, 3)).astype(np.float32)
+    _cpq_check(ctx, dev, V, F, Q)
+    o = rng.normal(size=(777, 3)) * 3
+    d = W.random_dirs(rng, 777)
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7))
+
+
+def test_exact_ties_go_to_lowest_index(ctx, dev):
+    a, b, c = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    V = np.array([a, b, c], np.float32)
+    F = np.array([[2, 0, 1], [0, 1, 2], [1, 2, 0]] * 5, np.int32)  # 15 copies of one triangle
+    Q = np.array([[0.25, 0.25, 1], [-1, -1, 0], [0.5, 0.5, 0], [3, 3, 3]], np.float32)
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), leaf_size=2)
+    prim, = _np(bvh.closest_point(_t(Q, dev))[0])
+    assert prim.tolist() == [0, 0, 0, 0]
+    R = W.pack_rays(np.array([[0.2, 0.3, 1.0], [0.2, 0.3, -1.0]]),
+                    np.array([[0, 0, -1.0], [0, 0, 1.0]]), 0.0, 10.0)
+    p, t, _ = _np(*bvh.trace_closest(_t(R, dev)))
+    assert p.tolist() == [0, 0] and t.tolist() == [1.0, 1.0]
+
+
+def test_shared_vertices_and_edges_of_a_closed_mesh(ctx, dev):
+    """Queries exactly at vertices and on edges of an icosphere: exact FP32 ties
+    between neighbouring triangles resolve to the lowest index."""
+    v, f = W.icosphere(3)
+    V, F = v.astype(np.float32), f.astype(np.int32)
+    mid = ((V[F[:, 0]] + V[F[:, 1]]) * np.float32(0.5)).astype(np.float32)
+    Q = np.concatenate([V, mid, V * np.float32(1.5), mid * np.float32(0.5)]).astype(np.float32)
+    _cpq_check(ctx, dev, V, F, Q)
+    # rays aimed at vertices and edge midpoints from outside
+    o = np.concatenate([V, mid]) * 3.0
+    d = -o / np.linalg.norm(o, axis=1, keepdims=True)
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7))
+
+
+def test_degenerate_triangles(ctx, dev):
+    rng = np.random.default_rng(5)
+    w = W.cpq_small(n_queries=3000)
+    V, F = w["verts"], w["tris"].copy()
+    k = rng.choice(F.shape[0], 300, replace=False)
+    F[k[:100], 1] = F[k[:100], 0]            # two equal vertices
+    F[k[100:200], 2] = F[k[100:200], 0]
+    F[k[200:], 1] = F[k[200:], 0]
+    F[k[200:], 2] = F[k[200:], 0]            # a point triangle
+    _cpq_check(ctx, dev, V, F, w["queries"])
+    o = rng.uniform(-1.5, 1.5, (3000, 3))
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, W.random_dirs(rng, 3000), 1e-5, 1e7))
+
+
+def test_ray_interval_bounds(ctx, dev):
+    """tmin/tmax are inclusive (rt.rgen:257-291): hits at exactly tmax count."""
+    w = W.rays_scene(n_rays=4096, grid=61, n_spheres=3)
+    V, F, R = w["verts"], w["tris"], w["rays"]
+    ep, et, _ = O.ray_closest_brute_f32(V, F, R)
+    hit = ep >= 0
+    R2 = R[hit].copy()
+    R2[:, 7] = et[hit]                   # tmax == t: still a hit
+    R3 = R[hit].copy()
+    R3[:, 7] = np.nextafter(et[hit], np.float32(-np.inf))  # just below: the hit is excluded
+    for RR in (R2, R3):
+        _ray_check(ctx, dev, V, F, RR)
+
+
+def test_out_of_range_index_is_rejected(ctx, dev):
+    V = np.zeros((3, 3), np.float32)
+    F = np.array([[0, 1, 3]], np.int32)
+    with pytest.raises(gpurt.GpurtError) as e:
+        gpurt.Bvh(ctx, _t(V, dev), _t(F, dev))
+    assert e.value.code == -1
+
+
+def test_bad_options_are_rejected(ctx, dev):
+    w = W.cpq_small(n_queries=16)
+    with pytest.raises(gpurt.GpurtError):
+        gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev), node_width=3)
+    bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
+    with pytest.raises(gpurt.GpurtError):
+        bvh.closest_point(_t(w["queries"], dev), stack_depth=24)
+
+
+def test_empty_query_sets(ctx, dev):
+    w = W.cpq_small(n_queries=16)
+    bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
+    prim, pt, dist, bary = bvh.closest_point(torch.empty((0, 3), device=dev))
+    assert prim.numel() == 0
+    p, t, uv = bvh.trace_closest(torch.empty((0, 8), device=dev))
+    assert p.numel() == 0
+
+
+def test_stats_count_visits(ctx, dev):
+    w = W.cpq_small(n_queries=4096)
+    bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
+    st = torch.zeros(2, dtype=torch.int64, device=dev)
+    prim, = _np(bvh.closest_point(_t(w["queries"], dev), stats=st)[0])
+    nodes, tris = st.cpu().tolist()
+    assert nodes >= 4096 and tris >= 4096
+    ep, _, _, _ = O.cpq_brute_f32(w["verts"], w["tris"], w["queries"])
+    assert np.array_equal(prim, ep)
+    # the BVH prunes: far fewer triangle tests than brute force
+    assert tris < 0.05 * 4096 * 5120

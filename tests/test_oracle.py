@@ -1,0 +1,284 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+The reference has no tests, fixtures or CPU BVH (its BVH lives in the Vulkan driver),
+so the oracle is pinned by: known answers a plausible mistake breaks (wrong region,
+swapped barycentric, sign error, dropped term), an independent FP64 formulation of
+each query, brute force over all triangles, sortedness/permutation invariants of the
+LBVH, and FP64 central differences for the gradients.
+"""
+import numpy as np
+import pytest
+
+from gpurt import workloads as W
+from oracle import oracle as O
+
+TRI = (np.array([0, 0, 0.]), np.array([1, 0, 0.]), np.array([0, 1, 0.]))
+
+# (query, closest point, squared distance, barycentrics) on the unit right triangle
+CP_KAT = [
+    ((0.25, 0.25, 1.0), (0.25, 0.25, 0.0), 1.0, (0.5, 0.25, 0.25)),   # face interior
+    ((-1.0, -1.0, 0.0), (0.0, 0.0, 0.0), 2.0, (1.0, 0.0, 0.0)),       # vertex a
+    ((2.0, -1.0, 0.0), (1.0, 0.0, 0.0), 2.0, (0.0, 1.0, 0.0)),        # vertex b
+    ((-1.0, 2.0, 0.0), (0.0, 1.0, 0.0), 2.0, (0.0, 0.0, 1.0)),        # vertex c
+    ((0.5, -1.0, 0.0), (0.5, 0.0, 0.0), 1.0, (0.5, 0.5, 0.0)),        # edge ab
+    ((-1.0, 0.5, 0.0), (0.0, 0.5, 0.0), 1.0, (0.5, 0.0, 0.5)),        # edge ac
+    ((1.0, 1.0, 0.0), (0.5, 0.5, 0.0), 0.5, (0.0, 0.5, 0.5)),         # edge bc
+    ((0.25, 0.25, -2.0), (0.25, 0.25, 0.0), 4.0, (0.5, 0.25, 0.25)),  # other side
+    ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.0, (1.0, 0.0, 0.0)),         # on a vertex
+]
+
+
+@pytest.mark.parametrize("p,cp,d2,bary", CP_KAT)
+def test_closest_point_known_answers(p, cp, d2, bary):
+    a, b, c = TRI
+    for fn in (O.cp_tri_f32, O.cp_tri_f64):
+        got_d2, got_cp, got_b = fn(np.array(p), a, b, c)
+        assert got_d2 == pytest.approx(d2, abs=1e-7)
+        np.testing.assert_allclose(got_cp, cp, atol=1e-7)
+        np.testing.assert_allclose(got_b, bary, atol=1e-7)
+        # barycentrics reproduce the closest point
+        np.testing.assert_allclose(got_b[0] * a + got_b[1] * b + got_b[2] * c, cp, atol=1e-6)
+
+
+def test_closest_point_f32_vs_f64_and_dense_sampling():
+    rng = np.random.default_rng(123)
+    for _ in range(400):
+        a, b, c = rng.normal(size=(3, 3))
+        p = rng.normal(size=3) * 2
+        d2f, cpf, bf = O.cp_tri_f32(p, a, b, c)
+        a32, b32, c32, p32 = (x.astype(np.float32).astype(np.float64) for x in (a, b, c, p))
+        d2d, cpd, bd = O.cp_tri_f64(p32, a32, b32, c32)
+        assert abs(np.sqrt(d2f) - np.sqrt(d2d)) <= 2e-6 * (1 + np.abs(p32).max())
+        # brute force over a dense barycentric grid never beats the closed form
+        s = np.linspace(0, 1, 61)
+        u, v = np.meshgrid(s, s)
+        m = u + v <= 1
+        pts = (1 - u[m] - v[m])[:, None] * a32 + u[m][:, None] * b32 + v[m][:, None] * c32
+        dmin = np.min(np.sum((pts - p32) ** 2, axis=1))
+        assert d2d <= dmin + 1e-12
+        assert dmin - d2d <= 0.05 * (1 + d2d)
+
+
+MT_KAT = [
+    # origin, direction, hit, t, u, v   (u weights pb, v weights pc: rt.rchit:13)
+    ((0.2, 0.3, 1.0), (0, 0, -1.0), True, 1.0, 0.2, 0.3),
+    ((0.2, 0.3, -1.0), (0, 0, 1.0), True, 1.0, 0.2, 0.3),   # back face: no culling
+    ((0.8, 0.8, 1.0), (0, 0, -1.0), False, 0, 0, 0),        # u + v > 1
+    ((-0.1, 0.3, 1.0), (0, 0, -1.0), False, 0, 0, 0),       # u < 0
+    ((0.0, 0.0, 1.0), (1.0, 0, 0), False, 0, 0, 0),         # parallel: det == 0
+    ((0.5, 0.0, 1.0), (0, 0, -1.0), True, 1.0, 0.5, 0.0),   # on edge ab (closed)
+    ((0.2, 0.3, 3.0), (0, 0, -2.0), True, 1.5, 0.2, 0.3),   # t in units of |d|
+    ((0.2, 0.3, 1.0), (0, 0, 1.0), True, -1.0, 0.2, 0.3),   # behind: t < 0 reported
+]
+
+
+@pytest.mark.parametrize("o,d,hit,t,u,v", MT_KAT)
+def test_ray_triangle_known_answers(o, d, hit, t, u, v):
+    a, b, c = TRI
+    for fn in (O.mt_f32, O.ray_tri_f64):
+        ok, gt, gu, gv = fn(np.array(o), np.array(d), a, b, c)
+        assert ok == hit
+        if hit:
+            assert gt == pytest.approx(t, abs=1e-6)
+            assert gu == pytest.approx(u, abs=1e-6)
+            assert gv == pytest.approx(v, abs=1e-6)
+
+
+def test_ray_interval_is_closed():
+    a, b, c = TRI
+    V = np.stack([a, b, c]).astype(np.float32)
+    F = np.array([[0, 1, 2]], np.uint32)
+    o, d = np.array([0.2, 0.3, 1.0]), np.array([0, 0, -1.0])
+    for tmin, tmax, expect in [(0.0, 1.0, 1), (1.0, 2.0, 1), (0.0, np.nextafter(np.float32(1), np.float32(0)), 0),
+                               (np.nextafter(np.float32(1), np.float32(2)), 5.0, 0)]:
+        R = W.pack_rays(o[None], d[None], tmin, tmax)
+        assert O.ray_any_brute_f32(V, F, R)[0] == expect
+        prim, t, _ = O.ray_closest_brute_f32(V, F, R)
+        assert (prim[0] == 0) == bool(expect)
+        if expect:
+            assert t[0] == 1.0
+
+
+def test_lowest_index_wins_exact_ties():
+    # the same triangle three times: every query and ray must report index 0
+    a, b, c = TRI
+    V = np.stack([a, b, c]).astype(np.float32)
+    F = np.array([[0, 1, 2], [1, 2, 0], [2, 0, 1]], np.uint32)
+    Q = np.array([[0.25, 0.25, 1.0], [-1, -1, 0]], np.float32)
+    prim, _, _, _ = O.cpq_brute_f32(V, F, Q)
+    assert prim.tolist() == [0, 0]
+    bvh = O.CpuBvh(V, F, 1)
+    assert bvh.closest_point(Q)[0].tolist() == [0, 0]
+    R = W.pack_rays(np.array([[0.2, 0.3, 1.0]]), np.array([[0, 0, -1.0]]), 0.0, 10.0)
+    assert O.ray_closest_brute_f32(V, F, R)[0].tolist() == [0]
+    assert bvh.trace_closest(R)[0].tolist() == [0]
+
+
+def _scale(V):
+    return float(np.abs(V).max())
+
+
+def test_cpq_brute_f32_vs_f64():
+    w = W.cpq_small(n_queries=2048)
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    p32, pt32, d32, b32 = O.cpq_brute_f32(V, F, Q)
+    p64, pt64, d64, b64 = O.cpq_brute_f64(V, F, Q)
+    s = _scale(V)
+    assert np.max(np.abs(d32 - d64)) <= 1e-6 * s
+    same = p32 == p64
+    assert np.max(np.abs(pt32[same] - pt64[same])) <= 1e-6 * s
+    # where the ids differ the two triangles are an FP32 near-tie in FP64 too, and the
+    # FP32 point is the FP64 closest point of the triangle FP32 chose
+    diff = np.nonzero(~same)[0]
+    assert diff.size < 0.1 * len(Q)
+    Fd = F.astype(np.int64)
+    V64 = V.astype(np.float64)
+    for i in diff:
+        tri = Fd[p32[i]]
+        d2, cp, _ = O.cp_tri_f64(Q[i].astype(np.float64), V64[tri[0]], V64[tri[1]], V64[tri[2]])
+        assert np.sqrt(d2) - d64[i] <= 1e-6 * s
+        assert np.max(np.abs(cp - pt32[i])) <= 1e-6 * s
+    # barycentrics reproduce the closest point
+    rec = np.einsum("ij,ijk->ik", b64, V64[Fd[p64]])
+    np.testing.assert_allclose(rec, pt64, atol=1e-12)
+
+
+def test_lbvh_matches_brute_force_cpq():
+    w = W.cpq_small(n_queries=8192)
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    bvh = O.CpuBvh(V, F, 4)
+    info = bvh.info()
+    assert info["bad"] == 0 and info["max_leaf"] <= 4
+    got = bvh.closest_point(Q)
+    exp = O.cpq_brute_f32(V, F, Q)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_lbvh_matches_brute_force_rays():
+    w = W.rays_scene(n_rays=4096, grid=61, n_spheres=4)
+    V, F = w["verts"], w["tris"]
+    bvh = O.CpuBvh(V, F, 4)
+    for g, e in zip(bvh.trace_closest(w["rays"]), O.ray_closest_brute_f32(V, F, w["rays"])):
+        assert np.array_equal(g, e)
+    assert np.array_equal(bvh.trace_any(w["rays_any"]), O.ray_any_brute_f32(V, F, w["rays_any"]))
+
+
+def test_ray_brute_f32_vs_f64():
+    w = W.rays_scene(n_rays=2048, grid=41, n_spheres=3)
+    V, F, R = w["verts"], w["tris"], w["rays"]
+    p32, t32, uv32 = O.ray_closest_brute_f32(V, F, R)
+    p64, t64, uv64 = O.ray_closest_brute_f64(V, F, R)
+    hit = p64 >= 0
+    assert (p32 != p64).mean() < 0.002
+    same = hit & (p32 == p64)
+    rel = np.abs(t32[same] - t64[same]) / t64[same]
+    assert rel.max() <= 1e-5
+    np.testing.assert_allclose(uv32[same], uv64[same], atol=1e-4)
+
+
+def test_morton_known_answers_and_lbvh_order():
+    lo = np.zeros(3, np.float32)
+    hi = np.ones(3, np.float32)
+    c = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5],
+                  [1 / 1024, 0, 0]], np.float32)
+    codes = O.morton_codes(c, lo, hi)
+    # x occupies bit 3k+2, y bit 3k+1, z bit 3k (Karras 2012 interleave)
+    assert codes.tolist() == [0, 0x3FFFFFFF, 1 << 29, 1 << 28, 1 << 27, 1 << 2]
+    w = W.cpq_small()
+    V, F = w["verts"], w["tris"]
+    bvh = O.CpuBvh(V, F, 4)
+    order = bvh.order()
+    assert np.array_equal(np.sort(order), np.arange(F.shape[0]))  # a permutation
+    tri = V[F.astype(np.int64)]
+    cen = ((tri.min(1) + tri.max(1)) * np.float32(0.5)).astype(np.float32)
+    k = O.morton_codes(cen, cen.min(0), cen.max(0))[order]
+    assert np.all((k[1:] > k[:-1]) | ((k[1:] == k[:-1]) & (order[1:] > order[:-1])))
+
+
+def test_icosphere_closed_forms():
+    v, f = W.icosphere(3)
+    assert f.shape == (1280, 3) and v.shape == (642, 3)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-12)
+    V, F = v.astype(np.float32), f.astype(np.uint32)
+    # a query on a vertex has distance 0 and closest point = the vertex
+    prim, pt, dist, _ = O.cpq_brute_f32(V, F, V[:50])
+    assert np.all(dist == 0) and np.array_equal(pt, V[:50])
+    # queries outside the sphere: r - 1 <= d <= distance to the nearest vertex
+    rng = np.random.default_rng(4)
+    dirs = W.random_dirs(rng, 500)
+    r = rng.uniform(1.1, 2.0, 500)
+    Q = (dirs * r[:, None]).astype(np.float32)
+    _, _, d, _ = O.cpq_brute_f64(V, F, Q)
+    assert np.all(d >= r - 1 - 1e-6)
+    nearest_v = np.min(np.linalg.norm(Q[:, None, :].astype(np.float64) - V[None], axis=2), axis=1)
+    assert np.all(d <= nearest_v + 1e-9)
+    # a ray from the centre hits at t in [inradius, 1]
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    inr = np.min(np.abs(np.sum(nrm * a, 1)) / np.linalg.norm(nrm, axis=1))
+    R = W.pack_rays(np.zeros((500, 3)), dirs, 0.0, 10.0)
+    p, t, _ = O.ray_closest_brute_f32(V, F, R)
+    assert np.all(p >= 0) and np.all(t >= inr - 1e-6) and np.all(t <= 1 + 1e-6)
+
+
+def test_cpq_gradients_match_fp64_central_differences():
+    w = W.cpq_small(n_queries=256)
+    V = w["verts"].astype(np.float64)
+    F = w["tris"]
+    Q = w["queries"].astype(np.float64)
+    prim, pt, dist, bary = O.cpq_brute_f64(V, F, Q)
+    gq, gv = O.cpq_grad_f64(V, F, Q, prim, pt, bary)
+    h = 1e-4
+    # w.r.t. queries; central differences are only meaningful where d is smooth on the
+    # stencil, so points whose analytic gradient changes within +-h (medial axis) skip
+    for k in range(3):
+        e = np.zeros(3); e[k] = h
+        rp = O.cpq_brute_f64(V, F, Q + e)
+        rm = O.cpq_brute_f64(V, F, Q - e)
+        gp = O.cpq_grad_f64(V, F, Q + e, rp[0], rp[1], rp[3])[0]
+        gm = O.cpq_grad_f64(V, F, Q - e, rm[0], rm[1], rm[3])[0]
+        smooth = np.max(np.abs(gp - gm), axis=1) < 1e-2
+        assert smooth.mean() > 0.95
+        fd = (rp[2] - rm[2]) / (2 * h)
+        np.testing.assert_allclose(gq[smooth, k], fd[smooth], atol=2e-5)
+    # w.r.t. vertices: loss = sum of distances; check the vertices with gradient mass
+    rng = np.random.default_rng(0)
+    touched = np.unique(F[prim].ravel())
+    for j in rng.choice(touched, 12, replace=False):
+        for k in range(3):
+            Vp = V.copy(); Vp[j, k] += h
+            Vm = V.copy(); Vm[j, k] -= h
+            fd = (O.cpq_brute_f64(Vp, F, Q)[2].sum() - O.cpq_brute_f64(Vm, F, Q)[2].sum()) / (2 * h)
+            assert gv[j, k] == pytest.approx(fd, abs=5e-5)
+
+
+def test_ray_gradients_match_fp64_central_differences():
+    v, f = W.icosphere(3)
+    V, F = v, f.astype(np.uint32)
+    rng = np.random.default_rng(2)
+    o = W.random_dirs(rng, 128) * 3
+    d = -o / np.linalg.norm(o, axis=1, keepdims=True) + rng.normal(0, 0.1, (128, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    R = np.concatenate([o, np.zeros((128, 1)), d, np.full((128, 1), 10.0)], 1)
+    prim, t, uv = O.ray_closest_brute_f64(V, F, R)
+    assert np.all(prim >= 0)
+    go, gd, gv = O.ray_grad_f64(V, F, R, prim, t, uv)
+    h = 1e-5
+    for k in range(3):
+        Rp = R.copy(); Rp[:, k] += h
+        Rm = R.copy(); Rm[:, k] -= h
+        fd = (O.ray_closest_brute_f64(V, F, Rp)[1] - O.ray_closest_brute_f64(V, F, Rm)[1]) / (2 * h)
+        ok = np.abs(fd) < 100  # skip rays whose hit switches triangle inside +-h
+        np.testing.assert_allclose(go[ok, k], fd[ok], atol=1e-4)
+        Rp = R.copy(); Rp[:, 4 + k] += h
+        Rm = R.copy(); Rm[:, 4 + k] -= h
+        fd = (O.ray_closest_brute_f64(V, F, Rp)[1] - O.ray_closest_brute_f64(V, F, Rm)[1]) / (2 * h)
+        np.testing.assert_allclose(gd[:, k], fd, atol=1e-3)
+    for j in np.unique(F[prim].ravel())[:10]:
+        for k in range(3):
+            Vp = V.copy(); Vp[j, k] += h
+            Vm = V.copy(); Vm[j, k] -= h
+            fd = (O.ray_closest_brute_f64(Vp, F, R)[1].sum() -
+                  O.ray_closest_brute_f64(Vm, F, R)[1].sum()) / (2 * h)
+            assert gv[j, k] == pytest.approx(fd, abs=1e-3)
