@@ -208,16 +208,42 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
             "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb}
 
 
+def profile_mode(args, torch, gpurt, W, dev, ctx):
+    """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
+    rays_scene, then the same number of 256 MiB device copies (a plain copy kernel of
+    known size, whose FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
+    w = W.rays_scene()
+    tv = torch.from_numpy(w["verts"]).to(dev)
+    tf = torch.from_numpy(w["tris"]).to(dev)
+    rays = torch.from_numpy(w["rays"]).to(dev)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    n = rays.shape[0]
+    out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
+           torch.empty((n, 2), device=dev))
+    for _ in range(args.profile_reps):
+        bvh.trace_closest(rays, out=out, stack_depth=args.stack)
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.profile_reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "rays_scene closest-hit", "launches": args.profile_reps,
+                      "rays": n, "copy_bytes": src.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
-    ap.add_argument("--stack", type=int, default=32, choices=[16, 32, 64])
+    ap.add_argument("--stack", type=int, default=8, choices=[8, 16, 32, 64])
     ap.add_argument("--cpu-rays", type=int, default=131072)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run closest-hit launches + calibration copies (for rocprofv3)")
+    ap.add_argument("--profile-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box "
@@ -232,6 +258,9 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     ctx = gpurt.Context(0)
+    if args.profile:
+        profile_mode(args, torch, gpurt, W, dev, ctx)
+        return
     r = ray_bench(args, torch, gpurt, W, O, dev, ctx)
     extra = {"rays": r.pop("extra_rays")}
     if not args.no_extra:

@@ -231,8 +231,9 @@ int gpurt_bvh_get_info(gpurt_bvh b, gpurt_bvh_info* out) {
 
 static int check_opts(const gpurt_query_opts* o) {
     if (!o) return GPURT_OK;
-    if (o->stack_depth != 0 && o->stack_depth != 16 && o->stack_depth != 32 && o->stack_depth != 64)
-        return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 16, 32 or 64");
+    if (o->stack_depth != 0 && o->stack_depth != 8 && o->stack_depth != 16 && o->stack_depth != 32 &&
+        o->stack_depth != 64)
+        return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 8, 16, 32 or 64");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 64)
         return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
     return GPURT_OK;

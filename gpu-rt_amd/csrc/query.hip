@@ -91,10 +91,17 @@ __device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&c)[W]) {
 // closest point
 // ------------------------------------------------------------------------------------
 
+// Resident waves per SIMD the register allocator targets: 8 (<= 64 VGPRs) for 2- and
+// 4-wide nodes; 8-wide nodes need more live registers for the child boxes.
+template <int W>
+struct WavesHint {
+    static constexpr int value = W == 8 ? 4 : 8;
+};
+
 constexpr float kCullSlack = 1.0f + 0x1p-16f;  // relative slack on squared distances
 
 template <int W, int S, int BS, bool STATS>
-__global__ __launch_bounds__(BS) void k_cpq(TravArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_cpq(TravArgs a) {
     __shared__ uint2 stk[S * BS];
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
     const float4* __restrict__ tris = a.tris;
@@ -195,7 +202,7 @@ __global__ __launch_bounds__(BS) void k_cpq(TravArgs a) {
 constexpr float kTSlack = 0x1p-20f;
 
 template <int W, int S, int BS, bool ANY, bool STATS>
-__global__ __launch_bounds__(BS) void k_ray(TravArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_ray(TravArgs a) {
     __shared__ uint2 stk[S * BS];
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
     const float4* __restrict__ tris = a.tris;
@@ -280,18 +287,21 @@ __global__ __launch_bounds__(BS) void k_ray(TravArgs a) {
                 if (!h) c[j] = kEmpty;
             }
             if (!ANY) sort_children<W>(tn, c);
-            // push all but the first hit child, then continue with the first
-            int first = -1;
+            // push all but the nearest hit child (far to near), then continue with it;
+            // the pending entry is carried in scalars so every index stays static
+            uint32_t pend = kEmpty;
+            float pend_t = 0.0f;
 #pragma unroll
             for (int j = W - 1; j >= 0; j--) {
                 if (c[j] == kEmpty) continue;
-                if (first >= 0) {
-                    if (!st.push(c[first], tn[first])) overflow = true;
+                if (pend != kEmpty) {
+                    if (!st.push(pend, pend_t)) overflow = true;
                 }
-                first = j;
+                pend = c[j];
+                pend_t = tn[j];
             }
-            if (first >= 0) {
-                cur = c[first];
+            if (pend != kEmpty) {
+                cur = pend;
                 have = true;
             }
         }
@@ -335,10 +345,11 @@ static void launch_kind(QKind kind, const TravArgs& a, unsigned grid, hipStream_
     }
 }
 
-// LDS per block is S * BS * 8 bytes = 32 KB for every (S, BS) pair below.
+// LDS per block is S * BS * 8 bytes: 16 KB for S = 8, 32 KB for the others.
 template <int W, bool STATS>
 static int dispatch_stack(int S, QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (S) {
+        case 8: launch_kind<W, 8, 256, STATS>(kind, a, grid, s); return 256;
         case 16: launch_kind<W, 16, 256, STATS>(kind, a, grid, s); return 256;
         case 32: launch_kind<W, 32, 128, STATS>(kind, a, grid, s); return 128;
         case 64: launch_kind<W, 64, 64, STATS>(kind, a, grid, s); return 64;
@@ -346,20 +357,26 @@ static int dispatch_stack(int S, QKind kind, const TravArgs& a, unsigned grid, h
     return 0;
 }
 
-static int block_for_stack(int S) { return S == 16 ? 256 : (S == 32 ? 128 : 64); }
+static int block_for_stack(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 
 static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpurt_query_opts* opts,
                      hipStream_t s) {
     if (n == 0) return GPURT_OK;
     if (n > 0xFFFFFFFFull - 1024u) return set_error(GPURT_E_INVALID, "too many queries in one call");
-    int S = opts && opts->stack_depth ? opts->stack_depth : 32;
-    if (S != 16 && S != 32 && S != 64)
-        return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 16, 32 or 64");
+    // default: the smallest LDS stack (most resident waves); 8-wide nodes push more
+    // entries per visit, so they default to 16
+    int S = opts && opts->stack_depth ? opts->stack_depth : (b->width == 8 ? 16 : 8);
+    if (S != 8 && S != 16 && S != 32 && S != 64)
+        return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 8, 16, 32 or 64");
     const int BS = block_for_stack(S);
     gpurt_ctx ctx = b->ctx;
-    // resident blocks per CU: LDS-limited (32 KB each) unless the caller asks for waves/CU
-    int blocks_per_cu = ctx->lds_per_cu / (S * BS * 8);
-    if (opts && opts->waves_per_cu > 0) blocks_per_cu = std::max(1, opts->waves_per_cu / (BS / 64));
+    // resident blocks per CU: limited by LDS (the stacks), by the 32 wave slots of a CU
+    // and by registers (WavesHint waves per SIMD), unless the caller asks for waves/CU
+    const int waves_per_block = BS / 64;
+    const int reg_waves = 4 * (b->width == 8 ? WavesHint<8>::value : WavesHint<4>::value);
+    int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8), std::min(32, reg_waves) / waves_per_block);
+    if (opts && opts->waves_per_cu > 0) blocks_per_cu = std::max(1, opts->waves_per_cu / waves_per_block);
+    blocks_per_cu = std::max(blocks_per_cu, 1);
     unsigned grid = (unsigned)std::min<size_t>(cdiv(n, BS), (size_t)ctx->n_cu * blocks_per_cu);
     if (grid == 0) grid = 1;
     const uint32_t total = grid * BS;
