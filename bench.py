@@ -208,6 +208,49 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
             "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb}
 
 
+def batched_bench(args, torch, gpurt, W, dev, ctx):
+    """256 meshes (log-uniform 1K..524K triangles), each with its own closest-point
+    queries and rays: one batch replay (all builds + one persistent query launch, a HIP
+    graph) against the plain path (per mesh: build, closest-point launch, closest-hit
+    launch), both rebuilding every BVH each step."""
+    t0 = time.time()
+    w = W.batched()
+    log(f"[bench] batched generated in {time.time() - t0:.1f}s: {w['meta']}")
+    meshes = [{k: torch.from_numpy(v).to(dev) for k, v in m.items()} for m in w["meshes"]]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack)
+    create_s = time.perf_counter() - t0
+    ms_b = timed(batch.run, max(3, args.steps // 4), 1)
+    res = batch.run()
+
+    plain_out = {}
+
+    def plain():
+        for i, m in enumerate(meshes):
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width)
+            p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack)
+            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack)
+            plain_out[i] = (p, d, rp, rt)
+            bvh.close()
+
+    ms_p = timed(plain, 2, 1)
+    torch.cuda.synchronize()
+    parity = all(torch.equal(plain_out[i][0], r["q_prim"]) and torch.equal(plain_out[i][1], r["q_dist"])
+                 and torch.equal(plain_out[i][2], r["r_prim"]) and torch.equal(plain_out[i][3], r["r_t"])
+                 for i, r in enumerate(res))
+    nq = sum(m["queries"].shape[0] for m in meshes)
+    nr = sum(m["rays"].shape[0] for m in meshes)
+    batch.close()
+    return {"workload": "batched", "n_meshes": len(meshes), "n_tris": w["meta"]["n_tris_total"],
+            "n_queries": nq, "n_rays": nr,
+            "batched_units_per_s": (nq + nr) / (ms_b * 1e-3), "batched_ms": ms_b,
+            "plain_units_per_s": (nq + nr) / (ms_p * 1e-3), "plain_ms": ms_p,
+            "batched_over_plain": ms_p / ms_b, "batch_create_s": create_s,
+            "parity_batched_vs_plain": bool(parity),
+            "note": "each step rebuilds all 256 BVHs and answers every query"}
+
+
 def profile_mode(args, torch, gpurt, W, dev, ctx):
     """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
     rays_scene, then the same number of 256 MiB device copies (a plain copy kernel of
@@ -241,6 +284,7 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=131072)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--profile", action="store_true",
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -268,6 +312,9 @@ def main():
         extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx)
         log("[bench] diff fwd / fwd+bwd")
         extra["diff"] = diff_bench(args, torch, gpurt, W, dev, ctx)
+        if not args.no_batched:
+            log("[bench] batched vs plain")
+            extra["batched"] = batched_bench(args, torch, gpurt, W, dev, ctx)
         if args.large:
             log("[bench] cpq_large")
             extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx)

@@ -159,7 +159,7 @@ int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, siz
     auto* b = new gpurt_bvh_s();
     b->ctx = ctx;
     b->n_verts = (int64_t)n_verts;
-    rc = build_segments(ctx, &sd, 1, W, leaf, tsize, trounds, s, true, b);
+    rc = build_segments(ctx, &sd, 1, W, leaf, tsize, trounds, s, b);
     if (rc) {
         delete b;
         return rc;

@@ -560,25 +560,135 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
     }
 }
 
+// level 0 of the wide tree: one node per mesh root
+__global__ void k_init_levels(int n_seg, const uint32_t* __restrict__ seg_root, uint32_t* __restrict__ wsrc,
+                              uint32_t* __restrict__ wseg) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_seg) {
+        wsrc[i] = seg_root[i];
+        wseg[i] = (uint32_t)i;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // host orchestration
 // ------------------------------------------------------------------------------------
 
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() {
+BuildBufs::~BuildBufs() {
+    for (void* p : allocs)
         if (p) (void)hipFree(p);
-    }
-};
+}
 
-static int dev_alloc(DevBuf& b, size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
+void* BuildBufs::release(void* p) {
+    for (auto& a : allocs)
+        if (a == p) a = nullptr;
+    return p;
+}
+
+template <typename T>
+static int balloc(BuildBufs& b, T*& p, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        b.p = nullptr;
         return set_error(GPURT_E_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
+    b.allocs.push_back(q);
+    p = static_cast<T*>(q);
+    return GPURT_OK;
+}
+
+int build_alloc(const SegDesc* host_segs, int n_seg, int W, BuildBufs& b) {
+    uint64_t N64 = 0;
+    for (int i = 0; i < n_seg; i++) {
+        if (host_segs[i].n_tris == 0) return set_error(GPURT_E_INVALID, "a mesh has no triangles");
+        N64 += host_segs[i].n_tris;
+    }
+    if (n_seg <= 0 || N64 == 0 || N64 > kMaxLeafFirst - 8)
+        return set_error(GPURT_E_INVALID, "triangle count out of range (1 .. 2^28-9)");
+    const uint32_t N = (uint32_t)N64;
+    b.N = N;
+    b.n_seg = n_seg;
+    b.width = W;
+    b.nblocks_sort = cdiv(N, kSortTile);
+    const size_t NI = N > 1 ? N - 1 : 1;
+    b.capacity = (uint32_t)(NI + n_seg);
+    int rc;
+    if ((rc = balloc(b, b.segs, n_seg)) || (rc = balloc(b, b.aabb_lo, N)) ||
+        (rc = balloc(b, b.aabb_hi, N)) || (rc = balloc(b, b.tri_v, 3 * (size_t)N)) ||
+        (rc = balloc(b, b.segstat, (size_t)kStat * n_seg)) || (rc = balloc(b, b.keys0, N)) ||
+        (rc = balloc(b, b.keys1, N)) || (rc = balloc(b, b.vals0, N)) || (rc = balloc(b, b.vals1, N)) ||
+        (rc = balloc(b, b.hist, 256 * (size_t)b.nblocks_sort)) || (rc = balloc(b, b.saabb, 2 * (size_t)N)) ||
+        (rc = balloc(b, b.left, NI)) || (rc = balloc(b, b.right, NI)) || (rc = balloc(b, b.range, NI)) ||
+        (rc = balloc(b, b.nparent, NI)) || (rc = balloc(b, b.lparent, N)) ||
+        (rc = balloc(b, b.seg_root, n_seg)) || (rc = balloc(b, b.is_root, NI)) ||
+        (rc = balloc(b, b.nbox, 2 * NI)) || (rc = balloc(b, b.flags, NI)) ||
+        (rc = balloc(b, b.wsrc, b.capacity)) || (rc = balloc(b, b.wseg, b.capacity)) ||
+        (rc = balloc(b, b.lvl, kMaxLevels + 1)) || (rc = balloc(b, b.counters, 4)) ||
+        (rc = balloc(b, b.tris, 3 * (size_t)N)) || (rc = balloc(b, b.leaf_order, N)))
+        return rc;
+    void* nodes = nullptr;
+    if (hipMalloc(&nodes, node_bytes(W) * b.capacity) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GPURT_E_OOM, "hipMalloc of BVH nodes failed");
+    }
+    b.allocs.push_back(nodes);
+    b.nodes = nodes;
+    GPURT_HIP(hipMemcpy(b.segs, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice));
+    return GPURT_OK;
+}
+
+// Enqueues the build up to (not including) the wide collapse.  No host synchronisation
+// and no memset nodes, so the sequence can be captured into a HIP graph.
+int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hipStream_t s) {
+    const uint32_t N = b.N;
+    const int n_seg = b.n_seg;
+    const size_t NI = N > 1 ? N - 1 : 1;
+    const unsigned gN = cdiv(N, kBlock);
+    hipLaunchKernelGGL(k_init_build, dim3(cdiv(std::max(n_seg, kMaxLevels + 1), kBlock)), dim3(kBlock),
+                       0, s, b.segs, n_seg, b.segstat, b.seg_root, b.lvl, b.counters);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(gN, 4096u)), dim3(kBlock), 0, s, b.lparent, (size_t)N,
+                       kEmpty);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s,
+                       b.nparent, NI, kEmpty);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s, b.flags,
+                       NI, 0u);
+    hipLaunchKernelGGL(k_prep, dim3(gN), dim3(kBlock), 0, s, b.segs, n_seg, N, b.aabb_lo, b.aabb_hi,
+                       b.tri_v, b.segstat, b.counters);
+    hipLaunchKernelGGL(k_morton, dim3(gN), dim3(kBlock), 0, s, N, b.aabb_lo, b.aabb_hi, b.segstat,
+                       b.keys0, b.vals0);
+    // radix sort over the bits in use: 30 Morton bits + mesh-id bits
+    int seg_bits = 0;
+    while ((1u << seg_bits) < (uint32_t)n_seg) seg_bits++;
+    const int key_bits = 30 + seg_bits;
+    uint64_t* kin = b.keys0;
+    uint64_t* kout = b.keys1;
+    uint32_t* vin = b.vals0;
+    uint32_t* vout = b.vals1;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, N, shift, b.hist,
+                           b.nblocks_sort);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, b.hist, 256u * b.nblocks_sort);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, vin, kout,
+                           vout, N, shift, b.hist, b.nblocks_sort);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, vin, b.aabb_lo, b.aabb_hi, b.tri_v,
+                       b.saabb, b.tris, b.leaf_order);
+    if (N > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(cdiv(N - 1, kBlock)), dim3(kBlock), 0, s, (int)N, kin, b.segs,
+                           b.left, b.right, b.range, b.nparent, b.lparent, b.seg_root, b.is_root);
+        hipLaunchKernelGGL(k_bottom_up, dim3(gN), dim3(kBlock), 0, s, (int)N, b.lparent, b.nparent,
+                           b.left, b.right, b.is_root, b.saabb, b.nbox, b.flags);
+        if (treelet_size > 0) {
+            int trc = run_treelets(b, treelet_size, treelet_rounds, s);
+            if (trc != GPURT_OK) return trc;
+        }
+    }
+    hipLaunchKernelGGL(k_init_levels, dim3(cdiv(n_seg, kBlock)), dim3(kBlock), 0, s, n_seg, b.seg_root,
+                       b.wsrc, b.wseg);
+    GPURT_HIP(hipGetLastError());
     return GPURT_OK;
 }
 
@@ -593,280 +703,95 @@ static void collapse_level(int W, const CollapseArgs& a, unsigned grid, hipStrea
     else launch_collapse_level<8>(a, grid, s);
 }
 
-int run_treelets(int N, int treelet_size, int rounds, const uint32_t* left, const uint32_t* right,
-                 uint32_t* left_rw, uint32_t* right_rw, uint2* range, uint32_t* node_parent,
-                 uint32_t* leaf_parent, const uint8_t* is_root, const float4* saabb, float4* nbox,
-                 uint32_t* flags, hipStream_t s);
-
-// Builds one BVH per segment into one node array and one packed-triangle array.
-// With n_seg == 1 the result is stored in *out (plain build).
-int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, int leaf_size,
-                   int treelet_size, int treelet_rounds, hipStream_t s, bool /*sync_host*/,
-                   gpurt_bvh_s* out) {
-    uint64_t N64 = 0;
-    for (int i = 0; i < n_seg; i++) N64 += host_segs[i].n_tris;
-    if (N64 == 0 || N64 > kMaxLeafFirst - 8)
-        return set_error(GPURT_E_INVALID, "triangle count out of range (1 .. 2^28-9)");
-    const uint32_t N = (uint32_t)N64;
-    (void)ctx;
-
-    DevBuf b_segs, b_aabb_lo, b_aabb_hi, b_tri_v, b_segstat, b_keys0, b_keys1, b_vals0, b_vals1,
-        b_hist, b_saabb, b_left, b_right, b_range, b_nparent, b_lparent, b_seg_root, b_is_root,
-        b_nbox, b_flags, b_wsrc, b_wseg, b_lvl, b_counters;
-    const uint32_t nblocks_sort = cdiv(N, kSortTile);
-    const size_t NI = N > 1 ? N - 1 : 1;
-    const uint32_t capacity = (uint32_t)std::max<uint64_t>(NI + n_seg, 1);
-    int rc;
-#define ALLOC(buf, bytes) \
-    if ((rc = dev_alloc(buf, bytes)) != GPURT_OK) return rc
-    ALLOC(b_segs, sizeof(SegDesc) * n_seg);
-    ALLOC(b_aabb_lo, sizeof(float4) * N);
-    ALLOC(b_aabb_hi, sizeof(float4) * N);
-    ALLOC(b_tri_v, sizeof(float4) * 3 * (size_t)N);
-    ALLOC(b_segstat, sizeof(uint32_t) * kStat * n_seg);
-    ALLOC(b_keys0, sizeof(uint64_t) * N);
-    ALLOC(b_keys1, sizeof(uint64_t) * N);
-    ALLOC(b_vals0, sizeof(uint32_t) * N);
-    ALLOC(b_vals1, sizeof(uint32_t) * N);
-    ALLOC(b_hist, sizeof(uint32_t) * 256 * (size_t)nblocks_sort);
-    ALLOC(b_saabb, sizeof(float4) * 2 * (size_t)N);
-    ALLOC(b_left, sizeof(uint32_t) * NI);
-    ALLOC(b_right, sizeof(uint32_t) * NI);
-    ALLOC(b_range, sizeof(uint2) * NI);
-    ALLOC(b_nparent, sizeof(uint32_t) * NI);
-    ALLOC(b_lparent, sizeof(uint32_t) * N);
-    ALLOC(b_seg_root, sizeof(uint32_t) * n_seg);
-    ALLOC(b_is_root, NI);
-    ALLOC(b_nbox, sizeof(float4) * 2 * NI);
-    ALLOC(b_flags, sizeof(uint32_t) * NI);
-    ALLOC(b_wsrc, sizeof(uint32_t) * capacity);
-    ALLOC(b_wseg, sizeof(uint32_t) * capacity);
-    ALLOC(b_lvl, sizeof(uint32_t) * (kMaxLevels + 1));
-    ALLOC(b_counters, sizeof(uint32_t) * 4);
-
-    // product buffers (owned by the bvh)
-    void* nodes = nullptr;
-    float4* tris = nullptr;
-    uint32_t* leaf_order = nullptr;
-    auto fail_free = [&]() {
-        if (nodes) (void)hipFree(nodes);
-        if (tris) (void)hipFree(tris);
-        if (leaf_order) (void)hipFree(leaf_order);
-    };
-    if (hipMalloc(&nodes, node_bytes(W) * capacity) != hipSuccess ||
-        hipMalloc((void**)&tris, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
-        hipMalloc((void**)&leaf_order, sizeof(uint32_t) * N) != hipSuccess) {
-        (void)hipGetLastError();
-        fail_free();
-        return set_error(GPURT_E_OOM, "hipMalloc of BVH storage failed");
-    }
-#undef ALLOC
-
-    auto* segs_d = (SegDesc*)b_segs.p;
-    auto* aabb_lo = (float4*)b_aabb_lo.p;
-    auto* aabb_hi = (float4*)b_aabb_hi.p;
-    auto* tri_v = (float4*)b_tri_v.p;
-    auto* segstat = (uint32_t*)b_segstat.p;
-    auto* keys0 = (uint64_t*)b_keys0.p;
-    auto* keys1 = (uint64_t*)b_keys1.p;
-    auto* vals0 = (uint32_t*)b_vals0.p;
-    auto* vals1 = (uint32_t*)b_vals1.p;
-    auto* hist = (uint32_t*)b_hist.p;
-    auto* saabb = (float4*)b_saabb.p;
-    auto* left = (uint32_t*)b_left.p;
-    auto* right = (uint32_t*)b_right.p;
-    auto* range = (uint2*)b_range.p;
-    auto* nparent = (uint32_t*)b_nparent.p;
-    auto* lparent = (uint32_t*)b_lparent.p;
-    auto* seg_root = (uint32_t*)b_seg_root.p;
-    auto* is_root = (uint8_t*)b_is_root.p;
-    auto* nbox = (float4*)b_nbox.p;
-    auto* flags = (uint32_t*)b_flags.p;
-    auto* wsrc = (uint32_t*)b_wsrc.p;
-    auto* wseg = (uint32_t*)b_wseg.p;
-    auto* lvl = (uint32_t*)b_lvl.p;
-    auto* counters = (uint32_t*)b_counters.p;
-
-#define CHECK_LAUNCH()                                        \
-    do {                                                      \
-        hipError_t e_ = hipGetLastError();                    \
-        if (e_ != hipSuccess) {                               \
-            fail_free();                                      \
-            return hip_error(e_, "kernel launch (build)");    \
-        }                                                     \
-    } while (0)
-
-    if (hipMemcpyAsync(segs_d, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice, s) !=
-        hipSuccess) {
-        fail_free();
-        return hip_error(hipGetLastError(), "hipMemcpyAsync(segments)");
-    }
-    const unsigned gN = cdiv(N, kBlock);
-    hipLaunchKernelGGL(k_init_build, dim3(cdiv(std::max(n_seg, kMaxLevels + 1), kBlock)), dim3(kBlock),
-                       0, s, segs_d, n_seg, segstat, seg_root, lvl, counters);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(gN, 4096u)), dim3(kBlock), 0, s, lparent, (size_t)N,
-                       kEmpty);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s,
-                       nparent, NI, kEmpty);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s, flags,
-                       NI, 0u);
-    hipLaunchKernelGGL(k_prep, dim3(gN), dim3(kBlock), 0, s, segs_d, n_seg, N, aabb_lo, aabb_hi,
-                       tri_v, segstat, counters);
-    hipLaunchKernelGGL(k_morton, dim3(gN), dim3(kBlock), 0, s, N, aabb_lo, aabb_hi, segstat, keys0,
-                       vals0);
-    CHECK_LAUNCH();
-
-    // radix sort over the bits in use: 30 Morton bits + mesh-id bits
-    int seg_bits = 0;
-    while ((1u << seg_bits) < (uint32_t)n_seg) seg_bits++;
-    const int key_bits = 30 + seg_bits;
-    uint64_t* kin = keys0;
-    uint64_t* kout = keys1;
-    uint32_t* vin = vals0;
-    uint32_t* vout = vals1;
-    for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks_sort), dim3(kBlock), 0, s, kin, N, shift, hist,
-                           nblocks_sort);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, hist, 256u * nblocks_sort);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks_sort), dim3(kBlock), 0, s, kin, vin, kout,
-                           vout, N, shift, hist, nblocks_sort);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
-    CHECK_LAUNCH();
-
-    hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, vin, aabb_lo, aabb_hi, tri_v, saabb,
-                       tris, leaf_order);
-    if (N > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(cdiv(N - 1, kBlock)), dim3(kBlock), 0, s, (int)N, kin,
-                           segs_d, left, right, range, nparent, lparent, seg_root, is_root);
-        hipLaunchKernelGGL(k_bottom_up, dim3(gN), dim3(kBlock), 0, s, (int)N, lparent, nparent, left,
-                           right, is_root, saabb, nbox, flags);
-        if (treelet_size > 0) {
-            int trc = run_treelets((int)N, treelet_size, treelet_rounds, left, right, left, right,
-                                   range, nparent, lparent, is_root, saabb, nbox, flags, s);
-            if (trc != GPURT_OK) {
-                fail_free();
-                return trc;
-            }
-        }
-    }
-    CHECK_LAUNCH();
-
-    // level 0 = one wide node per mesh root
-    std::vector<uint32_t> h_roots(n_seg), h_segid(n_seg);
-    if (hipMemcpyAsync(wsrc, seg_root, sizeof(uint32_t) * n_seg, hipMemcpyDeviceToDevice, s) !=
-        hipSuccess) {
-        fail_free();
-        return hip_error(hipGetLastError(), "hipMemcpyAsync(roots)");
-    }
-    for (int i = 0; i < n_seg; i++) h_segid[i] = (uint32_t)i;
-    if (hipMemcpyAsync(wseg, h_segid.data(), sizeof(uint32_t) * n_seg, hipMemcpyHostToDevice, s) !=
-        hipSuccess) {
-        fail_free();
-        return hip_error(hipGetLastError(), "hipMemcpyAsync(segment ids)");
-    }
-
+int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level_end, hipStream_t s) {
     CollapseArgs ca;
-    ca.left = left;
-    ca.right = right;
-    ca.range = range;
-    ca.saabb = saabb;
-    ca.nbox = nbox;
-    ca.segstat = segstat;
-    ca.wsrc = wsrc;
-    ca.wseg = wseg;
-    ca.lvl_cnt = lvl;
-    ca.counters = counters;
-    ca.nodes = nodes;
-    ca.capacity = capacity;
+    ca.left = b.left;
+    ca.right = b.right;
+    ca.range = b.range;
+    ca.saabb = b.saabb;
+    ca.nbox = b.nbox;
+    ca.segstat = b.segstat;
+    ca.wsrc = b.wsrc;
+    ca.wseg = b.wseg;
+    ca.lvl_cnt = b.lvl;
+    ca.counters = b.counters;
+    ca.nodes = b.nodes;
+    ca.capacity = b.capacity;
     ca.leaf_size = leaf_size;
-    const unsigned cgrid = std::min(cdiv(capacity, kBlock), 2048u);
+    const unsigned cgrid = std::min(cdiv(b.capacity, kBlock), 2048u);
+    for (int level = level_begin; level < level_end; level++) {
+        ca.level = level;
+        collapse_level(b.width, ca, cgrid, s);
+    }
+    GPURT_HIP(hipGetLastError());
+    return GPURT_OK;
+}
+
+// Plain (single-mesh) build: enqueue, run the collapse in groups of levels until the
+// tree ends (host checks between groups), then hand the products to *out.
+int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, int leaf_size,
+                   int treelet_size, int treelet_rounds, hipStream_t s, gpurt_bvh_s* out) {
+    (void)ctx;
+    BuildBufs b;
+    int rc;
+    if ((rc = build_alloc(host_segs, n_seg, W, b))) return rc;
+    if ((rc = build_enqueue_binary(b, treelet_size, treelet_rounds, s))) return rc;
     std::vector<uint32_t> h_lvl(kMaxLevels + 1, 0);
     int level = 0;
     bool done = false;
-    // launch levels in groups, checking after each group whether the tree has ended
     while (!done && level < kMaxLevels) {
         int group_end = std::min(level + 8, kMaxLevels);
-        for (; level < group_end; level++) {
-            ca.level = level;
-            collapse_level(W, ca, cgrid, s);
-        }
-        CHECK_LAUNCH();
-        if (hipMemcpyAsync(h_lvl.data(), lvl, sizeof(uint32_t) * (kMaxLevels + 1),
-                           hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            fail_free();
-            return hip_error(hipGetLastError(), "reading level counts");
-        }
+        if ((rc = build_enqueue_levels(b, leaf_size, level, group_end, s))) return rc;
+        level = group_end;
+        GPURT_HIP(hipMemcpyAsync(h_lvl.data(), b.lvl, sizeof(uint32_t) * (kMaxLevels + 1),
+                                 hipMemcpyDeviceToHost, s));
+        GPURT_HIP(hipStreamSynchronize(s));
         done = h_lvl[level] == 0;
     }
     uint32_t h_counters[4] = {0, 0, 0, 0};
-    if (hipMemcpyAsync(h_counters, counters, sizeof(h_counters), hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        fail_free();
-        return hip_error(hipGetLastError(), "reading build counters");
-    }
-    if (!done) {
-        fail_free();
-        return set_error(GPURT_E_UNSUPPORTED, "BVH deeper than the supported level count");
-    }
-    if (h_counters[1] & 1u) {
-        fail_free();
-        return set_error(GPURT_E_INVALID, "triangle index >= n_verts");
-    }
-    if (h_counters[1] & 2u) {
-        fail_free();
-        return set_error(GPURT_E_DEVICE, "wide-node capacity exceeded");
-    }
+    GPURT_HIP(hipMemcpyAsync(h_counters, b.counters, sizeof(h_counters), hipMemcpyDeviceToHost, s));
+    uint32_t st[kStat];
+    GPURT_HIP(hipMemcpyAsync(st, b.segstat, sizeof(st), hipMemcpyDeviceToHost, s));
+    GPURT_HIP(hipStreamSynchronize(s));
+    if (!done) return set_error(GPURT_E_UNSUPPORTED, "BVH deeper than the supported level count");
+    if (h_counters[1] & 1u) return set_error(GPURT_E_INVALID, "triangle index >= n_verts");
+    if (h_counters[1] & 2u) return set_error(GPURT_E_DEVICE, "wide-node capacity exceeded");
     int depth = 0;
     while (depth <= kMaxLevels && h_lvl[depth] > 0) depth++;
-
     // shrink the node array to the nodes in use
     const uint32_t n_nodes = h_counters[0];
+    void* nodes = b.nodes;
     void* tight = nullptr;
     if (hipMalloc(&tight, node_bytes(W) * n_nodes) == hipSuccess) {
-        if (hipMemcpyAsync(tight, nodes, node_bytes(W) * n_nodes, hipMemcpyDeviceToDevice, s) ==
-                hipSuccess &&
+        if (hipMemcpyAsync(tight, nodes, node_bytes(W) * n_nodes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
             hipStreamSynchronize(s) == hipSuccess) {
-            (void)hipFree(nodes);
-            nodes = tight;
+            nodes = tight;  // b frees the large array
         } else {
             (void)hipFree(tight);
             (void)hipGetLastError();
+            b.release(nodes);
         }
     } else {
         (void)hipGetLastError();
+        b.release(nodes);
     }
-    if (hipStreamSynchronize(s) != hipSuccess) {
-        fail_free();
-        return hip_error(hipGetLastError(), "build");
-    }
-    float scale = 0.f;
-    {
-        uint32_t st[kStat];
-        if (hipMemcpy(st, segstat, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) {
-            fail_free();
-            return hip_error(hipGetLastError(), "reading mesh scale");
-        }
-        uint32_t u = st[6];
-        u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
-        std::memcpy(&scale, &u, 4);
-    }
+    uint32_t u = st[6];
+    u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float scale;
+    std::memcpy(&scale, &u, 4);
     out->nodes = nodes;
-    out->tris = tris;
-    out->leaf_order = leaf_order;
+    out->tris = (float4*)b.release(b.tris);
+    out->leaf_order = (uint32_t*)b.release(b.leaf_order);
     out->n_nodes = n_nodes;
     out->node_capacity = n_nodes;
     out->depth = depth;
     out->width = W;
     out->leaf_size = leaf_size;
-    out->n_tris = N;
+    out->n_tris = b.N;
     out->scale = scale;
     out->pad = scale * 0x1p-18f;
+    out->level_cnt.assign(h_lvl.begin(), h_lvl.begin() + depth);
     return GPURT_OK;
-#undef CHECK_LAUNCH
 }
 
 }  // namespace gpurt_impl

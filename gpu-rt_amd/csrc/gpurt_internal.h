@@ -17,6 +17,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/gpurt.h"
 
@@ -25,7 +26,10 @@ namespace gpurt_impl {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMaxLeafFirst = 0x0FFFFFFFu;  // 2^28 - 1 packed triangles max
-constexpr int kMaxLevels = 128;                  // wide-node levels the collapse can emit
+// Wide-node levels the collapse can emit.  A mesh's binary radix tree is at most 58
+// levels deep (30 Morton bits + 28 index tie-break bits) and every wide level consumes
+// at least one binary level, so 64 launches always finish the tree.
+constexpr int kMaxLevels = 64;
 constexpr int kBlock = 256;
 
 __host__ __device__ inline uint32_t make_leaf(uint32_t first, uint32_t count) {
@@ -93,6 +97,7 @@ struct gpurt_bvh_s {
     uint32_t* tri_idx = nullptr;  // 3 * n_tris, original order (for backward / refit)
     float* verts = nullptr;       // 3 * n_verts compact positions, original order
     uint32_t* leaf_order = nullptr;  // packed position -> original triangle (refit)
+    std::vector<uint32_t> level_cnt;  // wide nodes per level (level l starts at sum of previous)
 };
 
 namespace gpurt_impl {
@@ -107,11 +112,41 @@ int hip_error(hipError_t e, const char* what);
         if (e_ != hipSuccess) return ::gpurt_impl::hip_error(e_, #call); \
     } while (0)
 
-// builder (build.hip)
-struct BuildScratch;
+// builder (build.hip): device buffers of one build over one or more meshes
+struct BuildBufs {
+    uint32_t N = 0;  // triangles over all meshes
+    int n_seg = 0;   // meshes
+    int width = 4;
+    uint32_t nblocks_sort = 0;
+    uint32_t capacity = 0;  // wide nodes the node array can hold
+    SegDesc* segs = nullptr;
+    float4 *aabb_lo = nullptr, *aabb_hi = nullptr, *tri_v = nullptr;
+    uint32_t* segstat = nullptr;
+    uint64_t *keys0 = nullptr, *keys1 = nullptr;
+    uint32_t *vals0 = nullptr, *vals1 = nullptr, *hist = nullptr;
+    float4* saabb = nullptr;  // sorted triangle boxes (lo, hi)
+    uint32_t *left = nullptr, *right = nullptr;
+    uint2* range = nullptr;
+    uint32_t *nparent = nullptr, *lparent = nullptr, *seg_root = nullptr;
+    uint8_t* is_root = nullptr;
+    float4* nbox = nullptr;  // binary node boxes (lo, hi)
+    uint32_t *flags = nullptr, *wsrc = nullptr, *wseg = nullptr, *lvl = nullptr, *counters = nullptr;
+    void* nodes = nullptr;
+    float4* tris = nullptr;
+    uint32_t* leaf_order = nullptr;
+    std::vector<void*> allocs;  // freed with the object unless release()d
+    BuildBufs() = default;
+    BuildBufs(const BuildBufs&) = delete;
+    BuildBufs& operator=(const BuildBufs&) = delete;
+    ~BuildBufs();
+    void* release(void* p);  // hands ownership of one allocation to the caller
+};
+int build_alloc(const SegDesc* host_segs, int n_seg, int width, BuildBufs& b);
+int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hipStream_t s);
+int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level_end, hipStream_t s);
+int run_treelets(BuildBufs& b, int treelet_size, int rounds, hipStream_t s);
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width, int leaf_size,
-                   int treelet_size, int treelet_rounds, hipStream_t stream, bool sync_host,
-                   gpurt_bvh_s* out_single);
+                   int treelet_size, int treelet_rounds, hipStream_t stream, gpurt_bvh_s* out);
 
 // traversal launchers (query.hip)
 int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,

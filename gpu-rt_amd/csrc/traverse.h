@@ -1,0 +1,264 @@
+// traverse.h -- per-query BVH traversal shared by the plain kernels (query.hip) and the
+// persistent batch kernel (batch.hip).
+//
+// One lane per query.  A lane keeps a stack of (node-or-leaf reference, lower bound)
+// pairs: the first S entries live in LDS laid out [slot][lane] (consecutive lanes hit
+// consecutive banks at every depth), deeper entries spill to a device buffer that the
+// launcher sizes from the tree depth, so no traversal can overflow.  A wide node is
+// fetched as 2*W float4 loads (W=4: one 128-byte line); children are bounded, sorted
+// by the bound with a sorting network, and pushed far-to-near so the nearest is taken
+// next without a stack round trip.  Culls compare against the best value with a small
+// relative slack and boxes are inflated at build time, so a child that can hold an
+// equal-or-better primitive is never culled: results do not depend on the tree's
+// shape, and ties resolve to the lowest primitive id.
+#pragma once
+
+#include "gpurt_internal.h"
+#include "tri_math.h"
+
+namespace gpurt_impl {
+
+template <int S, int BS>
+struct LaneStack {
+    uint2* lds;
+    uint2* spill;
+    uint32_t gid, total, cap;
+    int sp;
+    __device__ __forceinline__ bool push(uint32_t ref, float key) {
+        if ((uint32_t)sp >= cap) return false;
+        uint2 e = make_uint2(ref, __float_as_uint(key));
+        if (sp < S) lds[sp * BS] = e;
+        else spill[(size_t)(sp - S) * total + gid] = e;
+        sp++;
+        return true;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        sp--;
+        if (sp < S) return lds[sp * BS];
+        return spill[(size_t)(sp - S) * total + gid];
+    }
+};
+
+template <int W>
+__device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&c)[W]) {
+    auto cx = [&](int i, int j) {
+        if (d[j] < d[i]) {
+            float tf = d[i]; d[i] = d[j]; d[j] = tf;
+            uint32_t tc = c[i]; c[i] = c[j]; c[j] = tc;
+        }
+    };
+    if constexpr (W == 2) {
+        cx(0, 1);
+    } else if constexpr (W == 4) {
+        cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+    } else {
+        // Batcher odd-even merge sort network for 8 keys (19 comparators)
+        cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+        cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+        cx(1, 2); cx(5, 6);
+        cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+        cx(2, 4); cx(3, 5);
+        cx(1, 2); cx(3, 4); cx(5, 6);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// closest point
+// ------------------------------------------------------------------------------------
+
+// Resident waves per SIMD the register allocator targets: 8 (<= 64 VGPRs) for 2- and
+// 4-wide nodes; 8-wide nodes need more live registers for the child boxes.
+template <int W>
+struct WavesHint {
+    static constexpr int value = W == 8 ? 4 : 8;
+};
+
+constexpr float kCullSlack = 1.0f + 0x1p-16f;  // relative slack on squared distances
+constexpr float kTSlack = 0x1p-20f;             // relative slack on ray box exits
+
+struct CpqHit {
+    uint32_t id;  // 0xFFFFFFFF: none
+    float d2;
+    V3 cp, bary;
+};
+
+// Closest point of q on the subtree at `root`.  Returns false on stack overflow.
+template <int W, int S, int BS, bool STATS>
+__device__ __forceinline__ bool cpq_one(const WNode<W>* __restrict__ nodes,
+                                        const float4* __restrict__ tris, uint32_t root, V3 q,
+                                        LaneStack<S, BS>& st, CpqHit& h,
+                                        unsigned long long& n_nodes, unsigned long long& n_tris) {
+    float best = INFINITY, thr = INFINITY;
+    uint32_t bid = 0xFFFFFFFFu;
+    V3 bcp = v3(0.f, 0.f, 0.f), bb = v3(0.f, 0.f, 0.f);
+    st.sp = 0;
+    uint32_t cur = root;
+    bool have = true;
+    bool ok = true;
+    for (;;) {
+        if (!have) {
+            if (st.sp == 0) break;
+            uint2 e = st.pop();
+            if (__uint_as_float(e.y) > thr) continue;
+            cur = e.x;
+        }
+        have = false;
+        if (child_is_leaf(cur)) {
+            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4 A = tris[3 * (size_t)(f + k)];
+                const float4 B = tris[3 * (size_t)(f + k) + 1];
+                const float4 C = tris[3 * (size_t)(f + k) + 2];
+                V3 cp, br;
+                float d2 = closest_pt_tri(q, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z),
+                                          cp, br);
+                uint32_t id = __float_as_uint(A.w);
+                if (STATS) n_tris++;
+                if (d2 < best || (d2 == best && id < bid)) {
+                    best = d2;
+                    bid = id;
+                    bcp = cp;
+                    bb = br;
+                    thr = best * kCullSlack;
+                }
+            }
+            continue;
+        }
+        if (STATS) n_nodes++;
+        const WNode<W> nd = nodes[cur];
+        float d[W];
+        uint32_t c[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            float dx = fmaxf(fmaxf(nd.lox[j] - q.x, q.x - nd.hix[j]), 0.0f);
+            float dy = fmaxf(fmaxf(nd.loy[j] - q.y, q.y - nd.hiy[j]), 0.0f);
+            float dz = fmaxf(fmaxf(nd.loz[j] - q.z, q.z - nd.hiz[j]), 0.0f);
+            float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            c[j] = nd.child[j];
+            d[j] = (c[j] == kEmpty) ? INFINITY : dd;
+        }
+        sort_children<W>(d, c);
+#pragma unroll
+        for (int j = W - 1; j >= 1; j--) {
+            if (c[j] != kEmpty && d[j] <= thr) {
+                if (!st.push(c[j], d[j])) ok = false;
+            }
+        }
+        if (c[0] != kEmpty && d[0] <= thr) {
+            cur = c[0];
+            have = true;
+        }
+    }
+    h.id = bid;
+    h.d2 = best;
+    h.cp = bcp;
+    h.bary = bb;
+    return ok;
+}
+
+struct RayHit {
+    uint32_t id;  // 0xFFFFFFFF: none
+    float t, u, v;
+    bool any;
+};
+
+// Closest (ANY=false) or any (ANY=true) hit of the ray on the subtree at `root`.
+// Returns false on stack overflow.
+template <int W, int S, int BS, bool ANY, bool STATS>
+__device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
+                                        const float4* __restrict__ tris, uint32_t root, V3 o, V3 d,
+                                        float tmin, float tmax, LaneStack<S, BS>& st, RayHit& h,
+                                        unsigned long long& n_nodes, unsigned long long& n_tris) {
+    float best = tmax;  // shrinks to the best hit
+    uint32_t bid = 0xFFFFFFFFu;
+    float bu = 0.f, bv = 0.f;
+    bool hit_any = false;
+    V3 inv;
+    {
+        float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+        float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+        float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+        inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+    }
+    st.sp = 0;
+    uint32_t cur = root;
+    bool have = true;
+    bool ok = true;
+    for (;;) {
+        if (!have) {
+            if (st.sp == 0) break;
+            uint2 e = st.pop();
+            if (__uint_as_float(e.y) > best) continue;
+            cur = e.x;
+        }
+        have = false;
+        if (child_is_leaf(cur)) {
+            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4 A = tris[3 * (size_t)(f + k)];
+                const float4 B = tris[3 * (size_t)(f + k) + 1];
+                const float4 C = tris[3 * (size_t)(f + k) + 2];
+                if (STATS) n_tris++;
+                float t, u, v;
+                if (!ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), t, u, v))
+                    continue;
+                if (!(t >= tmin && t <= best)) continue;
+                if (ANY) {
+                    hit_any = true;
+                    break;
+                }
+                uint32_t id = __float_as_uint(A.w);
+                if (t < best || id < bid) {
+                    best = t;
+                    bid = id;
+                    bu = u;
+                    bv = v;
+                }
+            }
+            if (ANY && hit_any) break;
+            continue;
+        }
+        if (STATS) n_nodes++;
+        const WNode<W> nd = nodes[cur];
+        float tn[W];
+        uint32_t c[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            float t0x = (nd.lox[j] - o.x) * inv.x, t1x = (nd.hix[j] - o.x) * inv.x;
+            float t0y = (nd.loy[j] - o.y) * inv.y, t1y = (nd.hiy[j] - o.y) * inv.y;
+            float t0z = (nd.loz[j] - o.z) * inv.z, t1z = (nd.hiz[j] - o.z) * inv.z;
+            float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+            float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+            c[j] = nd.child[j];
+            bool hb = (c[j] != kEmpty) && (lo <= hi + fabsf(hi) * kTSlack);
+            tn[j] = hb ? lo : INFINITY;
+            if (!hb) c[j] = kEmpty;
+        }
+        if (!ANY) sort_children<W>(tn, c);
+        // push all but the nearest hit child (far to near), then continue with it; the
+        // pending entry is carried in scalars so every index stays static
+        uint32_t pend = kEmpty;
+        float pend_t = 0.0f;
+#pragma unroll
+        for (int j = W - 1; j >= 0; j--) {
+            if (c[j] == kEmpty) continue;
+            if (pend != kEmpty) {
+                if (!st.push(pend, pend_t)) ok = false;
+            }
+            pend = c[j];
+            pend_t = tn[j];
+        }
+        if (pend != kEmpty) {
+            cur = pend;
+            have = true;
+        }
+    }
+    h.id = bid;
+    h.t = best;
+    h.u = bu;
+    h.v = bv;
+    h.any = hit_any;
+    return ok;
+}
+
+}  // namespace gpurt_impl

@@ -3,9 +3,7 @@
 
 namespace gpurt_impl {
 
-int run_treelets(int, int, int, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint2*,
-                 uint32_t*, uint32_t*, const uint8_t*, const float4*, float4*, uint32_t*,
-                 hipStream_t) {
+int run_treelets(BuildBufs&, int, int, hipStream_t) {
     return set_error(GPURT_E_UNSUPPORTED, "treelet refinement is not available in this build");
 }
 

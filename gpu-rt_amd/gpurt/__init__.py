@@ -18,8 +18,8 @@ import torch
 from . import _lib
 from ._lib import BuildOpts, BvhInfo, GpurtError, QueryOpts
 
-__all__ = ["Context", "Bvh", "GpurtError", "closest_point_distance", "ray_hit_t", "make_rays",
-           "default_context"]
+__all__ = ["Context", "Bvh", "Batch", "GpurtError", "closest_point_distance", "ray_hit_t",
+           "make_rays", "default_context"]
 
 EPS = 1e-5          # rtcommon.glsl:7 -- ray tmin of trace_ray / visibility
 LARGE_DIST = 1e7    # rtcommon.glsl:6 -- ray tmax of trace_ray
@@ -213,6 +213,82 @@ class Bvh:
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().gpurt_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many meshes built and queried together (gpurt_batch_create / gpurt_batch_run).
+
+    meshes: list of dicts with device tensors "verts" (n,k>=3) float32, "tris" (m,3)
+    int32, and optionally "queries" (nq,3) float32 and "rays" (nr,8) float32.  Output
+    tensors are allocated once; every run() rebuilds each BVH from the current vertex
+    values and answers every query, as one recorded HIP graph.  results[i] holds mesh
+    i's outputs: q_prim, q_pt, q_dist, q_bary, r_prim, r_t, r_uv.
+    """
+
+    def __init__(self, ctx, meshes, node_width=4, leaf_size=4, treelet_size=0, stack_depth=0,
+                 waves_per_cu=0, with_bary=True, with_uv=True):
+        lib = _lib.load()
+        self.ctx = ctx
+        dev = ctx.device
+        self._keep = []
+        self.results = []
+        arr = (_lib.BatchMesh * len(meshes))()
+        for i, m in enumerate(meshes):
+            v, f = m["verts"], m["tris"]
+            _need(v, torch.float32, v.shape[1:], "verts", dev)
+            _need(f, f.dtype, (3,), "tris", dev)
+            q = m.get("queries")
+            r = m.get("rays")
+            nq = 0 if q is None else q.shape[0]
+            nr = 0 if r is None else r.shape[0]
+            if q is not None:
+                _need(q, torch.float32, (3,), "queries", dev)
+            if r is not None:
+                _need(r, torch.float32, (8,), "rays", dev)
+            out = {
+                "q_prim": torch.empty(nq, dtype=torch.int32, device=dev),
+                "q_pt": torch.empty((nq, 3), dtype=torch.float32, device=dev),
+                "q_dist": torch.empty(nq, dtype=torch.float32, device=dev),
+                "q_bary": torch.empty((nq, 3), dtype=torch.float32, device=dev) if with_bary else None,
+                "r_prim": torch.empty(nr, dtype=torch.int32, device=dev),
+                "r_t": torch.empty(nr, dtype=torch.float32, device=dev),
+                "r_uv": torch.empty((nr, 2), dtype=torch.float32, device=dev) if with_uv else None,
+            }
+            self.results.append(out)
+            self._keep += [v, f, q, r]
+            bm = arr[i]
+            bm.verts, bm.vstride_bytes, bm.n_verts = v.data_ptr(), v.shape[1] * 4, v.shape[0]
+            bm.tri_idx, bm.n_tris = f.data_ptr(), f.shape[0]
+            bm.q, bm.n_q = (q.data_ptr() if q is not None else None), nq
+            bm.q_prim, bm.q_pt, bm.q_dist = (out["q_prim"].data_ptr(), out["q_pt"].data_ptr(),
+                                             out["q_dist"].data_ptr())
+            bm.q_bary = out["q_bary"].data_ptr() if with_bary else None
+            bm.rays, bm.n_r = (r.data_ptr() if r is not None else None), nr
+            bm.r_prim, bm.r_t = out["r_prim"].data_ptr(), out["r_t"].data_ptr()
+            bm.r_uv = out["r_uv"].data_ptr() if with_uv else None
+        self._arr = arr
+        bo = BuildOpts(node_width, leaf_size, treelet_size, 0)
+        qo = QueryOpts(stack_depth, waves_per_cu, None)
+        torch.cuda.synchronize(dev)  # inputs must be resident before the first (eager) run
+        h = ctypes.c_void_p()
+        _lib.check(lib.gpurt_batch_create(ctx.handle, arr, len(meshes), ctypes.byref(bo),
+                                          ctypes.byref(qo), ctypes.byref(h)))
+        self._h = h
+
+    def run(self):
+        _lib.check(_lib.load().gpurt_batch_run(self._h, _stream(self.ctx.device)))
+        return self.results
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().gpurt_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

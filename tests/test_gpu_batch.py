@@ -1,0 +1,109 @@
+"""Batched path: many meshes built and queried in one recorded graph must give exactly
+the plain per-mesh answers (and the oracle's)."""
+import numpy as np
+import pytest
+import torch
+
+import gpurt
+from gpurt import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_meshes(seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    kinds = [("ico", 1), ("ico", 2), ("terrain", 9), ("ico", 3), ("tri", 1), ("tri", 2),
+             ("terrain", 20), ("ico", 0), ("terrain", 3)]
+    for k, (kind, p) in enumerate(kinds):
+        if kind == "ico":
+            v, f = W.displaced_icosphere(p, np.random.default_rng([seed, k]))
+        elif kind == "terrain":
+            v, f = W.terrain(p, np.random.default_rng([seed, k]), octaves=3)
+            v = v / 50.0
+        else:
+            v = rng.normal(size=(3 * p, 3))
+            f = np.arange(3 * p).reshape(-1, 3)
+        V, F = v.astype(np.float32), f.astype(np.int32)
+        nq = int(rng.integers(1, 700))
+        nr = int(rng.integers(1, 700))
+        q = rng.uniform(-1.5, 1.5, (nq, 3)).astype(np.float32)
+        o = rng.uniform(-2, 2, (nr, 3))
+        rays = W.pack_rays(o, W.random_dirs(rng, nr), 1e-5, 1e7)
+        out.append({"verts": V, "tris": F, "queries": q, "rays": rays})
+    return out
+
+
+def _dev(meshes, dev):
+    return [{k: torch.from_numpy(v).to(dev) for k, v in m.items()} for m in meshes]
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_batch_matches_oracle_and_plain(ctx, dev, width):
+    host = _small_meshes()
+    dm = _dev(host, dev)
+    print("batch width", width, "meshes", len(host), flush=True)
+    batch = gpurt.Batch(ctx, dm, node_width=width)
+    res = batch.run()
+    torch.cuda.synchronize()
+    for i, (h, r) in enumerate(zip(host, res)):
+        ep, ept, ed, eb = O.cpq_brute_f32(h["verts"], h["tris"], h["queries"])
+        assert np.array_equal(r["q_prim"].cpu().numpy(), ep), i
+        assert np.array_equal(r["q_dist"].cpu().numpy(), ed), i
+        assert np.array_equal(r["q_pt"].cpu().numpy(), ept), i
+        assert np.array_equal(r["q_bary"].cpu().numpy(), eb), i
+        rp, rt, ruv = O.ray_closest_brute_f32(h["verts"], h["tris"], h["rays"])
+        assert np.array_equal(r["r_prim"].cpu().numpy(), rp), i
+        assert np.array_equal(r["r_t"].cpu().numpy(), rt), i
+        assert np.array_equal(r["r_uv"].cpu().numpy(), ruv), i
+        # the plain per-mesh path gives the same answers
+        bvh = gpurt.Bvh(ctx, dm[i]["verts"], dm[i]["tris"], node_width=width)
+        p, _, d, _ = bvh.closest_point(dm[i]["queries"])
+        assert torch.equal(p, r["q_prim"]) and torch.equal(d, r["q_dist"])
+
+
+def test_batch_replay_rebuilds_from_current_vertices(ctx, dev):
+    host = _small_meshes(seed=3)
+    dm = _dev(host, dev)
+    batch = gpurt.Batch(ctx, dm)
+    first = [{k: (v.clone() if v is not None else None) for k, v in r.items()} for r in batch.run()]
+    again = batch.run()
+    torch.cuda.synchronize()
+    for a, b in zip(first, again):
+        assert torch.equal(a["q_prim"], b["q_prim"]) and torch.equal(a["r_t"], b["r_t"])
+    # move the vertices in place: the next replay must rebuild every BVH from them
+    for m in dm:
+        m["verts"].mul_(1.25).add_(0.1)
+    res = batch.run()
+    torch.cuda.synchronize()
+    for m, r in zip(dm, res):
+        V = m["verts"].cpu().numpy()
+        F = m["tris"].cpu().numpy()
+        ep, _, ed, _ = O.cpq_brute_f32(V, F, m["queries"].cpu().numpy())
+        assert np.array_equal(r["q_prim"].cpu().numpy(), ep)
+        assert np.array_equal(r["q_dist"].cpu().numpy(), ed)
+        rp, rt, _ = O.ray_closest_brute_f32(V, F, m["rays"].cpu().numpy())
+        assert np.array_equal(r["r_prim"].cpu().numpy(), rp)
+        assert np.array_equal(r["r_t"].cpu().numpy(), rt)
+
+
+def test_batch_meshes_without_queries_or_rays(ctx, dev):
+    host = _small_meshes(seed=5)[:4]
+    host[1] = {"verts": host[1]["verts"], "tris": host[1]["tris"]}
+    host[2] = {"verts": host[2]["verts"], "tris": host[2]["tris"], "rays": host[2]["rays"]}
+    dm = _dev(host, dev)
+    res = gpurt.Batch(ctx, dm).run()
+    torch.cuda.synchronize()
+    assert res[1]["q_prim"].numel() == 0 and res[1]["r_prim"].numel() == 0
+    rp, rt, _ = O.ray_closest_brute_f32(host[2]["verts"], host[2]["tris"], host[2]["rays"])
+    assert np.array_equal(res[2]["r_prim"].cpu().numpy(), rp)
+
+
+def test_batch_rejects_bad_index(ctx, dev):
+    host = _small_meshes()[:3]
+    host[2]["tris"] = host[2]["tris"].copy()
+    host[2]["tris"][0, 0] = host[2]["verts"].shape[0] + 5
+    with pytest.raises(gpurt.GpurtError) as e:
+        gpurt.Batch(ctx, _dev(host, dev))
+    assert e.value.code == -1
