@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched-vs-plain leg (for profiling it)")
     ap.add_argument("--profile", action="store_true",
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -304,6 +306,9 @@ def main():
     ctx = gpurt.Context(0)
     if args.profile:
         profile_mode(args, torch, gpurt, W, dev, ctx)
+        return
+    if args.only_batched:
+        print(json.dumps(batched_bench(args, torch, gpurt, W, dev, ctx)), flush=True)
         return
     r = ray_bench(args, torch, gpurt, W, O, dev, ctx)
     extra = {"rays": r.pop("extra_rays")}

@@ -131,7 +131,7 @@ static int check_build_opts(const gpurt_build_opts* o, int& W, int& leaf, int& t
     if (leaf < 1 || leaf > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
     if (tsize != 0 && (tsize < 5 || tsize > 7))
         return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
-    if (trounds < 0 || trounds > 8) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0..8");
+    if (trounds < 0 || trounds > 1) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0 or 1");
     if (tsize && trounds == 0) trounds = 1;
     return GPURT_OK;
 }

@@ -42,10 +42,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     const void* nodes_v, const float4* __restrict__ tris, const BatchMeshDev* __restrict__ meshes,
     const uint2* __restrict__ chunks, uint32_t n_chunks, uint32_t* counter, uint2* spill,
     uint32_t spill_slots, uint32_t* err) {
-    __shared__ uint2 stk[S * BS];
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(nodes_v);
     LaneStack<S, BS> st;
-    st.lds = stk + threadIdx.x;
+    st.lane = threadIdx.x;
     st.spill = spill;
     st.gid = blockIdx.x * BS + threadIdx.x;
     st.total = gridDim.x * BS;
@@ -54,7 +53,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     unsigned long long dn = 0, dt = 0;
     for (;;) {
         uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(counter, 1u);
+        if (lane == 0) c = gatomic_add(counter, 1u);
         // wave-uniform from here on: the chunk and mesh descriptors live in SGPRs
         c = __builtin_amdgcn_readfirstlane(c);
         if (c >= n_chunks) break;
@@ -69,7 +68,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
                 RayHit h;
                 if (!ray_one<W, S, BS, false, false>(nodes, tris, m, v3(r0.x, r0.y, r0.z),
                                                      v3(r1.x, r1.y, r1.z), r0.w, r1.w, st, h, dn, dt))
-                    atomicOr(err, 1u);
+                    gatomic_or(err, 1u);
                 const bool hit = h.id != 0xFFFFFFFFu;
                 md.r_prim[i] = hit ? (int32_t)h.id : -1;
                 md.r_t[i] = hit ? h.t : INFINITY;
@@ -82,7 +81,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
             if (i < md.n_q) {
                 const V3 q = v3(md.q[3 * (size_t)i], md.q[3 * (size_t)i + 1], md.q[3 * (size_t)i + 2]);
                 CpqHit h;
-                if (!cpq_one<W, S, BS, false>(nodes, tris, m, q, st, h, dn, dt)) atomicOr(err, 1u);
+                if (!cpq_one<W, S, BS, false>(nodes, tris, m, q, st, h, dn, dt)) gatomic_or(err, 1u);
                 md.q_prim[i] = (int32_t)h.id;
                 md.q_pt[3 * (size_t)i] = h.cp.x;
                 md.q_pt[3 * (size_t)i + 1] = h.cp.y;
@@ -102,7 +101,7 @@ template <int W, int S, int BS>
 static void launch_batch_query(unsigned grid, hipStream_t s, const void* nodes, const float4* tris,
                                const BatchMeshDev* meshes, const uint2* chunks, uint32_t n_chunks,
                                uint32_t* counter, uint2* spill, uint32_t spill_slots, uint32_t* err) {
-    hipLaunchKernelGGL((k_batch_query<W, S, BS>), dim3(grid), dim3(BS), 0, s, nodes, tris, meshes, chunks,
+    hipLaunchKernelGGL((k_batch_query<W, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, nodes, tris, meshes, chunks,
                        n_chunks, counter, spill, spill_slots, err);
 }
 
@@ -172,9 +171,9 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, b->counter_d);
     int bs = 0;
     switch (b->width) {
-        case 2: bs = launch_batch_w<2>(b->stack, b->grid, s, b->build.nodes, b->build.tris, b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
-        case 4: bs = launch_batch_w<4>(b->stack, b->grid, s, b->build.nodes, b->build.tris, b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
-        case 8: bs = launch_batch_w<8>(b->stack, b->grid, s, b->build.nodes, b->build.tris, b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
+        case 2: bs = launch_batch_w<2>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
+        case 4: bs = launch_batch_w<4>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
+        case 8: bs = launch_batch_w<8>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
     }
     if (!bs) return set_error(GPURT_E_UNSUPPORTED, "batch stack/width");
     GPURT_HIP(hipGetLastError());
@@ -202,7 +201,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     if (leaf < 1 || leaf > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
     if (tsize != 0 && (tsize < 5 || tsize > 7))
         return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
-    if (trounds < 0 || trounds > 8) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0..8");
+    if (trounds < 0 || trounds > 1) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0 or 1");
     if (tsize && !trounds) trounds = 1;
     int S = (qopts && qopts->stack_depth) ? qopts->stack_depth : (W == 8 ? 16 : 8);
     if (S != 8 && S != 16 && S != 32 && S != 64)
@@ -247,7 +246,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->width = W;
     b->leaf_size = leaf;
     b->stack = S;
-    int rc = build_alloc(segs.data(), (int)n_meshes, W, b->build);
+    int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {
         delete b;
         return rc;

@@ -15,7 +15,7 @@
 //   k_karras       Karras (HPG 2012) binary radix tree over the sorted keys; because the
 //                  mesh id is in the high key bits every mesh is one subtree, whose root
 //                  is recorded
-//   k_bottom_up    node AABBs by atomic-counter bottom-up climb (second arrival merges)
+//   k_boxes_level  node AABBs bottom-up, one launch per common-prefix length (no atomics)
 //   k_treelet      optional SAH treelet restructuring (treelet.hip)
 //   k_collapse     one launch per wide level: a node opens the child with the largest
 //                  surface area until it has W children; subtrees of <= leaf_size
@@ -24,22 +24,13 @@
 #include <cstring>
 #include <vector>
 
-#include "gpurt_internal.h"
+#include "build_common.h"
 
 namespace gpurt_impl {
 
 // ------------------------------------------------------------------------------------
 // helpers
 // ------------------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t f2o(float f) {
-    uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float o2f(uint32_t u) {
-    u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
-    return __uint_as_float(u);
-}
 
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t x) {
     x &= 0x3ffu;
@@ -77,8 +68,6 @@ __device__ __forceinline__ int find_seg(const SegDesc* segs, int n_seg, uint32_t
     return lo;
 }
 
-// per-mesh statistics: [cmin xyz, cmax xyz, max|coord|, unused] as ordered uints
-constexpr int kStat = 8;
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,20 +75,36 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void k_init_build(const SegDesc* segs, int n_seg, uint32_t* segstat, uint32_t* seg_root,
+__global__ void k_init_build(const SegDesc* segs, int n_seg, uint32_t* segstat_rep, uint32_t* seg_root,
                              uint32_t* lvl_cnt, uint32_t* counters) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_seg) {
+    if (i < n_seg * kRep) {
+        uint32_t* st = segstat_rep + (size_t)i * kStat;
         for (int k = 0; k < 3; k++) {
-            segstat[i * kStat + k] = 0xFFFFFFFFu;
-            segstat[i * kStat + 3 + k] = 0u;
+            st[k] = 0xFFFFFFFFu;
+            st[3 + k] = 0u;
         }
-        segstat[i * kStat + 6] = 0u;
-        segstat[i * kStat + 7] = 0u;
-        seg_root[i] = kLeafBit | segs[i].tri_offset;  // single-triangle mesh: leaf root
+        st[6] = 0u;
+        st[7] = 0u;
     }
+    if (i < n_seg) seg_root[i] = kLeafBit | segs[i].tri_offset;  // single-triangle mesh: leaf root
     if (i <= kMaxLevels) lvl_cnt[i] = (i == 0) ? (uint32_t)n_seg : 0u;
     if (i < 4) counters[i] = (i == 0) ? (uint32_t)n_seg : 0u;  // [0] wide nodes, [1] error
+}
+
+// folds the kRep replicas of every mesh's statistics into segstat
+__global__ void k_reduce_stats(int n_seg, const uint32_t* __restrict__ segstat_rep,
+                               uint32_t* __restrict__ segstat) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_seg * kStat) return;
+    const int sgi = i / kStat, k = i % kStat;
+    const bool is_min = k < 3;
+    uint32_t v = is_min ? 0xFFFFFFFFu : 0u;
+    for (int r = 0; r < kRep; r++) {
+        uint32_t x = segstat_rep[((size_t)r * n_seg + sgi) * kStat + k];
+        v = is_min ? min(v, x) : max(v, x);
+    }
+    segstat[i] = v;
 }
 
 // ------------------------------------------------------------------------------------
@@ -148,7 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_prep(const SegDesc* __restrict__ seg
         tri_v[3 * (size_t)g + 1] = make_float4(v[1][0], v[1][1], v[1][2], 0.0f);
         tri_v[3 * (size_t)g + 2] = make_float4(v[2][0], v[2][1], v[2][2], 0.0f);
     }
-    // reduce over the wave when it covers one mesh, else per lane
+    // reduce over the wave when it covers one mesh, else per lane; this block's replica
+    segstat += (size_t)(blockIdx.x % kRep) * n_seg * kStat;
     int s0 = __shfl(s, 0);
     bool uniform = __all(s == s0 || !valid);
     uint32_t o[7];
@@ -224,14 +230,34 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restric
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of `total` counts in one block of 1024 threads
-__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ a, uint32_t total) {
-    __shared__ uint32_t part[1024];
-    uint32_t chunk = (total + 1023) / 1024;
-    uint32_t beg = threadIdx.x * chunk;
-    uint32_t end = min(beg + chunk, total);
+// Exclusive scan of `total` counts in three launches: per-tile sums, one block scanning
+// the tile sums, then per-tile scans offset by their tile's prefix.
+constexpr int kScanTile = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* __restrict__ a, uint32_t total,
+                                                       uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t red[kBlock];
+    const uint32_t base = blockIdx.x * kScanTile;
     uint32_t s = 0;
-    for (uint32_t i = beg; i < end; i++) s += a[i];
+    for (int k = 0; k < kScanTile / kBlock; k++) {
+        uint32_t i = base + k * kBlock + threadIdx.x;
+        if (i < total) s += a[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ t, uint32_t n) {
+    __shared__ uint32_t part[1024];
+    uint32_t chunk = (n + 1023) / 1024;
+    uint32_t beg = threadIdx.x * chunk, end = min(beg + chunk, n);
+    uint32_t s = 0;
+    for (uint32_t i = beg; i < end; i++) s += t[i];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -242,9 +268,38 @@ __global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ a, ui
     }
     uint32_t run = part[threadIdx.x] - s;
     for (uint32_t i = beg; i < end; i++) {
-        uint32_t x = a[i];
-        a[i] = run;
+        uint32_t x = t[i];
+        t[i] = run;
         run += x;
+    }
+}
+
+// one tile of kScanTile entries per block: 16 consecutive entries per thread
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ a, uint32_t total,
+                                                       const uint32_t* __restrict__ tile_off) {
+    __shared__ uint32_t part[kBlock];
+    constexpr int kPer = kScanTile / kBlock;
+    const uint32_t beg = blockIdx.x * kScanTile + threadIdx.x * kPer;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        v[k] = (beg + k < total) ? a[beg + k] : 0u;
+        s += v[k];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = tile_off[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        if (beg + k < total) a[beg + k] = run;
+        run += v[k];
     }
 }
 
@@ -335,7 +390,9 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
                                                    uint32_t* __restrict__ node_parent,
                                                    uint32_t* __restrict__ leaf_parent,
                                                    uint32_t* __restrict__ seg_root,
-                                                   uint8_t* __restrict__ is_root) {
+                                                   uint8_t* __restrict__ is_root,
+                                                   uint8_t* __restrict__ dlev,
+                                                   uint32_t* __restrict__ cnt) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= N - 1) return;
     int d = (kdelta(keys, N, i, i + 1) - kdelta(keys, N, i, i - 1)) >= 0 ? 1 : -1;
@@ -359,6 +416,8 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
     left[i] = L;
     right[i] = R;
     range[i] = make_uint2((uint32_t)first, (uint32_t)last);
+    dlev[i] = (uint8_t)dnode;
+    cnt[i] = (uint32_t)(last - first + 1);
     if (L & kLeafBit) leaf_parent[split] = (uint32_t)i;
     else node_parent[split] = (uint32_t)i;
     if (R & kLeafBit) leaf_parent[split + 1] = (uint32_t)i;
@@ -374,73 +433,32 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
 }
 
 // ------------------------------------------------------------------------------------
-// k_bottom_up: internal node boxes
+// internal node boxes and SAH costs
 // ------------------------------------------------------------------------------------
 
-__device__ __forceinline__ void ref_box(uint32_t r, const float4* __restrict__ saabb,
-                                        const float4* __restrict__ nbox, float4& lo, float4& hi) {
-    if (r & kLeafBit) {
-        size_t p = r & ~kLeafBit;
-        lo = saabb[2 * p];
-        hi = saabb[2 * p + 1];
-    } else {
-        lo = nbox[2 * (size_t)r];
-        hi = nbox[2 * (size_t)r + 1];
-    }
-}
-
-__device__ __forceinline__ void ref_box_coherent(uint32_t r, const float4* saabb, const float4* nbox,
-                                                 float4& lo, float4& hi) {
-    if (r & kLeafBit) {
-        size_t p = r & ~kLeafBit;
-        lo = saabb[2 * p];
-        hi = saabb[2 * p + 1];
-    } else {
-        // written by another thread of this launch: bypass non-coherent caches
-        const float* b = reinterpret_cast<const float*>(nbox + 2 * (size_t)r);
-        lo.x = __hip_atomic_load(b + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lo.y = __hip_atomic_load(b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lo.z = __hip_atomic_load(b + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lo.w = 0.0f;
-        hi.x = __hip_atomic_load(b + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hi.y = __hip_atomic_load(b + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hi.z = __hip_atomic_load(b + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hi.w = 0.0f;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_bottom_up(int N, const uint32_t* __restrict__ leaf_parent,
-                                                      const uint32_t* __restrict__ node_parent,
-                                                      const uint32_t* __restrict__ left,
-                                                      const uint32_t* __restrict__ right,
-                                                      const uint8_t* __restrict__ is_root,
-                                                      const float4* __restrict__ saabb,
-                                                      float4* nbox, uint32_t* flags) {
-    int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= N) return;
-    uint32_t cur = leaf_parent[p];
-    if (cur == kEmpty) return;  // single-triangle mesh
-    // each internal node is merged by the second of its two children to arrive
-    for (int guard = 0; guard < 256; guard++) {
-        __threadfence();
-        uint32_t old = atomicAdd(&flags[cur], 1u);
-        if (old == 0u) return;
-        __threadfence();
+// Node boxes without atomics or fences: a child's common-prefix length (Karras' delta
+// of its range) is strictly greater than its parent's, so launching once per delta
+// value from the largest down makes every child box final before its parent reads it.
+// Each launch scans the delta byte of every internal node and merges the matching ones.
+__global__ __launch_bounds__(kBlock) void k_boxes_level(int n_int, int level, const uint8_t* __restrict__ dlev,
+                                                        const uint32_t* __restrict__ left,
+                                                        const uint32_t* __restrict__ right,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const float4* __restrict__ saabb,
+                                                        float4* __restrict__ nbox, float* __restrict__ cost,
+                                                        int leaf_size) {
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_int; i += stride) {
+        if (dlev[i] != (uint8_t)level) continue;
         float4 al, ah, bl, bh;
-        ref_box_coherent(left[cur], saabb, nbox, al, ah);
-        ref_box_coherent(right[cur], saabb, nbox, bl, bh);
-        float* o = reinterpret_cast<float*>(nbox + 2 * (size_t)cur);
-        __hip_atomic_store(o + 0, fminf(al.x, bl.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 1, fminf(al.y, bl.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 2, fminf(al.z, bl.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 4, fmaxf(ah.x, bh.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 5, fmaxf(ah.y, bh.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 6, fmaxf(ah.z, bh.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(o + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (is_root[cur]) return;
-        cur = node_parent[cur];
-        if (cur == kEmpty) return;
+        const uint32_t L = left[i], R = right[i];
+        ref_box(L, saabb, nbox, al, ah);
+        ref_box(R, saabb, nbox, bl, bh);
+        const float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+        const float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+        nbox[2 * (size_t)i] = lo;
+        nbox[2 * (size_t)i + 1] = hi;
+        cost[i] = sah_node_cost(lo, hi, cnt[i], ref_cost(L, saabb, cost), ref_cost(R, saabb, cost), leaf_size);
     }
 }
 
@@ -452,31 +470,58 @@ struct CollapseArgs {
     const uint32_t* left;
     const uint32_t* right;
     const uint2* range;
+    const uint32_t* cnt;  // triangles under each internal node
     const float4* saabb;
     const float4* nbox;
     const uint32_t* segstat;
     uint32_t* wsrc;  // wide node -> binary ref
     uint32_t* wseg;  // wide node -> mesh
     uint32_t* lvl_cnt;
-    uint32_t* counters;  // [0] wide nodes allocated, [1] error bits
+    uint32_t* counters;  // [0] wide nodes allocated, [1] error bits, [2] triangles re-packed
     void* nodes;
     uint32_t capacity;
     int leaf_size;
     int level;
+    // re-pack (after treelet restructuring a leaf's triangles are no longer one sorted
+    // range): leaves get fresh contiguous slots and their triangles are copied there
+    int repack;
+    const float4* tris_in;
+    const uint32_t* order_in;
+    float4* tris_out;
+    uint32_t* order_out;
 };
 
-__device__ __forceinline__ uint32_t ref_count(uint32_t r, const uint2* range) {
-    if (r & kLeafBit) return 1u;
-    uint2 fl = range[r];
-    return fl.y - fl.x + 1u;
+__device__ __forceinline__ uint32_t ref_count(uint32_t r, const uint32_t* cnt) {
+    return (r & kLeafBit) ? 1u : cnt[r];
 }
 __device__ __forceinline__ uint32_t ref_first(uint32_t r, const uint2* range) {
     if (r & kLeafBit) return r & ~kLeafBit;
     return range[r].x;
 }
-__device__ __forceinline__ float box_area(float4 lo, float4 hi) {
-    float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
-    return 2.0f * (ex * ez + ex * ey + ey * ez);
+
+// Copies the <= leaf_size triangles under binary ref r to fresh slots; returns the
+// first slot.
+__device__ uint32_t repack_leaf(const CollapseArgs& a, uint32_t r, uint32_t count) {
+    const uint32_t base = gatomic_add(&a.counters[2], count);
+    uint32_t stack[8];
+    int sp = 0;
+    uint32_t out = base;
+    stack[sp++] = r;
+    while (sp > 0) {
+        const uint32_t x = stack[--sp];
+        if (x & kLeafBit) {
+            const size_t p = x & ~kLeafBit;
+            a.tris_out[3 * (size_t)out] = a.tris_in[3 * p];
+            a.tris_out[3 * (size_t)out + 1] = a.tris_in[3 * p + 1];
+            a.tris_out[3 * (size_t)out + 2] = a.tris_in[3 * p + 2];
+            a.order_out[out] = a.order_in[p];
+            out++;
+        } else if (sp + 2 <= 8) {
+            stack[sp++] = a.right[x];
+            stack[sp++] = a.left[x];
+        }
+    }
+    return base;
 }
 
 template <int W>
@@ -492,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
         float pad = o2f(a.segstat[seg * kStat + 6]) * 0x1p-18f;
         uint32_t refs[W];
         int n = 0;
-        if (ref_count(bref, a.range) <= (uint32_t)a.leaf_size) {
+        if (ref_count(bref, a.cnt) <= (uint32_t)a.leaf_size) {
             refs[n++] = bref;
         } else {
             refs[n++] = a.left[bref];
@@ -504,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                 for (int j = 0; j < W; j++) {
                     if (j >= n) break;
                     uint32_t r = refs[j];
-                    if (ref_count(r, a.range) <= (uint32_t)a.leaf_size) continue;
+                    if (ref_count(r, a.cnt) <= (uint32_t)a.leaf_size) continue;
                     float4 lo, hi;
                     ref_box(r, a.saabb, a.nbox, lo, hi);
                     float ar = box_area(lo, hi);
@@ -526,10 +571,14 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                 ref_box(r, a.saabb, a.nbox, lo, hi);
                 out.lox[j] = lo.x - pad; out.loy[j] = lo.y - pad; out.loz[j] = lo.z - pad;
                 out.hix[j] = hi.x + pad; out.hiy[j] = hi.y + pad; out.hiz[j] = hi.z + pad;
-                uint32_t c = ref_count(r, a.range);
+                uint32_t c = ref_count(r, a.cnt);
                 out.aux[j] = c;
-                if (c <= (uint32_t)a.leaf_size) out.child[j] = make_leaf(ref_first(r, a.range), c);
-                else n_internal++;
+                if (c <= (uint32_t)a.leaf_size) {
+                    const uint32_t first = a.repack ? repack_leaf(a, r, c) : ref_first(r, a.range);
+                    out.child[j] = make_leaf(first, c);
+                } else {
+                    n_internal++;
+                }
             } else {
                 out.lox[j] = out.loy[j] = out.loz[j] = INFINITY;
                 out.hix[j] = out.hiy[j] = out.hiz[j] = -INFINITY;
@@ -538,8 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
             }
         }
         if (n_internal) {
-            uint32_t base = atomicAdd(&a.counters[0], n_internal);
-            atomicAdd(&a.lvl_cnt[a.level + 1], n_internal);
+            uint32_t base = gatomic_add(&a.counters[0], n_internal);
+            gatomic_add(&a.lvl_cnt[a.level + 1], n_internal);
             uint32_t m = 0;
 #pragma unroll
             for (int j = 0; j < W; j++) {
@@ -550,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                         a.wseg[idx] = seg;
                         out.child[j] = idx;
                     } else {
-                        atomicOr(&a.counters[1], 2u);
+                        gatomic_or(&a.counters[1], 2u);
                         out.child[j] = kEmpty;
                     }
                 }
@@ -585,20 +634,54 @@ void* BuildBufs::release(void* p) {
     return p;
 }
 
-template <typename T>
-static int balloc(BuildBufs& b, T*& p, size_t count) {
-    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    void* q = nullptr;
-    if (hipMalloc(&q, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        return set_error(GPURT_E_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+// All scratch of a build lives in one allocation carved into 256-byte aligned pieces
+// (one hipMalloc instead of ~25); the products handed to a BVH are separate.
+struct Carver {
+    size_t off = 0;
+    char* base = nullptr;
+    template <typename T>
+    void take(T*& p, size_t count) {
+        size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        off = (off + 255) & ~size_t(255);
+        p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += bytes;
     }
-    b.allocs.push_back(q);
-    p = static_cast<T*>(q);
-    return GPURT_OK;
+};
+
+static void carve_all(BuildBufs& b, Carver& c, size_t NI) {
+    const uint32_t N = b.N;
+    const int n_seg = b.n_seg;
+    c.take(b.segs, n_seg);
+    c.take(b.aabb_lo, N);
+    c.take(b.aabb_hi, N);
+    c.take(b.tri_v, 3 * (size_t)N);
+    c.take(b.segstat_rep, (size_t)kStat * n_seg * kRep);
+    c.take(b.segstat, (size_t)kStat * n_seg);
+    c.take(b.keys0, N);
+    c.take(b.keys1, N);
+    c.take(b.vals0, N);
+    c.take(b.vals1, N);
+    c.take(b.hist, 256 * (size_t)b.nblocks_sort);
+    c.take(b.scan_tmp, cdiv(256 * (size_t)b.nblocks_sort, kScanTile));
+    c.take(b.saabb, 2 * (size_t)N);
+    c.take(b.left, NI);
+    c.take(b.right, NI);
+    c.take(b.range, NI);
+    c.take(b.nparent, NI);
+    c.take(b.lparent, N);
+    c.take(b.seg_root, n_seg);
+    c.take(b.is_root, NI);
+    c.take(b.dlev, NI);
+    c.take(b.cnt, NI);
+    c.take(b.cost, NI);
+    c.take(b.nbox, 2 * NI);
+    c.take(b.wsrc, b.capacity);
+    c.take(b.wseg, b.capacity);
+    c.take(b.lvl, kMaxLevels + 1);
+    c.take(b.counters, 4);
 }
 
-int build_alloc(const SegDesc* host_segs, int n_seg, int W, BuildBufs& b) {
+int build_alloc(const SegDesc* host_segs, int n_seg, int W, int leaf_size, int treelet_size, BuildBufs& b) {
     uint64_t N64 = 0;
     for (int i = 0; i < n_seg; i++) {
         if (host_segs[i].n_tris == 0) return set_error(GPURT_E_INVALID, "a mesh has no triangles");
@@ -610,30 +693,52 @@ int build_alloc(const SegDesc* host_segs, int n_seg, int W, BuildBufs& b) {
     b.N = N;
     b.n_seg = n_seg;
     b.width = W;
+    b.leaf_size = leaf_size;
     b.nblocks_sort = cdiv(N, kSortTile);
     const size_t NI = N > 1 ? N - 1 : 1;
     b.capacity = (uint32_t)(NI + n_seg);
-    int rc;
-    if ((rc = balloc(b, b.segs, n_seg)) || (rc = balloc(b, b.aabb_lo, N)) ||
-        (rc = balloc(b, b.aabb_hi, N)) || (rc = balloc(b, b.tri_v, 3 * (size_t)N)) ||
-        (rc = balloc(b, b.segstat, (size_t)kStat * n_seg)) || (rc = balloc(b, b.keys0, N)) ||
-        (rc = balloc(b, b.keys1, N)) || (rc = balloc(b, b.vals0, N)) || (rc = balloc(b, b.vals1, N)) ||
-        (rc = balloc(b, b.hist, 256 * (size_t)b.nblocks_sort)) || (rc = balloc(b, b.saabb, 2 * (size_t)N)) ||
-        (rc = balloc(b, b.left, NI)) || (rc = balloc(b, b.right, NI)) || (rc = balloc(b, b.range, NI)) ||
-        (rc = balloc(b, b.nparent, NI)) || (rc = balloc(b, b.lparent, N)) ||
-        (rc = balloc(b, b.seg_root, n_seg)) || (rc = balloc(b, b.is_root, NI)) ||
-        (rc = balloc(b, b.nbox, 2 * NI)) || (rc = balloc(b, b.flags, NI)) ||
-        (rc = balloc(b, b.wsrc, b.capacity)) || (rc = balloc(b, b.wseg, b.capacity)) ||
-        (rc = balloc(b, b.lvl, kMaxLevels + 1)) || (rc = balloc(b, b.counters, 4)) ||
-        (rc = balloc(b, b.tris, 3 * (size_t)N)) || (rc = balloc(b, b.leaf_order, N)))
-        return rc;
-    void* nodes = nullptr;
-    if (hipMalloc(&nodes, node_bytes(W) * b.capacity) != hipSuccess) {
+    Carver sizing;
+    carve_all(b, sizing, NI);
+    void* scratch = nullptr;
+    if (hipMalloc(&scratch, sizing.off) != hipSuccess) {
         (void)hipGetLastError();
-        return set_error(GPURT_E_OOM, "hipMalloc of BVH nodes failed");
+        return set_error(GPURT_E_OOM, "hipMalloc of " + std::to_string(sizing.off) + " bytes of build scratch failed");
+    }
+    b.allocs.push_back(scratch);
+    Carver c;
+    c.base = static_cast<char*>(scratch);
+    carve_all(b, c, NI);
+    void* nodes = nullptr;
+    void* tris = nullptr;
+    void* order = nullptr;
+    if (hipMalloc(&nodes, node_bytes(W) * b.capacity) != hipSuccess ||
+        hipMalloc(&tris, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
+        hipMalloc(&order, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
+        (void)hipGetLastError();
+        for (void* p : {nodes, tris, order})
+            if (p) (void)hipFree(p);
+        return set_error(GPURT_E_OOM, "hipMalloc of BVH storage failed");
     }
     b.allocs.push_back(nodes);
+    b.allocs.push_back(tris);
+    b.allocs.push_back(order);
     b.nodes = nodes;
+    b.tris = static_cast<float4*>(tris);
+    b.leaf_order = static_cast<uint32_t*>(order);
+    if (treelet_size > 0) {
+        void* t2 = nullptr;
+        void* o2 = nullptr;
+        if (hipMalloc(&t2, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
+            hipMalloc(&o2, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
+            (void)hipGetLastError();
+            if (t2) (void)hipFree(t2);
+            return set_error(GPURT_E_OOM, "hipMalloc of re-pack buffers failed");
+        }
+        b.allocs.push_back(t2);
+        b.allocs.push_back(o2);
+        b.tris2 = static_cast<float4*>(t2);
+        b.leaf_order2 = static_cast<uint32_t*>(o2);
+    }
     GPURT_HIP(hipMemcpy(b.segs, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice));
     return GPURT_OK;
 }
@@ -645,16 +750,16 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
     const int n_seg = b.n_seg;
     const size_t NI = N > 1 ? N - 1 : 1;
     const unsigned gN = cdiv(N, kBlock);
-    hipLaunchKernelGGL(k_init_build, dim3(cdiv(std::max(n_seg, kMaxLevels + 1), kBlock)), dim3(kBlock),
-                       0, s, b.segs, n_seg, b.segstat, b.seg_root, b.lvl, b.counters);
+    hipLaunchKernelGGL(k_init_build, dim3(cdiv(std::max(n_seg * kRep, kMaxLevels + 1), kBlock)),
+                       dim3(kBlock), 0, s, b.segs, n_seg, b.segstat_rep, b.seg_root, b.lvl, b.counters);
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(gN, 4096u)), dim3(kBlock), 0, s, b.lparent, (size_t)N,
                        kEmpty);
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s,
                        b.nparent, NI, kEmpty);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s, b.flags,
-                       NI, 0u);
     hipLaunchKernelGGL(k_prep, dim3(gN), dim3(kBlock), 0, s, b.segs, n_seg, N, b.aabb_lo, b.aabb_hi,
-                       b.tri_v, b.segstat, b.counters);
+                       b.tri_v, b.segstat_rep, b.counters);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(cdiv((size_t)n_seg * kStat, kBlock)), dim3(kBlock), 0, s, n_seg,
+                       b.segstat_rep, b.segstat);
     hipLaunchKernelGGL(k_morton, dim3(gN), dim3(kBlock), 0, s, N, b.aabb_lo, b.aabb_hi, b.segstat,
                        b.keys0, b.vals0);
     // radix sort over the bits in use: 30 Morton bits + mesh-id bits
@@ -665,10 +770,14 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
     uint64_t* kout = b.keys1;
     uint32_t* vin = b.vals0;
     uint32_t* vout = b.vals1;
+    const uint32_t hist_n = 256u * b.nblocks_sort;
+    const unsigned scan_tiles = cdiv(hist_n, kScanTile);
     for (int shift = 0; shift < key_bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, N, shift, b.hist,
                            b.nblocks_sort);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, b.hist, 256u * b.nblocks_sort);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(kBlock), 0, s, b.hist, hist_n, b.scan_tmp);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, b.scan_tmp, scan_tiles);
+        hipLaunchKernelGGL(k_scan_apply, dim3(scan_tiles), dim3(kBlock), 0, s, b.hist, hist_n, b.scan_tmp);
         hipLaunchKernelGGL(k_radix_scatter, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, vin, kout,
                            vout, N, shift, b.hist, b.nblocks_sort);
         std::swap(kin, kout);
@@ -677,13 +786,24 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
     hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, vin, b.aabb_lo, b.aabb_hi, b.tri_v,
                        b.saabb, b.tris, b.leaf_order);
     if (N > 1) {
+        const int n_int = (int)(N - 1);
         hipLaunchKernelGGL(k_karras, dim3(cdiv(N - 1, kBlock)), dim3(kBlock), 0, s, (int)N, kin, b.segs,
-                           b.left, b.right, b.range, b.nparent, b.lparent, b.seg_root, b.is_root);
-        hipLaunchKernelGGL(k_bottom_up, dim3(gN), dim3(kBlock), 0, s, (int)N, b.lparent, b.nparent,
-                           b.left, b.right, b.is_root, b.saabb, b.nbox, b.flags);
+                           b.left, b.right, b.range, b.nparent, b.lparent, b.seg_root, b.is_root, b.dlev,
+                           b.cnt);
+        // delta values in use: distinct keys give [64 - key_bits, 63]; equal keys give
+        // 64 + clz32(i ^ j) with i, j < N, i.e. [96 - ceil(log2 N), 95]
+        int nbits = 0;
+        while ((1u << nbits) < N) nbits++;
+        const unsigned lgrid = std::min(cdiv(n_int, kBlock), 2048u);
+        for (int level = 95; level >= 64 - key_bits; level--) {
+            if (level >= 64 && level < 96 - nbits) continue;
+            hipLaunchKernelGGL(k_boxes_level, dim3(lgrid), dim3(kBlock), 0, s, n_int, level, b.dlev, b.left,
+                               b.right, b.cnt, b.saabb, b.nbox, b.cost, b.leaf_size);
+        }
         if (treelet_size > 0) {
-            int trc = run_treelets(b, treelet_size, treelet_rounds, s);
+            int trc = run_treelets(b, treelet_size, treelet_rounds, key_bits, nbits, s);
             if (trc != GPURT_OK) return trc;
+            b.repacked = true;
         }
     }
     hipLaunchKernelGGL(k_init_levels, dim3(cdiv(n_seg, kBlock)), dim3(kBlock), 0, s, n_seg, b.seg_root,
@@ -708,6 +828,7 @@ int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level
     ca.left = b.left;
     ca.right = b.right;
     ca.range = b.range;
+    ca.cnt = b.cnt;
     ca.saabb = b.saabb;
     ca.nbox = b.nbox;
     ca.segstat = b.segstat;
@@ -718,6 +839,11 @@ int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level
     ca.nodes = b.nodes;
     ca.capacity = b.capacity;
     ca.leaf_size = leaf_size;
+    ca.repack = b.repacked ? 1 : 0;
+    ca.tris_in = b.tris;
+    ca.order_in = b.leaf_order;
+    ca.tris_out = b.tris2;
+    ca.order_out = b.leaf_order2;
     const unsigned cgrid = std::min(cdiv(b.capacity, kBlock), 2048u);
     for (int level = level_begin; level < level_end; level++) {
         ca.level = level;
@@ -734,7 +860,7 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     (void)ctx;
     BuildBufs b;
     int rc;
-    if ((rc = build_alloc(host_segs, n_seg, W, b))) return rc;
+    if ((rc = build_alloc(host_segs, n_seg, W, leaf_size, treelet_size, b))) return rc;
     if ((rc = build_enqueue_binary(b, treelet_size, treelet_rounds, s))) return rc;
     std::vector<uint32_t> h_lvl(kMaxLevels + 1, 0);
     int level = 0;
@@ -780,8 +906,9 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     float scale;
     std::memcpy(&scale, &u, 4);
     out->nodes = nodes;
-    out->tris = (float4*)b.release(b.tris);
-    out->leaf_order = (uint32_t*)b.release(b.leaf_order);
+    if (b.repacked && h_counters[2] != b.N) return set_error(GPURT_E_DEVICE, "leaf re-pack lost triangles");
+    out->tris = (float4*)b.release(b.final_tris());
+    out->leaf_order = (uint32_t*)b.release(b.final_order());
     out->n_nodes = n_nodes;
     out->node_capacity = n_nodes;
     out->depth = depth;

@@ -52,6 +52,19 @@ static_assert(sizeof(WNode<8>) == 256, "node size");
 
 inline size_t node_bytes(int W) { return (size_t)32 * (size_t)W; }
 
+// Atomics on global memory through a global-address-space pointer: no flat-address
+// (LDS / scratch aperture) checks are generated around them.
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add((global_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gatomic_or(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_or((global_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gatomic_max_u32(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_max((global_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One mesh of a (possibly single-mesh) build.
 struct SegDesc {
     const float* verts;
@@ -117,11 +130,15 @@ struct BuildBufs {
     uint32_t N = 0;  // triangles over all meshes
     int n_seg = 0;   // meshes
     int width = 4;
+    int leaf_size = 4;
+    bool repacked = false;  // treelets changed the tree: leaves re-packed into tris2
     uint32_t nblocks_sort = 0;
     uint32_t capacity = 0;  // wide nodes the node array can hold
     SegDesc* segs = nullptr;
     float4 *aabb_lo = nullptr, *aabb_hi = nullptr, *tri_v = nullptr;
-    uint32_t* segstat = nullptr;
+    uint32_t* segstat_rep = nullptr;  // per-mesh statistics, replicated for atomics
+    uint32_t* segstat = nullptr;      // per-mesh statistics, reduced
+    uint32_t* scan_tmp = nullptr;
     uint64_t *keys0 = nullptr, *keys1 = nullptr;
     uint32_t *vals0 = nullptr, *vals1 = nullptr, *hist = nullptr;
     float4* saabb = nullptr;  // sorted triangle boxes (lo, hi)
@@ -129,11 +146,18 @@ struct BuildBufs {
     uint2* range = nullptr;
     uint32_t *nparent = nullptr, *lparent = nullptr, *seg_root = nullptr;
     uint8_t* is_root = nullptr;
-    float4* nbox = nullptr;  // binary node boxes (lo, hi)
-    uint32_t *flags = nullptr, *wsrc = nullptr, *wseg = nullptr, *lvl = nullptr, *counters = nullptr;
+    uint8_t* dlev = nullptr;  // common-prefix length (Karras delta) of each internal node
+    uint32_t* cnt = nullptr;  // triangles under each internal node
+    float* cost = nullptr;    // SAH cost of each internal node's subtree
+    float4* nbox = nullptr;   // binary node boxes (lo, hi)
+    uint32_t *wsrc = nullptr, *wseg = nullptr, *lvl = nullptr, *counters = nullptr;
     void* nodes = nullptr;
-    float4* tris = nullptr;
-    uint32_t* leaf_order = nullptr;
+    float4* tris = nullptr;          // triangles in sorted (Morton) order
+    uint32_t* leaf_order = nullptr;  // sorted position -> triangle
+    float4* tris2 = nullptr;         // re-packed leaves (treelet builds only)
+    uint32_t* leaf_order2 = nullptr;
+    float4* final_tris() const { return repacked ? tris2 : tris; }
+    uint32_t* final_order() const { return repacked ? leaf_order2 : leaf_order; }
     std::vector<void*> allocs;  // freed with the object unless release()d
     BuildBufs() = default;
     BuildBufs(const BuildBufs&) = delete;
@@ -141,10 +165,11 @@ struct BuildBufs {
     ~BuildBufs();
     void* release(void* p);  // hands ownership of one allocation to the caller
 };
-int build_alloc(const SegDesc* host_segs, int n_seg, int width, BuildBufs& b);
+int build_alloc(const SegDesc* host_segs, int n_seg, int width, int leaf_size, int treelet_size,
+                BuildBufs& b);
 int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hipStream_t s);
 int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level_end, hipStream_t s);
-int run_treelets(BuildBufs& b, int treelet_size, int rounds, hipStream_t s);
+int run_treelets(BuildBufs& b, int treelet_size, int rounds, int key_bits, int nbits, hipStream_t s);
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width, int leaf_size,
                    int treelet_size, int treelet_rounds, hipStream_t stream, gpurt_bvh_s* out);
 

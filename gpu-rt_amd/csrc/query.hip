@@ -36,12 +36,11 @@ struct TravArgs {
 template <int W, int S, int BS, bool STATS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_cpq(
     TravArgs a) {
-    __shared__ uint2 stk[S * BS];
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
-    st.lds = stk + threadIdx.x;
+    st.lane = threadIdx.x;
     st.spill = a.spill;
     st.gid = gid;
     st.total = total;
@@ -72,12 +71,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
 template <int W, int S, int BS, bool ANY, bool STATS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_ray(
     TravArgs a) {
-    __shared__ uint2 stk[S * BS];
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
-    st.lds = stk + threadIdx.x;
+    st.lane = threadIdx.x;
     st.spill = a.spill;
     st.gid = gid;
     st.total = total;
@@ -118,13 +116,13 @@ template <int W, int S, int BS, bool STATS>
 static void launch_kind(QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (kind) {
         case QKind::Cpq:
-            hipLaunchKernelGGL((k_cpq<W, S, BS, STATS>), dim3(grid), dim3(BS), 0, s, a);
+            hipLaunchKernelGGL((k_cpq<W, S, BS, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
         case QKind::Closest:
-            hipLaunchKernelGGL((k_ray<W, S, BS, false, STATS>), dim3(grid), dim3(BS), 0, s, a);
+            hipLaunchKernelGGL((k_ray<W, S, BS, false, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
         case QKind::Any:
-            hipLaunchKernelGGL((k_ray<W, S, BS, true, STATS>), dim3(grid), dim3(BS), 0, s, a);
+            hipLaunchKernelGGL((k_ray<W, S, BS, true, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
     }
 }

@@ -18,24 +18,36 @@
 
 namespace gpurt_impl {
 
+// Per-lane traversal stack: slots [0, S) in LDS at lds_stack[slot * BS + lane]
+// (dynamic shared memory of S * BS * 8 bytes, indexed directly so the compiler keeps
+// LDS addressing), deeper slots in the device spill buffer at
+// spill[(slot - S) * total + gid].
+extern __shared__ uint2 lds_stack[];
+
 template <int S, int BS>
 struct LaneStack {
-    uint2* lds;
     uint2* spill;
-    uint32_t gid, total, cap;
+    uint32_t lane, gid, total, cap;
     int sp;
+    // The spill side uses nontemporal builtins: besides not polluting the caches, they
+    // keep the compiler from merging the two sides into one flat (generic) access.
     __device__ __forceinline__ bool push(uint32_t ref, float key) {
         if ((uint32_t)sp >= cap) return false;
-        uint2 e = make_uint2(ref, __float_as_uint(key));
-        if (sp < S) lds[sp * BS] = e;
-        else spill[(size_t)(sp - S) * total + gid] = e;
+        if (sp < S) {
+            lds_stack[sp * BS + lane] = make_uint2(ref, __float_as_uint(key));
+        } else {
+            uint32_t* p = reinterpret_cast<uint32_t*>(spill + (size_t)(sp - S) * total + gid);
+            __builtin_nontemporal_store(ref, p);
+            __builtin_nontemporal_store(__float_as_uint(key), p + 1);
+        }
         sp++;
         return true;
     }
     __device__ __forceinline__ uint2 pop() {
         sp--;
-        if (sp < S) return lds[sp * BS];
-        return spill[(size_t)(sp - S) * total + gid];
+        if (sp < S) return lds_stack[sp * BS + lane];
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(spill + (size_t)(sp - S) * total + gid);
+        return make_uint2(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1));
     }
 };
 

@@ -1,10 +1,199 @@
-// treelet.hip -- SAH treelet restructuring of the binary LBVH (Karras & Aila, HPG 2013).
-#include "gpurt_internal.h"
+// treelet.hip -- SAH treelet restructuring of the binary LBVH (Karras & Aila, "Fast
+// Parallel Construction of High-Quality Bounding Volume Hierarchies", HPG 2013).
+//
+// For every internal node r with more than leaf_size triangles, bottom-up:
+//   1. form a treelet: start from r's two children and repeatedly open the treelet leaf
+//      (an internal node) with the largest surface area, up to N leaves (N = 5..7);
+//   2. find the binary topology over those leaves with the least SAH cost by dynamic
+//      programming over all 2^N - 1 leaf subsets (one 64-lane wave per treelet: lanes
+//      own subsets; each subset tries every split that keeps its lowest leaf on one
+//      side);
+//   3. if it is cheaper than the current one, rewrite the treelet's N - 1 internal nodes
+//      (r keeps its index, so ancestors are untouched) with their new children, boxes,
+//      triangle counts and costs.
+// Bottom-up order is the same as the box pass: one launch per Karras prefix length,
+// largest first.  Two roots of one launch have equal prefix length, so neither is an
+// ancestor of the other and their treelets are disjoint; everything below a root was
+// finished by earlier launches.  (That argument holds for the tree as Karras built it,
+// so one round is supported.)  Subtrees of <= leaf_size triangles become single leaves
+// in the wide collapse and are priced as leaves, so they are never restructured.
+#include <algorithm>
+
+#include "build_common.h"
 
 namespace gpurt_impl {
 
-int run_treelets(BuildBufs&, int, int, hipStream_t) {
-    return set_error(GPURT_E_UNSUPPORTED, "treelet refinement is not available in this build");
+template <int N>
+__global__ __launch_bounds__(64) void k_treelet(int n_int, int level, int leaf_size,
+                                                const uint8_t* __restrict__ dlev, uint32_t* left,
+                                                uint32_t* right, uint32_t* cnt, float* cost, float4* nbox,
+                                                const float4* __restrict__ saabb) {
+    constexpr int NS = 1 << N;
+    __shared__ uint32_t s_leaf[N], s_int[N];
+    __shared__ float4 s_llo[N], s_lhi[N];
+    __shared__ uint32_t s_lcnt[N];
+    __shared__ float s_lcost[N];
+    __shared__ float4 s_lo[NS], s_hi[NS];
+    __shared__ uint32_t s_cnt[NS];
+    __shared__ float s_copt[NS];
+    __shared__ uint8_t s_part[NS];
+    __shared__ int s_nl;
+    __shared__ float s_cur;
+    const int lane = threadIdx.x;
+    const int n_chunks = (n_int + 63) / 64;
+    for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+        const int i = ch * 64 + lane;
+        const bool cand = i < n_int && dlev[i] == (uint8_t)level && cnt[i] > (uint32_t)leaf_size;
+        uint64_t mask = __ballot(cand);
+        while (mask) {
+            const int r = ch * 64 + (__ffsll((unsigned long long)mask) - 1);
+            mask &= mask - 1;
+            if (lane == 0) {
+                // refresh r's cost from its (finished) children, then form the treelet
+                float4 lo = nbox[2 * (size_t)r], hi = nbox[2 * (size_t)r + 1];
+                s_cur = sah_node_cost(lo, hi, cnt[r], ref_cost(left[r], saabb, cost),
+                                      ref_cost(right[r], saabb, cost), leaf_size);
+                cost[r] = s_cur;
+                s_leaf[0] = left[r];
+                s_leaf[1] = right[r];
+                s_int[0] = (uint32_t)r;
+                int nl = 2, ni = 1;
+                while (nl < N) {
+                    int best = -1;
+                    float best_a = -1.0f;
+                    for (int j = 0; j < nl; j++) {
+                        const uint32_t x = s_leaf[j];
+                        if (x & kLeafBit) continue;
+                        const float a = box_area(nbox[2 * (size_t)x], nbox[2 * (size_t)x + 1]);
+                        if (a > best_a) {
+                            best_a = a;
+                            best = j;
+                        }
+                    }
+                    if (best < 0) break;
+                    const uint32_t x = s_leaf[best];
+                    s_int[ni++] = x;
+                    s_leaf[best] = left[x];
+                    s_leaf[nl++] = right[x];
+                }
+                s_nl = nl;
+            }
+            __syncthreads();
+            const int nl = s_nl;
+            if (lane < nl) {
+                const uint32_t x = s_leaf[lane];
+                float4 lo, hi;
+                ref_box(x, saabb, nbox, lo, hi);
+                s_llo[lane] = lo;
+                s_lhi[lane] = hi;
+                s_lcnt[lane] = (x & kLeafBit) ? 1u : cnt[x];
+                s_lcost[lane] = ref_cost(x, saabb, cost);
+            }
+            __syncthreads();
+            const int full = (1 << nl) - 1;
+            for (int sub = lane + 1; sub <= full; sub += 64) {
+                float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+                float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                uint32_t c = 0;
+                for (int j = 0; j < nl; j++) {
+                    if (!(sub & (1 << j))) continue;
+                    lo.x = fminf(lo.x, s_llo[j].x); lo.y = fminf(lo.y, s_llo[j].y); lo.z = fminf(lo.z, s_llo[j].z);
+                    hi.x = fmaxf(hi.x, s_lhi[j].x); hi.y = fmaxf(hi.y, s_lhi[j].y); hi.z = fmaxf(hi.z, s_lhi[j].z);
+                    c += s_lcnt[j];
+                }
+                s_lo[sub] = lo;
+                s_hi[sub] = hi;
+                s_cnt[sub] = c;
+                if ((sub & (sub - 1)) == 0) {
+                    s_copt[sub] = s_lcost[__ffs(sub) - 1];
+                    s_part[sub] = 0;
+                }
+            }
+            __syncthreads();
+            for (int k = 2; k <= nl; k++) {
+                for (int sub = lane + 1; sub <= full; sub += 64) {
+                    if (__popc(sub) != k) continue;
+                    const int low = sub & -sub;
+                    float best = INFINITY;
+                    int bp = low;
+                    for (int P = (sub - 1) & sub; P; P = (P - 1) & sub) {
+                        if (!(P & low)) continue;
+                        const float c = s_copt[P] + s_copt[sub ^ P];
+                        if (c < best) {
+                            best = c;
+                            bp = P;
+                        }
+                    }
+                    const float a = box_area(s_lo[sub], s_hi[sub]);
+                    s_copt[sub] = (s_cnt[sub] <= (uint32_t)leaf_size) ? kCt * a * (float)s_cnt[sub] : kCi * a + best;
+                    s_part[sub] = (uint8_t)bp;
+                }
+                __syncthreads();
+            }
+            if (lane == 0 && nl >= 3 && s_copt[full] < s_cur * (1.0f - 1e-5f)) {
+                // rebuild the treelet top-down from the optimal partitions, reusing its
+                // internal node indices (r first)
+                int st_sub[N], st_node[N];
+                int sp = 0, pool = 1;
+                st_sub[sp] = full;
+                st_node[sp] = r;
+                sp++;
+                while (sp > 0) {
+                    sp--;
+                    const int sub = st_sub[sp];
+                    const int node = st_node[sp];
+                    const int P = s_part[sub];
+                    const int Q = sub ^ P;
+                    uint32_t ref[2];
+                    const int parts[2] = {P, Q};
+                    for (int h = 0; h < 2; h++) {
+                        const int part = parts[h];
+                        if ((part & (part - 1)) == 0) {
+                            ref[h] = s_leaf[__ffs(part) - 1];
+                        } else {
+                            const uint32_t idx = s_int[pool++];
+                            ref[h] = idx;
+                            st_sub[sp] = part;
+                            st_node[sp] = (int)idx;
+                            sp++;
+                        }
+                    }
+                    left[node] = ref[0];
+                    right[node] = ref[1];
+                    nbox[2 * (size_t)node] = s_lo[sub];
+                    nbox[2 * (size_t)node + 1] = s_hi[sub];
+                    cnt[node] = s_cnt[sub];
+                    cost[node] = s_copt[sub];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int N>
+static void launch_treelet_levels(BuildBufs& b, int key_bits, int nbits, hipStream_t s) {
+    const int n_int = (int)(b.N - 1);
+    const unsigned grid = std::min(cdiv(n_int, 64), 4096u);
+    for (int level = 95; level >= 64 - key_bits; level--) {
+        if (level >= 64 && level < 96 - nbits) continue;
+        hipLaunchKernelGGL(k_treelet<N>, dim3(grid), dim3(64), 0, s, n_int, level, b.leaf_size, b.dlev, b.left,
+                           b.right, b.cnt, b.cost, b.nbox, b.saabb);
+    }
+}
+
+int run_treelets(BuildBufs& b, int treelet_size, int rounds, int key_bits, int nbits, hipStream_t s) {
+    if (rounds > 1)
+        return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds > 1 is not supported (one bottom-up round)");
+    if (b.N < 3) return GPURT_OK;
+    switch (treelet_size) {
+        case 5: launch_treelet_levels<5>(b, key_bits, nbits, s); break;
+        case 6: launch_treelet_levels<6>(b, key_bits, nbits, s); break;
+        case 7: launch_treelet_levels<7>(b, key_bits, nbits, s); break;
+        default: return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 5..7");
+    }
+    GPURT_HIP(hipGetLastError());
+    return GPURT_OK;
 }
 
 }  // namespace gpurt_impl

@@ -71,8 +71,8 @@ typedef struct gpurt_ray {
 typedef struct gpurt_build_opts {
     int32_t node_width;     /* children per node: 2, 4 or 8 (0 = 4)                  */
     int32_t leaf_size;      /* max triangles per leaf, 1..8 (0 = 4)                   */
-    int32_t treelet_size;   /* leaves per treelet for SAH refinement, 0 = off, 5..7  */
-    int32_t treelet_rounds; /* refinement rounds (0 = 1 when treelet_size > 0)        */
+    int32_t treelet_size;   /* leaves per treelet for SAH restructuring, 0 = off, 5..7 */
+    int32_t treelet_rounds; /* bottom-up restructuring rounds: 0 or 1 (0 = 1 when on) */
 } gpurt_build_opts;
 
 typedef struct gpurt_query_opts {

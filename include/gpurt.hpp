@@ -1,0 +1,236 @@
+// gpurt.hpp -- C++ host layer over the C ABI (gpurt.h), shaped like the reference's
+// VK::Mesh / VK::Accel so a caller of those can switch with few edits.
+//
+//   gpurt::Mesh   mirrors VK::Mesh (src/vk/mesh.h:14-58): owns host vertex/index arrays
+//                 (48-byte Vertex {pos, norm, tang} with u in pos.w and v in norm.w,
+//                 scene.cpp:310) and their device copies, computes the bounding box on
+//                 recreate (mesh.cpp:73-75) and uploads lazily in sync() (mesh.cpp:78-85).
+//   gpurt::Accel  mirrors VK::Accel's bottom-level constructor (vulkan.h:256-284,
+//                 vulkan.cpp:881-936): move-only, recreate(mesh), destroy(), a public
+//                 handle `accel`.  Queries replace traceRayEXT (rt.rgen:257-291).
+//
+// Errors throw gpurt::Error (the reference calls die()/exit, log.h:46-49; a library must
+// not end the process).  Query arguments are device pointers; calls are asynchronous on
+// the given stream.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gpurt.h"
+
+namespace gpurt {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc) {
+    if (rc != GPURT_OK) throw Error(rc, gpurt_last_error());
+}
+
+inline void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(GPURT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct Vec4 {
+    float x = 0, y = 0, z = 0, w = 0;
+};
+
+struct BBox {  // lib/bbox.h:15-42 semantics: empty = (FLT_MAX, -FLT_MAX)
+    float min[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float max[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void enclose(const float* p) {
+        for (int k = 0; k < 3; k++) {
+            min[k] = std::min(min[k], p[k]);
+            max[k] = std::max(max[k], p[k]);
+        }
+    }
+};
+
+class Context {
+public:
+    explicit Context(int device = 0) { check(gpurt_ctx_create(device, &h_)); }
+    ~Context() { gpurt_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    Context(Context&& o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+    Context& operator=(Context&& o) noexcept {
+        std::swap(h_, o.h_);
+        return *this;
+    }
+    gpurt_ctx handle() const { return h_; }
+
+private:
+    gpurt_ctx h_ = nullptr;
+};
+
+template <typename T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    explicit DeviceArray(size_t n) { resize(n); }
+    ~DeviceArray() { reset(); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+    DeviceArray& operator=(DeviceArray&& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        return *this;
+    }
+    void resize(size_t n) {
+        if (n == n_) return;
+        reset();
+        if (n) check_hip(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+        n_ = n;
+    }
+    void reset() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void upload(const T* src, size_t n) {
+        resize(n);
+        if (n) check_hip(hipMemcpy(p_, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    void download(T* dst) const {
+        if (n_) check_hip(hipMemcpy(dst, p_, n_ * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+    T* data() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Mirrors VK::Mesh.
+class Mesh {
+public:
+    typedef uint32_t Index;
+    struct Vertex {
+        Vec4 pos;   // xyz = position, w = texture u
+        Vec4 norm;  // xyz = normal, w = texture v
+        Vec4 tang;
+    };
+    static_assert(sizeof(Vertex) == 48, "Mesh::Vertex is 48 bytes like the reference's");
+
+    Mesh() = default;
+    Mesh(std::vector<Vertex>&& vertices, std::vector<Index>&& indices) {
+        recreate(std::move(vertices), std::move(indices));
+    }
+    Mesh(const Mesh&) = delete;
+    Mesh& operator=(const Mesh&) = delete;
+    Mesh(Mesh&&) = default;
+    Mesh& operator=(Mesh&&) = default;
+
+    void recreate(std::vector<Vertex>&& vertices, std::vector<Index>&& indices) {
+        if (indices.size() % 3 != 0) throw Error(GPURT_E_INVALID, "index count is not a multiple of 3");
+        verts_ = std::move(vertices);
+        idxs_ = std::move(indices);
+        dirty_ = true;
+        BBox box;
+        for (auto& v : verts_) box.enclose(&v.pos.x);
+        bbox_ = box;
+    }
+    // upload if the host arrays changed since the last upload
+    void sync() const {
+        if (!dirty_) return;
+        vbuf_.upload(verts_.data(), verts_.size());
+        ibuf_.upload(idxs_.data(), idxs_.size());
+        dirty_ = false;
+    }
+    const std::vector<Vertex>& verts() const { return verts_; }
+    const std::vector<Index>& inds() const { return idxs_; }
+    BBox bbox() const { return bbox_; }
+    const float* device_verts() const {
+        sync();
+        return reinterpret_cast<const float*>(vbuf_.data());
+    }
+    const uint32_t* device_inds() const {
+        sync();
+        return ibuf_.data();
+    }
+
+private:
+    std::vector<Vertex> verts_;
+    std::vector<Index> idxs_;
+    BBox bbox_;
+    mutable DeviceArray<Vertex> vbuf_;
+    mutable DeviceArray<Index> ibuf_;
+    mutable bool dirty_ = false;
+};
+
+// Mirrors VK::Accel (bottom level).  `accel` is the public handle like
+// VkAccelerationStructureKHR accel (vulkan.h:273).
+class Accel {
+public:
+    Accel() = default;
+    Accel(const Context& ctx, const Mesh& mesh, const gpurt_build_opts* opts = nullptr,
+          hipStream_t stream = nullptr) {
+        recreate(ctx, mesh, opts, stream);
+    }
+    ~Accel() { destroy(); }
+    Accel(const Accel&) = delete;
+    Accel& operator=(const Accel&) = delete;
+    Accel(Accel&& o) noexcept : accel(std::exchange(o.accel, nullptr)) {}
+    Accel& operator=(Accel&& o) noexcept {
+        std::swap(accel, o.accel);
+        return *this;
+    }
+
+    // Accel::recreate(const Mesh&) (vulkan.cpp:881-936): sync the mesh, build, wait.
+    void recreate(const Context& ctx, const Mesh& mesh, const gpurt_build_opts* opts = nullptr,
+                  hipStream_t stream = nullptr) {
+        destroy();
+        mesh.sync();
+        check(gpurt_bvh_build(ctx.handle(), mesh.device_verts(), sizeof(Mesh::Vertex), mesh.verts().size(),
+                              mesh.device_inds(), mesh.inds().size() / 3, opts, stream, &accel));
+    }
+    // moved vertices, same topology
+    void refit(const Mesh& mesh, hipStream_t stream = nullptr) {
+        mesh.sync();
+        check(gpurt_bvh_refit(accel, mesh.device_verts(), sizeof(Mesh::Vertex), stream));
+    }
+    void destroy() {
+        gpurt_bvh_destroy(accel);
+        accel = nullptr;
+    }
+    gpurt_bvh_info info() const {
+        gpurt_bvh_info i;
+        check(gpurt_bvh_get_info(accel, &i));
+        return i;
+    }
+
+    // trace_ray (rt.rgen:257-270): closest hit, payload = (prim, t, uv) per ray
+    void trace_closest(const gpurt_ray* rays, size_t n, int32_t* prim, float* t, float* uv,
+                       const gpurt_query_opts* opts = nullptr, hipStream_t stream = nullptr) const {
+        check(gpurt_trace_closest(accel, rays, n, prim, t, uv, opts, stream));
+    }
+    // visibility (rt.rgen:272-291): occluded[i] = 1 when something is hit in [tmin, tmax]
+    void trace_any(const gpurt_ray* rays, size_t n, uint8_t* occluded, const gpurt_query_opts* opts = nullptr,
+                   hipStream_t stream = nullptr) const {
+        check(gpurt_trace_any(accel, rays, n, occluded, opts, stream));
+    }
+    void closest_point(const float* q, size_t n, int32_t* prim, float* pt, float* dist, float* bary,
+                       const gpurt_query_opts* opts = nullptr, hipStream_t stream = nullptr) const {
+        check(gpurt_closest_point(accel, q, n, prim, pt, dist, bary, opts, stream));
+    }
+
+    gpurt_bvh accel = nullptr;
+};
+
+}  // namespace gpurt

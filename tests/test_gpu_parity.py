@@ -202,3 +202,36 @@ def test_stats_count_visits(ctx, dev):
     assert np.array_equal(prim, ep)
     # the BVH prunes: far fewer triangle tests than brute force
     assert tris < 0.05 * 4096 * 5120
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_refit_after_moving_vertices(ctx, dev, width):
+    """Refit (same triangles, moved vertices) answers exactly like brute force on the
+    moved mesh, for closest points and rays, and twice in a row."""
+    w = W.cpq_small(n_queries=3000)
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    tv = _t(V, dev)
+    bvh = gpurt.Bvh(ctx, tv, _t(F, dev), node_width=width)
+    rng = np.random.default_rng(9)
+    o = rng.uniform(-1.5, 1.5, (3000, 3))
+    R = W.pack_rays(o, W.random_dirs(rng, 3000), 1e-5, 1e7)
+    for step in range(2):
+        V = (V * np.float32(1.0 + 0.2 * (step + 1)) +
+             rng.normal(0, 0.02, V.shape).astype(np.float32)).astype(np.float32)
+        tv.copy_(_t(V, dev))
+        bvh.refit(tv)
+        prim, pt, dist, bary = _np(*bvh.closest_point(_t(Q, dev)))
+        ep, ept, ed, eb = O.cpq_brute_f32(V, F, Q)
+        assert np.array_equal(prim, ep) and np.array_equal(dist, ed) and np.array_equal(pt, ept)
+        p, t, uv = _np(*bvh.trace_closest(_t(R, dev)))
+        rp, rt, ruv = O.ray_closest_brute_f32(V, F, R)
+        assert np.array_equal(p, rp) and np.array_equal(t, rt)
+        # backward uses the refitted positions
+        g = torch.ones(len(Q), device=dev)
+        gv = torch.zeros((V.shape[0], 3), device=dev)
+        tq = _t(Q, dev)
+        prim_t, pt_t, dist_t, bary_t = bvh.closest_point(tq)
+        bvh.closest_point_bwd(tq, prim_t, bary_t, pt_t, dist_t, g, None, gv)
+        _, egv = O.cpq_grad_f64(V.astype(np.float64), F, Q.astype(np.float64), ep, ept.astype(np.float64),
+                                eb.astype(np.float64))
+        np.testing.assert_allclose(gv.cpu().numpy(), egv, atol=2e-4)
