@@ -54,6 +54,24 @@ def timed(fn, steps, warmup, stream=None):
     return s.elapsed_time(e) / steps  # ms
 
 
+def measured_traffic(args, n_tris, n_rays):
+    """HBM bytes per closest-hit launch from the committed counter summary
+    (profiles/rays_scene_traffic.json: FETCH_SIZE and WRITE_SIZE passes, each scaled by
+    what the same counter reads for a 256 MiB device copy), when it was measured on this
+    exact configuration; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "rays_scene_traffic.json")
+    try:
+        m = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if (m["n_tris"], m["n_rays"], m["node_width"], m["stack_depth"], m["treelet_size"]) != \
+            (n_tris, n_rays, args.width, args.stack, args.treelet):
+        return None
+    rd = m["fetch_kb_raw"] * m["copy_kib"] / m["copy_fetch_kb_raw"]
+    wr = m["write_kb_raw"] * m["copy_kib"] / m["copy_write_kb_raw"]
+    return (rd + wr) * 1024.0
+
+
 def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     t0 = time.time()
     w = W.rays_scene()
@@ -66,7 +84,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     n = rays.shape[0]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
     log(f"[bench] GPU build {build_ms:.1f} ms, {info}")
@@ -127,12 +145,13 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "kernel_ms": ms_closest,
     }
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = measured_traffic(args, int(F.shape[0]), n)
     res = {
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
         "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
                    "passes": ["closest_hit", "any_hit"], "node_width": args.width,
-                   "leaf_size": 4, "stack_depth": args.stack},
+                   "leaf_size": 4, "stack_depth": args.stack, "treelet_size": args.treelet},
         "roofline": roof,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
                          "sample": f"{m} closest-hit + {m} any-hit rays of rays_scene on the "
@@ -155,7 +174,7 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
     q = torch.from_numpy(Q).to(dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
     build_ms = (time.perf_counter() - t0) * 1e3
     n = q.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
@@ -181,7 +200,7 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     q0 = torch.from_numpy(w["queries"]).to(dev)
     o0 = torch.from_numpy(w["ray_o"]).to(dev)
     d0 = torch.from_numpy(w["ray_d"]).to(dev)
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
     nq, nr = q0.shape[0], o0.shape[0]
 
     def fwd():
@@ -219,7 +238,8 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     meshes = [{k: torch.from_numpy(v).to(dev) for k, v in m.items()} for m in w["meshes"]]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack)
+    batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
+                        treelet_size=args.treelet)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
     res = batch.run()
@@ -228,7 +248,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
     def plain():
         for i, m in enumerate(meshes):
-            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width)
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, treelet_size=args.treelet)
             p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack)
             rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack)
             plain_out[i] = (p, d, rp, rt)
@@ -281,6 +301,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
     ap.add_argument("--stack", type=int, default=8, choices=[8, 16, 32, 64])
+    ap.add_argument("--treelet", type=int, default=0, choices=[0, 5, 6, 7],
+                    help="SAH treelet restructuring (leaves per treelet), 0 = off")
     ap.add_argument("--cpu-rays", type=int, default=131072)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")

@@ -416,7 +416,6 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
     left[i] = L;
     right[i] = R;
     range[i] = make_uint2((uint32_t)first, (uint32_t)last);
-    dlev[i] = (uint8_t)dnode;
     cnt[i] = (uint32_t)(last - first + 1);
     if (L & kLeafBit) leaf_parent[split] = (uint32_t)i;
     else node_parent[split] = (uint32_t)i;
@@ -430,6 +429,9 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
     }
     is_root[i] = root ? 1 : 0;
     if (root) seg_root[sf] = (uint32_t)i;
+    // nodes above the mesh roots (their range spans several meshes) belong to no BVH:
+    // prefix length 0xFF keeps the box and treelet passes away from them
+    dlev[i] = (sf == sl) ? (uint8_t)dnode : (uint8_t)0xFF;
 }
 
 // ------------------------------------------------------------------------------------
@@ -607,6 +609,27 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
         }
         nodes[wi] = out;
     }
+}
+
+// SAH cost of a finished wide tree: sum over nodes of kCi * area for internal children
+// and kCt * count * area for leaves, accumulated in double per wave then atomically.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_sah_cost(uint32_t n_nodes, const WNode<W>* __restrict__ nodes,
+                                                     double* acc) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    double c = 0.0;
+    if (k < n_nodes) {
+        const WNode<W>& nd = nodes[k];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            if (nd.child[j] == kEmpty) continue;
+            const float a = box_area(make_float4(nd.lox[j], nd.loy[j], nd.loz[j], 0.0f),
+                                     make_float4(nd.hix[j], nd.hiy[j], nd.hiz[j], 0.0f));
+            c += child_is_leaf(nd.child[j]) ? (double)kCt * leaf_count(nd.child[j]) * a : (double)kCi * a;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c != 0.0) atomicAdd(acc, c);
 }
 
 // level 0 of the wide tree: one node per mesh root
@@ -905,6 +928,23 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
     float scale;
     std::memcpy(&scale, &u, 4);
+    // SAH cost relative to the root box area
+    double sah = 0.0;
+    {
+        double* acc = reinterpret_cast<double*>(b.hist);  // scratch no longer needed
+        GPURT_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
+        if (W == 2) hipLaunchKernelGGL(k_sah_cost<2>, dim3(cdiv(n_nodes, kBlock)), dim3(kBlock), 0, s, n_nodes, (const WNode<2>*)nodes, acc);
+        else if (W == 4) hipLaunchKernelGGL(k_sah_cost<4>, dim3(cdiv(n_nodes, kBlock)), dim3(kBlock), 0, s, n_nodes, (const WNode<4>*)nodes, acc);
+        else hipLaunchKernelGGL(k_sah_cost<8>, dim3(cdiv(n_nodes, kBlock)), dim3(kBlock), 0, s, n_nodes, (const WNode<8>*)nodes, acc);
+        float rl[4], rh[4];
+        GPURT_HIP(hipMemcpyAsync(&sah, acc, sizeof(double), hipMemcpyDeviceToHost, s));
+        GPURT_HIP(hipMemcpyAsync(rl, b.nbox, sizeof(rl), hipMemcpyDeviceToHost, s));
+        GPURT_HIP(hipMemcpyAsync(rh, b.nbox + 1, sizeof(rh), hipMemcpyDeviceToHost, s));
+        GPURT_HIP(hipStreamSynchronize(s));
+        const float ex = rh[0] - rl[0], ey = rh[1] - rl[1], ez = rh[2] - rl[2];
+        const double root_area = 2.0 * ((double)ex * ez + (double)ex * ey + (double)ey * ez);
+        sah = (b.N > 1 && root_area > 0.0) ? sah / root_area : 0.0;
+    }
     out->nodes = nodes;
     if (b.repacked && h_counters[2] != b.N) return set_error(GPURT_E_DEVICE, "leaf re-pack lost triangles");
     out->tris = (float4*)b.release(b.final_tris());
@@ -917,6 +957,7 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     out->n_tris = b.N;
     out->scale = scale;
     out->pad = scale * 0x1p-18f;
+    out->sah_cost = (float)sah;
     out->level_cnt.assign(h_lvl.begin(), h_lvl.begin() + depth);
     return GPURT_OK;
 }

@@ -29,9 +29,10 @@ Workloads (sizes):
                   U(ball r=0.5) (seed 10)
   batched         256 meshes, triangle targets log-uniform [1024, 524288] (seed 6): even
                   index -> displaced icosphere at the level nearest the target (L3..L7),
-                  odd index -> terrain grid with 2(G-1)^2 ~ target; query and ray counts
-                  log-uniform [256, 262144] (seed 7), 80/20 near-surface/uniform queries,
-                  rays from the mesh's bounding sphere inwards
+                  odd index -> terrain grid with 2(G-1)^2 ~ target scaled to x,z in [-1,1];
+                  query and ray counts log-uniform [256, 262144] (seed 7); queries 80%
+                  near-surface + 20% uniform in the mesh's bounding box grown 20% about
+                  its centre; rays from the mesh's bounding sphere inwards
 """
 import numpy as np
 
@@ -220,9 +221,14 @@ def rays_scene(n_rays=1 << 20, grid=708, n_spheres=64, sphere_level=5):
 
 
 def near_surface_queries(v, f, rng, n, frac_surface=0.8, sigma=0.005, box=2.0):
+    """80% surface samples + N(0, sigma) normal offsets, 20% uniform in a box: the cube
+    [-box, box]^3 for a number, or an explicit (lo, hi) pair."""
     n_s = int(round(n * frac_surface))
     qs = surface_samples(v, f, rng, n_s, sigma)
-    qu = rng.uniform(-box, box, (n - n_s, 3))
+    if isinstance(box, tuple):
+        qu = rng.uniform(box[0], box[1], (n - n_s, 3))
+    else:
+        qu = rng.uniform(-box, box, (n - n_s, 3))
     q = np.concatenate([qs, qu], 0)
     return q[rng.permutation(n)]
 
@@ -268,8 +274,10 @@ def batched(n_meshes=256, tri_lo=1024, tri_hi=524288, q_lo=256, q_hi=262144):
             v = v / 50.0  # unit-scale terrain
         V, F = _as_f32(v, f)
         n = int(counts[k])
+        lo, hi = V.min(0).astype(np.float64), V.max(0).astype(np.float64)
+        mid, half = (lo + hi) / 2, (hi - lo) / 2 * 1.2
         q = near_surface_queries(V.astype(np.float64), F, np.random.default_rng([7, k]), n,
-                                 box=1.5)
+                                 box=(mid - half, mid + half))
         rr = np.random.default_rng([7, k, 1])
         c = V.mean(0).astype(np.float64)
         rad = float(np.linalg.norm(V - c, axis=1).max()) * 1.5 + 1e-3

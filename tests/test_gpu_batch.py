@@ -107,3 +107,16 @@ def test_batch_rejects_bad_index(ctx, dev):
     with pytest.raises(gpurt.GpurtError) as e:
         gpurt.Batch(ctx, _dev(host, dev))
     assert e.value.code == -1
+
+
+def test_batch_with_treelets_matches_oracle(ctx, dev):
+    host = _small_meshes(seed=7)
+    res = gpurt.Batch(ctx, _dev(host, dev), treelet_size=6).run()
+    torch.cuda.synchronize()
+    for h, r in zip(host, res):
+        ep, _, ed, _ = O.cpq_brute_f32(h["verts"], h["tris"], h["queries"])
+        assert np.array_equal(r["q_prim"].cpu().numpy(), ep)
+        assert np.array_equal(r["q_dist"].cpu().numpy(), ed)
+        rp, rt, _ = O.ray_closest_brute_f32(h["verts"], h["tris"], h["rays"])
+        assert np.array_equal(r["r_prim"].cpu().numpy(), rp)
+        assert np.array_equal(r["r_t"].cpu().numpy(), rt)

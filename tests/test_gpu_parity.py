@@ -235,3 +235,30 @@ def test_refit_after_moving_vertices(ctx, dev, width):
         _, egv = O.cpq_grad_f64(V.astype(np.float64), F, Q.astype(np.float64), ep, ept.astype(np.float64),
                                 eb.astype(np.float64))
         np.testing.assert_allclose(gv.cpu().numpy(), egv, atol=2e-4)
+
+
+@pytest.mark.parametrize("tsize", [5, 6, 7])
+def test_treelet_builds_are_bitexact(ctx, dev, tsize):
+    """SAH treelet restructuring changes the tree, never the answers."""
+    w = W.cpq_small(n_queries=4096)
+    V, F, Q = w["verts"], w["tris"], w["queries"]
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), treelet_size=tsize)
+    prim, pt, dist, bary = _np(*bvh.closest_point(_t(Q, dev)))
+    ep, ept, ed, eb = O.cpq_brute_f32(V, F, Q)
+    assert np.array_equal(prim, ep) and np.array_equal(dist, ed) and np.array_equal(bary, eb)
+    ws = W.rays_scene(n_rays=4096, grid=81, n_spheres=4)
+    _ray_check(ctx, dev, ws["verts"], ws["tris"], ws["rays"], ws["rays_any"])
+    bvh2 = gpurt.Bvh(ctx, _t(ws["verts"], dev), _t(ws["tris"], dev), treelet_size=tsize)
+    p, t, uv = _np(*bvh2.trace_closest(_t(ws["rays"], dev)))
+    rp, rt, ruv = O.ray_closest_brute_f32(ws["verts"], ws["tris"], ws["rays"])
+    assert np.array_equal(p, rp) and np.array_equal(t, rt) and np.array_equal(uv, ruv)
+
+
+def test_treelets_lower_the_sah_cost(ctx, dev):
+    for V, F in [W.displaced_icosphere(6, np.random.default_rng(1)),
+                 W.terrain(201, np.random.default_rng(2))]:
+        tv, tf = _t(V.astype(np.float32), dev), _t(F.astype(np.int32), dev)
+        base = gpurt.Bvh(ctx, tv, tf).info()["sah_cost"]
+        opt = gpurt.Bvh(ctx, tv, tf, treelet_size=7).info()["sah_cost"]
+        print("sah", base, "->", opt)
+        assert 0 < opt < base * 0.99
