@@ -239,7 +239,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
-                        treelet_size=args.treelet)
+                        treelet_size=args.batch_treelet)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
     res = batch.run()
@@ -248,7 +248,8 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
     def plain():
         for i, m in enumerate(meshes):
-            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, treelet_size=args.treelet)
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width,
+                            treelet_size=args.batch_treelet)
             p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack)
             rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack)
             plain_out[i] = (p, d, rp, rt)
@@ -267,7 +268,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "batched_units_per_s": (nq + nr) / (ms_b * 1e-3), "batched_ms": ms_b,
             "plain_units_per_s": (nq + nr) / (ms_p * 1e-3), "plain_ms": ms_p,
             "batched_over_plain": ms_p / ms_b, "batch_create_s": create_s,
-            "parity_batched_vs_plain": bool(parity),
+            "parity_batched_vs_plain": bool(parity), "treelet_size": args.batch_treelet,
             "note": "each step rebuilds all 256 BVHs and answers every query"}
 
 
@@ -301,8 +302,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
     ap.add_argument("--stack", type=int, default=8, choices=[8, 16, 32, 64])
-    ap.add_argument("--treelet", type=int, default=0, choices=[0, 5, 6, 7],
-                    help="SAH treelet restructuring (leaves per treelet), 0 = off")
+    ap.add_argument("--treelet", type=int, default=7, choices=[0, 5, 6, 7],
+                    help="SAH treelet restructuring of the single-mesh BVHs (leaves per "
+                         "treelet), 0 = off; the batched leg rebuilds every step and uses "
+                         "--batch-treelet")
+    ap.add_argument("--batch-treelet", type=int, default=0, choices=[0, 5, 6, 7])
     ap.add_argument("--cpu-rays", type=int, default=131072)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")

@@ -438,20 +438,66 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
 // internal node boxes and SAH costs
 // ------------------------------------------------------------------------------------
 
+// Internal nodes bucketed by prefix length (a counting sort, order within a bucket
+// arbitrary): lvl_hist[256] counts, lvl_begin[257] prefix sums, lvl_order the nodes.
+__global__ __launch_bounds__(kBlock) void k_level_hist(int n_int, const uint8_t* __restrict__ dlev,
+                                                       uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n_int) atomicAdd(&h[dlev[i]], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) gatomic_add(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_level_scan(const uint32_t* __restrict__ hist, uint32_t* __restrict__ begin,
+                             uint32_t* __restrict__ fill) {
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int l = 0; l < 256; l++) {
+            begin[l] = run;
+            fill[l] = run;
+            run += hist[l];
+        }
+        begin[256] = run;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_scatter(int n_int, const uint8_t* __restrict__ dlev,
+                                                          uint32_t* __restrict__ fill,
+                                                          uint32_t* __restrict__ order) {
+    __shared__ uint32_t h[256], base[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t l = 0, rank = 0;
+    if (i < n_int) {
+        l = dlev[i];
+        rank = atomicAdd(&h[l], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) base[threadIdx.x] = gatomic_add(&fill[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (i < n_int) order[base[l] + rank] = (uint32_t)i;
+}
+
 // Node boxes without atomics or fences: a child's common-prefix length (Karras' delta
 // of its range) is strictly greater than its parent's, so launching once per delta
 // value from the largest down makes every child box final before its parent reads it.
 // Each launch scans the delta byte of every internal node and merges the matching ones.
-__global__ __launch_bounds__(kBlock) void k_boxes_level(int n_int, int level, const uint8_t* __restrict__ dlev,
+__global__ __launch_bounds__(kBlock) void k_boxes_level(int level, const uint32_t* __restrict__ lvl_begin,
+                                                        const uint32_t* __restrict__ lvl_order,
                                                         const uint32_t* __restrict__ left,
                                                         const uint32_t* __restrict__ right,
                                                         const uint32_t* __restrict__ cnt,
                                                         const float4* __restrict__ saabb,
                                                         float4* __restrict__ nbox, float* __restrict__ cost,
                                                         int leaf_size) {
-    const int stride = gridDim.x * kBlock;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_int; i += stride) {
-        if (dlev[i] != (uint8_t)level) continue;
+    const uint32_t beg = lvl_begin[level], end = lvl_begin[level + 1];
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t k = beg + blockIdx.x * kBlock + threadIdx.x; k < end; k += stride) {
+        const uint32_t i = lvl_order[k];
         float4 al, ah, bl, bh;
         const uint32_t L = left[i], R = right[i];
         ref_box(L, saabb, nbox, al, ah);
@@ -697,6 +743,10 @@ static void carve_all(BuildBufs& b, Carver& c, size_t NI) {
     c.take(b.dlev, NI);
     c.take(b.cnt, NI);
     c.take(b.cost, NI);
+    c.take(b.lvl_order, NI);
+    c.take(b.lvl_hist, 256);
+    c.take(b.lvl_begin, 257);
+    c.take(b.lvl_fill, 256);
     c.take(b.nbox, 2 * NI);
     c.take(b.wsrc, b.capacity);
     c.take(b.wseg, b.capacity);
@@ -813,15 +863,21 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
         hipLaunchKernelGGL(k_karras, dim3(cdiv(N - 1, kBlock)), dim3(kBlock), 0, s, (int)N, kin, b.segs,
                            b.left, b.right, b.range, b.nparent, b.lparent, b.seg_root, b.is_root, b.dlev,
                            b.cnt);
-        // delta values in use: distinct keys give [64 - key_bits, 63]; equal keys give
+        // bucket the internal nodes by prefix length, then one launch per length in use,
+        // longest first; distinct keys give [64 - key_bits, 63], equal keys give
         // 64 + clz32(i ^ j) with i, j < N, i.e. [96 - ceil(log2 N), 95]
         int nbits = 0;
         while ((1u << nbits) < N) nbits++;
-        const unsigned lgrid = std::min(cdiv(n_int, kBlock), 2048u);
+        const unsigned gI = cdiv(n_int, kBlock);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, b.lvl_hist, (size_t)256, 0u);
+        hipLaunchKernelGGL(k_level_hist, dim3(gI), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_hist);
+        hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, s, b.lvl_hist, b.lvl_begin, b.lvl_fill);
+        hipLaunchKernelGGL(k_level_scatter, dim3(gI), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_fill, b.lvl_order);
+        const unsigned lgrid = std::min(gI, 1024u);
         for (int level = 95; level >= 64 - key_bits; level--) {
             if (level >= 64 && level < 96 - nbits) continue;
-            hipLaunchKernelGGL(k_boxes_level, dim3(lgrid), dim3(kBlock), 0, s, n_int, level, b.dlev, b.left,
-                               b.right, b.cnt, b.saabb, b.nbox, b.cost, b.leaf_size);
+            hipLaunchKernelGGL(k_boxes_level, dim3(lgrid), dim3(kBlock), 0, s, level, b.lvl_begin, b.lvl_order,
+                               b.left, b.right, b.cnt, b.saabb, b.nbox, b.cost, b.leaf_size);
         }
         if (treelet_size > 0) {
             int trc = run_treelets(b, treelet_size, treelet_rounds, key_bits, nbits, s);

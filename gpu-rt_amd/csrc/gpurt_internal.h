@@ -149,6 +149,10 @@ struct BuildBufs {
     uint8_t* dlev = nullptr;  // common-prefix length (Karras delta) of each internal node
     uint32_t* cnt = nullptr;  // triangles under each internal node
     float* cost = nullptr;    // SAH cost of each internal node's subtree
+    uint32_t* lvl_order = nullptr;  // internal nodes bucketed by prefix length
+    uint32_t* lvl_hist = nullptr;
+    uint32_t* lvl_begin = nullptr;  // bucket l = lvl_order[lvl_begin[l] .. lvl_begin[l+1])
+    uint32_t* lvl_fill = nullptr;
     float4* nbox = nullptr;   // binary node boxes (lo, hi)
     uint32_t *wsrc = nullptr, *wseg = nullptr, *lvl = nullptr, *counters = nullptr;
     void* nodes = nullptr;

@@ -24,10 +24,10 @@
 namespace gpurt_impl {
 
 template <int N>
-__global__ __launch_bounds__(64) void k_treelet(int n_int, int level, int leaf_size,
-                                                const uint8_t* __restrict__ dlev, uint32_t* left,
-                                                uint32_t* right, uint32_t* cnt, float* cost, float4* nbox,
-                                                const float4* __restrict__ saabb) {
+__global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __restrict__ lvl_begin,
+                                                const uint32_t* __restrict__ lvl_order, int leaf_size,
+                                                uint32_t* left, uint32_t* right, uint32_t* cnt, float* cost,
+                                                float4* nbox, const float4* __restrict__ saabb) {
     constexpr int NS = 1 << N;
     __shared__ uint32_t s_leaf[N], s_int[N];
     __shared__ float4 s_llo[N], s_lhi[N];
@@ -40,14 +40,12 @@ __global__ __launch_bounds__(64) void k_treelet(int n_int, int level, int leaf_s
     __shared__ int s_nl;
     __shared__ float s_cur;
     const int lane = threadIdx.x;
-    const int n_chunks = (n_int + 63) / 64;
-    for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
-        const int i = ch * 64 + lane;
-        const bool cand = i < n_int && dlev[i] == (uint8_t)level && cnt[i] > (uint32_t)leaf_size;
-        uint64_t mask = __ballot(cand);
-        while (mask) {
-            const int r = ch * 64 + (__ffsll((unsigned long long)mask) - 1);
-            mask &= mask - 1;
+    // one wave per treelet root of this prefix length (bucket from the box pass)
+    const uint32_t beg = lvl_begin[level], end = lvl_begin[level + 1];
+    for (uint32_t k = beg + blockIdx.x; k < end; k += gridDim.x) {
+        const int r = (int)lvl_order[k];
+        if (cnt[r] <= (uint32_t)leaf_size) continue;  // becomes one leaf: nothing to do
+        {
             if (lane == 0) {
                 // refresh r's cost from its (finished) children, then form the treelet
                 float4 lo = nbox[2 * (size_t)r], hi = nbox[2 * (size_t)r + 1];
@@ -173,16 +171,16 @@ __global__ __launch_bounds__(64) void k_treelet(int n_int, int level, int leaf_s
 
 template <int N>
 static void launch_treelet_levels(BuildBufs& b, int key_bits, int nbits, hipStream_t s) {
-    const int n_int = (int)(b.N - 1);
-    const unsigned grid = std::min(cdiv(n_int, 64), 4096u);
+    const unsigned grid = std::min(cdiv(b.N, 64), 8192u);
     for (int level = 95; level >= 64 - key_bits; level--) {
         if (level >= 64 && level < 96 - nbits) continue;
-        hipLaunchKernelGGL(k_treelet<N>, dim3(grid), dim3(64), 0, s, n_int, level, b.leaf_size, b.dlev, b.left,
-                           b.right, b.cnt, b.cost, b.nbox, b.saabb);
+        hipLaunchKernelGGL(k_treelet<N>, dim3(grid), dim3(64), 0, s, level, b.lvl_begin, b.lvl_order, b.leaf_size,
+                           b.left, b.right, b.cnt, b.cost, b.nbox, b.saabb);
     }
 }
 
 int run_treelets(BuildBufs& b, int treelet_size, int rounds, int key_bits, int nbits, hipStream_t s) {
+    // buckets (b.lvl_begin / b.lvl_order) were filled by the box pass
     if (rounds > 1)
         return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds > 1 is not supported (one bottom-up round)");
     if (b.N < 3) return GPURT_OK;
