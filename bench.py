@@ -280,7 +280,7 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
     rays = torch.from_numpy(w["rays"]).to(dev)
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
     n = rays.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
            torch.empty((n, 2), device=dev))

@@ -1,9 +1,9 @@
 // query.hip -- plain (per-mesh) closest-point and ray traversal launches for gfx950.
 //
 // Replaces the hardware traversal behind traceRayEXT (src/shaders/rt/rt.rgen:257-291)
-// and adds the closest-point query on the same BVH.  Each kernel is a grid-stride loop
-// of one query per lane over the per-query traversal in traverse.h, sized to the waves
-// that fit on the chip at once.
+// and adds the closest-point query on the same BVH.  Each kernel runs the streaming
+// (lane-refill) traversal of traverse.h over the queries with a grid sized to the waves
+// that fit on the chip at once: lane g takes queries g, g + total, g + 2 total, ...
 #include <algorithm>
 
 #include "traverse.h"
@@ -46,22 +46,20 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     st.total = total;
     st.cap = S + a.spill_slots;
     unsigned long long n_nodes = 0, n_tris = 0;
-    for (uint32_t qi = gid; qi < a.n; qi += total) {
-        const V3 q = v3(a.q[3 * (size_t)qi], a.q[3 * (size_t)qi + 1], a.q[3 * (size_t)qi + 2]);
-        CpqHit h;
-        if (!cpq_one<W, S, BS, STATS>(nodes, a.tris, a.root, q, st, h, n_nodes, n_tris))
-            atomicOr(a.err, 1u);
-        a.prim[qi] = (int32_t)h.id;
-        a.pt[3 * (size_t)qi] = h.cp.x;
-        a.pt[3 * (size_t)qi + 1] = h.cp.y;
-        a.pt[3 * (size_t)qi + 2] = h.cp.z;
-        a.dist[qi] = (h.id != 0xFFFFFFFFu) ? sqrtf(h.d2) : INFINITY;
+    auto out = [&](uint32_t qi, uint32_t id, float d2, V3 cp, V3 br) {
+        a.prim[qi] = (int32_t)id;
+        a.pt[3 * (size_t)qi] = cp.x;
+        a.pt[3 * (size_t)qi + 1] = cp.y;
+        a.pt[3 * (size_t)qi + 2] = cp.z;
+        a.dist[qi] = (id != 0xFFFFFFFFu) ? sqrtf(d2) : INFINITY;
         if (a.bary) {
-            a.bary[3 * (size_t)qi] = h.bary.x;
-            a.bary[3 * (size_t)qi + 1] = h.bary.y;
-            a.bary[3 * (size_t)qi + 2] = h.bary.z;
+            a.bary[3 * (size_t)qi] = br.x;
+            a.bary[3 * (size_t)qi + 1] = br.y;
+            a.bary[3 * (size_t)qi + 2] = br.z;
         }
-    }
+    };
+    if (!cpq_stream<W, S, BS, STATS>(nodes, a.tris, a.root, a.q, a.n, gid, total, st, out, n_nodes, n_tris))
+        gatomic_or(a.err, 1u);
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
         atomicAdd(&a.stats[1], n_tris);
@@ -81,25 +79,22 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     st.total = total;
     st.cap = S + a.spill_slots;
     unsigned long long n_nodes = 0, n_tris = 0;
-    for (uint32_t ri = gid; ri < a.n; ri += total) {
-        const float4 r0 = a.rays[2 * (size_t)ri];
-        const float4 r1 = a.rays[2 * (size_t)ri + 1];
-        RayHit h;
-        if (!ray_one<W, S, BS, ANY, STATS>(nodes, a.tris, a.root, v3(r0.x, r0.y, r0.z),
-                                           v3(r1.x, r1.y, r1.z), r0.w, r1.w, st, h, n_nodes, n_tris))
-            atomicOr(a.err, 1u);
+    auto out = [&](uint32_t ri, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
-            a.occ[ri] = h.any ? 1 : 0;
+            a.occ[ri] = any ? 1 : 0;
         } else {
-            const bool hit = h.id != 0xFFFFFFFFu;
-            a.prim[ri] = hit ? (int32_t)h.id : -1;
-            a.t[ri] = hit ? h.t : INFINITY;
+            const bool hit = id != 0xFFFFFFFFu;
+            a.prim[ri] = hit ? (int32_t)id : -1;
+            a.t[ri] = hit ? t : INFINITY;
             if (a.uv) {
-                a.uv[2 * (size_t)ri] = hit ? h.u : 0.0f;
-                a.uv[2 * (size_t)ri + 1] = hit ? h.v : 0.0f;
+                a.uv[2 * (size_t)ri] = hit ? u : 0.0f;
+                a.uv[2 * (size_t)ri + 1] = hit ? v : 0.0f;
             }
         }
-    }
+    };
+    if (!ray_stream<W, S, BS, ANY, STATS>(nodes, a.tris, a.root, a.rays, a.n, gid, total, st, out, n_nodes,
+                                          n_tris))
+        gatomic_or(a.err, 1u);
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
         atomicAdd(&a.stats[1], n_tris);

@@ -274,3 +274,210 @@ __device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
 }
 
 }  // namespace gpurt_impl
+
+namespace gpurt_impl {
+
+// ------------------------------------------------------------------------------------
+// Streaming traversal: lane-level refill
+//
+// A lane runs one traversal step per loop iteration; when its query finishes it writes
+// the result and loads its next query (index += stride) in the same iteration, so lanes
+// of a wave do not wait for the wave's slowest query before taking new work (the
+// per-query loop above keeps a finished lane idle until all 64 finish).  Every query
+// ends (each node is visited at most once), a lane leaves when its indices run out,
+// and the loop ends when no lane of the wave is active.
+// ------------------------------------------------------------------------------------
+
+template <int W, int S, int BS, bool ANY, bool STATS, typename Out>
+__device__ __forceinline__ bool ray_stream(const WNode<W>* __restrict__ nodes,
+                                           const float4* __restrict__ tris, uint32_t root,
+                                           const float4* __restrict__ rays, uint32_t n, uint32_t first,
+                                           uint32_t stride, LaneStack<S, BS>& st, Out out,
+                                           unsigned long long& n_nodes, unsigned long long& n_tris) {
+    bool ok = true;
+    uint32_t ri = first;
+    bool active = ri < n;
+    V3 o = v3(0.f, 0.f, 0.f), d = o, inv = o;
+    float tmin = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+    uint32_t bid = 0xFFFFFFFFu, cur = root;
+    bool hit_any = false, have = false;
+    auto start = [&](uint32_t i) {
+        const float4 r0 = rays[2 * (size_t)i];
+        const float4 r1 = rays[2 * (size_t)i + 1];
+        o = v3(r0.x, r0.y, r0.z);
+        d = v3(r1.x, r1.y, r1.z);
+        tmin = r0.w;
+        best = r1.w;
+        const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+        const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+        const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+        inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+        bid = 0xFFFFFFFFu;
+        bu = bv = 0.f;
+        hit_any = false;
+        st.sp = 0;
+        cur = root;
+        have = true;
+    };
+    if (active) start(ri);
+    while (__any(active)) {
+        if (!active) continue;
+        if (!have) {
+            if (st.sp == 0 || (ANY && hit_any)) {
+                out(ri, bid, best, bu, bv, hit_any);
+                ri += stride;
+                if (ri < n) start(ri);
+                else active = false;
+                continue;
+            }
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) > best) continue;
+            cur = e.x;
+        }
+        have = false;
+        if (child_is_leaf(cur)) {
+            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4 A = tris[3 * (size_t)(f + k)];
+                const float4 B = tris[3 * (size_t)(f + k) + 1];
+                const float4 C = tris[3 * (size_t)(f + k) + 2];
+                if (STATS) n_tris++;
+                float t, u, v;
+                if (!ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), t, u, v)) continue;
+                if (!(t >= tmin && t <= best)) continue;
+                if (ANY) {
+                    hit_any = true;
+                    break;
+                }
+                const uint32_t id = __float_as_uint(A.w);
+                if (t < best || id < bid) {
+                    best = t;
+                    bid = id;
+                    bu = u;
+                    bv = v;
+                }
+            }
+            continue;
+        }
+        if (STATS) n_nodes++;
+        const WNode<W> nd = nodes[cur];
+        float tn[W];
+        uint32_t c[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const float t0x = (nd.lox[j] - o.x) * inv.x, t1x = (nd.hix[j] - o.x) * inv.x;
+            const float t0y = (nd.loy[j] - o.y) * inv.y, t1y = (nd.hiy[j] - o.y) * inv.y;
+            const float t0z = (nd.loz[j] - o.z) * inv.z, t1z = (nd.hiz[j] - o.z) * inv.z;
+            const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+            const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+            c[j] = nd.child[j];
+            const bool hb = (c[j] != kEmpty) && (lo <= hi + fabsf(hi) * kTSlack);
+            tn[j] = hb ? lo : INFINITY;
+            if (!hb) c[j] = kEmpty;
+        }
+        if (!ANY) sort_children<W>(tn, c);
+        uint32_t pend = kEmpty;
+        float pend_t = 0.0f;
+#pragma unroll
+        for (int j = W - 1; j >= 0; j--) {
+            if (c[j] == kEmpty) continue;
+            if (pend != kEmpty) {
+                if (!st.push(pend, pend_t)) ok = false;
+            }
+            pend = c[j];
+            pend_t = tn[j];
+        }
+        if (pend != kEmpty) {
+            cur = pend;
+            have = true;
+        }
+    }
+    return ok;
+}
+
+template <int W, int S, int BS, bool STATS, typename Out>
+__device__ __forceinline__ bool cpq_stream(const WNode<W>* __restrict__ nodes,
+                                           const float4* __restrict__ tris, uint32_t root,
+                                           const float* __restrict__ qs, uint32_t n, uint32_t first,
+                                           uint32_t stride, LaneStack<S, BS>& st, Out out,
+                                           unsigned long long& n_nodes, unsigned long long& n_tris) {
+    bool ok = true;
+    uint32_t qi = first;
+    bool active = qi < n;
+    V3 q = v3(0.f, 0.f, 0.f), bcp = q, bb = q;
+    float best = INFINITY, thr = INFINITY;
+    uint32_t bid = 0xFFFFFFFFu, cur = root;
+    bool have = false;
+    auto start = [&](uint32_t i) {
+        q = v3(qs[3 * (size_t)i], qs[3 * (size_t)i + 1], qs[3 * (size_t)i + 2]);
+        best = thr = INFINITY;
+        bid = 0xFFFFFFFFu;
+        bcp = bb = v3(0.f, 0.f, 0.f);
+        st.sp = 0;
+        cur = root;
+        have = true;
+    };
+    if (active) start(qi);
+    while (__any(active)) {
+        if (!active) continue;
+        if (!have) {
+            if (st.sp == 0) {
+                out(qi, bid, best, bcp, bb);
+                qi += stride;
+                if (qi < n) start(qi);
+                else active = false;
+                continue;
+            }
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) > thr) continue;
+            cur = e.x;
+        }
+        have = false;
+        if (child_is_leaf(cur)) {
+            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4 A = tris[3 * (size_t)(f + k)];
+                const float4 B = tris[3 * (size_t)(f + k) + 1];
+                const float4 C = tris[3 * (size_t)(f + k) + 2];
+                V3 cp, br;
+                const float d2 = closest_pt_tri(q, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), cp, br);
+                const uint32_t id = __float_as_uint(A.w);
+                if (STATS) n_tris++;
+                if (d2 < best || (d2 == best && id < bid)) {
+                    best = d2;
+                    bid = id;
+                    bcp = cp;
+                    bb = br;
+                    thr = best * kCullSlack;
+                }
+            }
+            continue;
+        }
+        if (STATS) n_nodes++;
+        const WNode<W> nd = nodes[cur];
+        float dd[W];
+        uint32_t c[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const float dx = fmaxf(fmaxf(nd.lox[j] - q.x, q.x - nd.hix[j]), 0.0f);
+            const float dy = fmaxf(fmaxf(nd.loy[j] - q.y, q.y - nd.hiy[j]), 0.0f);
+            const float dz = fmaxf(fmaxf(nd.loz[j] - q.z, q.z - nd.hiz[j]), 0.0f);
+            c[j] = nd.child[j];
+            dd[j] = (c[j] == kEmpty) ? INFINITY : fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        }
+        sort_children<W>(dd, c);
+#pragma unroll
+        for (int j = W - 1; j >= 1; j--) {
+            if (c[j] != kEmpty && dd[j] <= thr) {
+                if (!st.push(c[j], dd[j])) ok = false;
+            }
+        }
+        if (c[0] != kEmpty && dd[0] <= thr) {
+            cur = c[0];
+            have = true;
+        }
+    }
+    return ok;
+}
+
+}  // namespace gpurt_impl
