@@ -44,11 +44,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     uint32_t spill_slots, uint32_t* err) {
     const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(nodes_v);
     LaneStack<S, BS> st;
-    st.lane = threadIdx.x;
-    st.spill = spill;
-    st.gid = blockIdx.x * BS + threadIdx.x;
-    st.total = gridDim.x * BS;
-    st.cap = S + spill_slots;
+    st.init(spill, spill_slots, blockIdx.x * BS + threadIdx.x, gridDim.x * BS);
     const uint32_t lane = threadIdx.x & 63u;
     unsigned long long dn = 0, dt = 0;
     for (;;) {

@@ -40,11 +40,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
-    st.lane = threadIdx.x;
-    st.spill = a.spill;
-    st.gid = gid;
-    st.total = total;
-    st.cap = S + a.spill_slots;
+    st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
     auto out = [&](uint32_t qi, uint32_t id, float d2, V3 cp, V3 br) {
         a.prim[qi] = (int32_t)id;
@@ -63,6 +59,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
         atomicAdd(&a.stats[1], n_tris);
+        atomicAdd(&a.stats[2], (unsigned long long)st.spills);
     }
 }
 
@@ -73,11 +70,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
-    st.lane = threadIdx.x;
-    st.spill = a.spill;
-    st.gid = gid;
-    st.total = total;
-    st.cap = S + a.spill_slots;
+    st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
     auto out = [&](uint32_t ri, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
@@ -98,6 +91,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
         atomicAdd(&a.stats[1], n_tris);
+        atomicAdd(&a.stats[2], (unsigned long long)st.spills);
     }
 }
 

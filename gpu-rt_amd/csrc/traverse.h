@@ -20,25 +20,44 @@ namespace gpurt_impl {
 
 // Per-lane traversal stack: slots [0, S) in LDS at lds_stack[slot * BS + lane]
 // (dynamic shared memory of S * BS * 8 bytes, indexed directly so the compiler keeps
-// LDS addressing), deeper slots in the device spill buffer at
-// spill[(slot - S) * total + gid].
+// LDS addressing); deeper slots in the device spill buffer, lane-contiguous (a lane's
+// slot S + k at byte (gid * spill_slots + k) * 8), accessed with cached buffer
+// instructions: a lane's consecutive spills share cache lines, the hardware range check
+// of the buffer descriptor bounds every access, and the buffer intrinsics cannot be
+// merged with the LDS side into one flat access.
 extern __shared__ uint2 lds_stack[];
+
+constexpr int kBufferDword3 = 0x00020000;  // gfx9 raw buffer descriptor word 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t spill_rsrc(void* spill, uint32_t slots, uint32_t total) {
+    const uint64_t bytes = (uint64_t)slots * total * sizeof(uint2);
+    return __builtin_amdgcn_make_buffer_rsrc(spill, 0, (int)(bytes > 0xFFFFFFF0ull ? 0xFFFFFFF0ull : bytes),
+                                             kBufferDword3);
+}
 
 template <int S, int BS>
 struct LaneStack {
-    uint2* spill;
-    uint32_t lane, gid, total, cap;
+    __amdgpu_buffer_rsrc_t spill;
+    uint32_t lane, base, cap;  // base: byte offset of this lane's spill slots
     int sp;
-    // The spill side uses nontemporal builtins: besides not polluting the caches, they
-    // keep the compiler from merging the two sides into one flat (generic) access.
+    uint32_t spills;  // pushes that went to the spill buffer (reported by counting kernels)
+    __device__ __forceinline__ void init(void* spill_buf, uint32_t spill_slots, uint32_t gid, uint32_t total) {
+        spill = spill_rsrc(spill_buf, spill_slots, total);
+        lane = threadIdx.x;
+        base = gid * spill_slots * (uint32_t)sizeof(uint2);
+        cap = S + spill_slots;
+        sp = 0;
+        spills = 0;
+    }
     __device__ __forceinline__ bool push(uint32_t ref, float key) {
         if ((uint32_t)sp >= cap) return false;
         if (sp < S) {
             lds_stack[sp * BS + lane] = make_uint2(ref, __float_as_uint(key));
         } else {
-            uint32_t* p = reinterpret_cast<uint32_t*>(spill + (size_t)(sp - S) * total + gid);
-            __builtin_nontemporal_store(ref, p);
-            __builtin_nontemporal_store(__float_as_uint(key), p + 1);
+            const uint32_t off = base + (uint32_t)(sp - S) * (uint32_t)sizeof(uint2);
+            spills++;
+            __builtin_amdgcn_raw_buffer_store_b32(ref, spill, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(key), spill, off + 4, 0, 0);
         }
         sp++;
         return true;
@@ -46,8 +65,9 @@ struct LaneStack {
     __device__ __forceinline__ uint2 pop() {
         sp--;
         if (sp < S) return lds_stack[sp * BS + lane];
-        const uint32_t* p = reinterpret_cast<const uint32_t*>(spill + (size_t)(sp - S) * total + gid);
-        return make_uint2(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1));
+        const uint32_t off = base + (uint32_t)(sp - S) * (uint32_t)sizeof(uint2);
+        return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(spill, off, 0, 0),
+                          __builtin_amdgcn_raw_buffer_load_b32(spill, off + 4, 0, 0));
     }
 };
 
@@ -192,6 +212,7 @@ __device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
         float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
         inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
     }
+    const float oix = -o.x * inv.x, oiy = -o.y * inv.y, oiz = -o.z * inv.z;
     st.sp = 0;
     uint32_t cur = root;
     bool have = true;
@@ -236,9 +257,9 @@ __device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
         uint32_t c[W];
 #pragma unroll
         for (int j = 0; j < W; j++) {
-            float t0x = (nd.lox[j] - o.x) * inv.x, t1x = (nd.hix[j] - o.x) * inv.x;
-            float t0y = (nd.loy[j] - o.y) * inv.y, t1y = (nd.hiy[j] - o.y) * inv.y;
-            float t0z = (nd.loz[j] - o.z) * inv.z, t1z = (nd.hiz[j] - o.z) * inv.z;
+            const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
+            const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
+            const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
             float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
             float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
             c[j] = nd.child[j];
@@ -298,6 +319,7 @@ __device__ __forceinline__ bool ray_stream(const WNode<W>* __restrict__ nodes,
     uint32_t ri = first;
     bool active = ri < n;
     V3 o = v3(0.f, 0.f, 0.f), d = o, inv = o;
+    float oix = 0.f, oiy = 0.f, oiz = 0.f;
     float tmin = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
     uint32_t bid = 0xFFFFFFFFu, cur = root;
     bool hit_any = false, have = false;
@@ -312,6 +334,9 @@ __device__ __forceinline__ bool ray_stream(const WNode<W>* __restrict__ nodes,
         const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
         const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
         inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+        oix = -o.x * inv.x;
+        oiy = -o.y * inv.y;
+        oiz = -o.z * inv.z;
         bid = 0xFFFFFFFFu;
         bu = bv = 0.f;
         hit_any = false;
@@ -365,9 +390,11 @@ __device__ __forceinline__ bool ray_stream(const WNode<W>* __restrict__ nodes,
         uint32_t c[W];
 #pragma unroll
         for (int j = 0; j < W; j++) {
-            const float t0x = (nd.lox[j] - o.x) * inv.x, t1x = (nd.hix[j] - o.x) * inv.x;
-            const float t0y = (nd.loy[j] - o.y) * inv.y, t1y = (nd.hiy[j] - o.y) * inv.y;
-            const float t0z = (nd.loz[j] - o.z) * inv.z, t1z = (nd.hiz[j] - o.z) * inv.z;
+            // t = (b - o) / d as one FMA with o/d precomputed; its rounding is far below
+            // the build-time box inflation, so the test stays conservative
+            const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
+            const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
+            const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
             const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
             const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
             c[j] = nd.child[j];

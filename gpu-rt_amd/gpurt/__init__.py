@@ -146,12 +146,13 @@ class Bvh:
                                                _stream(self.device)))
 
     def _qopts(self, stack_depth, waves_per_cu, stats):
-        """stats: optional int64 device tensor of 2 counters (nodes visited, triangles
-        tested), accumulated by the counting kernel variant."""
+        """stats: optional int64 device tensor of 4 counters (wide nodes visited,
+        triangles tested, stack spills, reserved), accumulated by the counting kernel
+        variant."""
         if stats is not None:
-            if (stats.dtype != torch.int64 or stats.numel() < 2 or stats.device != self.device
+            if (stats.dtype != torch.int64 or stats.numel() < 4 or stats.device != self.device
                     or not stats.is_contiguous()):
-                raise ValueError("stats must be a contiguous int64 tensor of >= 2 elements "
+                raise ValueError("stats must be a contiguous int64 tensor of >= 4 elements "
                                  "on the BVH's device")
         return QueryOpts(stack_depth, waves_per_cu,
                          ctypes.c_void_p(stats.data_ptr()) if stats is not None else None)

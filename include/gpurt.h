@@ -79,8 +79,9 @@ typedef struct gpurt_query_opts {
     int32_t stack_depth;  /* per-lane LDS stack entries: 8, 16, 32 or 64 (0 = library
                              default); deeper entries spill to a device buffer         */
     int32_t waves_per_cu; /* resident waves per CU for the launch, 0 = library choice  */
-    uint64_t* stats;      /* optional DEVICE uint64[2], accumulated: nodes visited,
-                             triangles tested (slower counting kernel variant)        */
+    uint64_t* stats;      /* optional DEVICE uint64[4], accumulated: wide nodes visited,
+                             triangles tested, stack entries spilled past the LDS
+                             part, reserved (slower counting kernel variant)          */
 } gpurt_query_opts;
 
 typedef struct gpurt_bvh_info {

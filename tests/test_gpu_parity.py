@@ -194,9 +194,9 @@ def test_empty_query_sets(ctx, dev):
 def test_stats_count_visits(ctx, dev):
     w = W.cpq_small(n_queries=4096)
     bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
-    st = torch.zeros(2, dtype=torch.int64, device=dev)
+    st = torch.zeros(4, dtype=torch.int64, device=dev)
     prim, = _np(bvh.closest_point(_t(w["queries"], dev), stats=st)[0])
-    nodes, tris = st.cpu().tolist()
+    nodes, tris, spills, _ = st.cpu().tolist()
     assert nodes >= 4096 and tris >= 4096
     ep, _, _, _ = O.cpq_brute_f32(w["verts"], w["tris"], w["queries"])
     assert np.array_equal(prim, ep)
