@@ -15,6 +15,11 @@
 
 namespace gpurt_impl {
 
+// The offset q - p is re-formed relative to the triangle's first vertex,
+// (q - v0) - (b1 (v1 - v0) + b2 (v2 - v0)), like the forward pass, instead of from the
+// rounded closest point: its error then scales with the triangle, not the coordinates,
+// which keeps the direction accurate for queries very close to the surface.  Without
+// barycentrics it falls back to q - pt.
 __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __restrict__ q,
                                                     const int32_t* __restrict__ prim,
                                                     const float* __restrict__ bary,
@@ -22,8 +27,8 @@ __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __r
                                                     const float* __restrict__ dist,
                                                     const float* __restrict__ g,
                                                     const uint32_t* __restrict__ tri_idx,
-                                                    uint32_t n_tris, float* __restrict__ g_q,
-                                                    float* g_verts) {
+                                                    uint32_t n_tris, const float* __restrict__ verts,
+                                                    float* __restrict__ g_q, float* g_verts) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int32_t p = prim[i];
@@ -33,9 +38,22 @@ __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __r
     const bool live = p >= 0 && (uint32_t)p < n_tris && d > 0.0f && d < INFINITY;
     if (live) {
         const float s = g[i] / d;
-        gx = (q[3 * (size_t)i] - pt[3 * (size_t)i]) * s;
-        gy = (q[3 * (size_t)i + 1] - pt[3 * (size_t)i + 1]) * s;
-        gz = (q[3 * (size_t)i + 2] - pt[3 * (size_t)i + 2]) * s;
+        const V3 qq = v3(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2]);
+        V3 df;
+        if (bary) {
+            const size_t i0 = tri_idx[3 * (size_t)p], i1 = tri_idx[3 * (size_t)p + 1], i2 = tri_idx[3 * (size_t)p + 2];
+            const V3 a = v3(verts[3 * i0], verts[3 * i0 + 1], verts[3 * i0 + 2]);
+            const V3 e1 = sub(v3(verts[3 * i1], verts[3 * i1 + 1], verts[3 * i1 + 2]), a);
+            const V3 e2 = sub(v3(verts[3 * i2], verts[3 * i2 + 1], verts[3 * i2 + 2]), a);
+            const float b1 = bary[3 * (size_t)i + 1], b2 = bary[3 * (size_t)i + 2];
+            const V3 ap = sub(qq, a);
+            df = v3(ap.x - (e1.x * b1 + e2.x * b2), ap.y - (e1.y * b1 + e2.y * b2), ap.z - (e1.z * b1 + e2.z * b2));
+        } else {
+            df = sub(qq, v3(pt[3 * (size_t)i], pt[3 * (size_t)i + 1], pt[3 * (size_t)i + 2]));
+        }
+        gx = df.x * s;
+        gy = df.y * s;
+        gz = df.z * s;
     }
     if (g_q) {
         g_q[3 * (size_t)i] = gx;
@@ -120,7 +138,7 @@ int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim
                    float* g_q, float* g_verts, hipStream_t s) {
     if (n == 0) return GPURT_OK;
     hipLaunchKernelGGL(k_cpq_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim,
-                       bary, pt, dist, g, b->tri_idx, (uint32_t)b->n_tris, g_q, g_verts);
+                       bary, pt, dist, g, b->tri_idx, (uint32_t)b->n_tris, b->verts, g_q, g_verts);
     GPURT_HIP(hipGetLastError());
     return GPURT_OK;
 }

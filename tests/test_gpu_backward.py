@@ -32,6 +32,11 @@ def test_distance_gradient_matches_closed_form(ctx, dev):
     tv.requires_grad_(True)
     bvh = gpurt.Bvh(ctx, tv.detach(), _t(F, dev))
     g = torch.from_numpy(np.random.default_rng(1).uniform(0.5, 2.0, len(Q)).astype(np.float32)).to(dev)
+    # the unsigned distance has a kink on the surface: queries within 1e-4 of it have an
+    # ill-conditioned FP32 direction, so their weight is zero here (both sides)
+    with torch.no_grad():
+        d0 = gpurt.closest_point_distance(bvh, tq.detach(), tv.detach())[0]
+    g = g * (d0 > 1e-4).float()
     dist, prim = gpurt.closest_point_distance(bvh, tq, tv)
     (dist * g).sum().backward()
     torch.cuda.synchronize()
@@ -48,7 +53,9 @@ def test_distance_gradient_matches_closed_form(ctx, dev):
     d = dist.detach().cpu().numpy()
     ok = d > 1e-4
     np.testing.assert_allclose(gq[ok], eq[ok], atol=2e-3 * 2.0, rtol=1e-3)
-    np.testing.assert_allclose(np.linalg.norm(gq[ok] / g.cpu().numpy()[ok, None], axis=1), 1.0, atol=1e-4)
+    # unit direction; the bound allows FP32 rounding of an offset of length d
+    nrm = np.linalg.norm(gq[ok] / g.cpu().numpy()[ok, None], axis=1)
+    assert np.all(np.abs(nrm - 1.0) <= 1e-5 + 2e-9 / d[ok])
     # vertex gradients: sums of ~10 FP32 atomic contributions per vertex
     np.testing.assert_allclose(gv, ev, atol=5e-3, rtol=1e-3)
 
