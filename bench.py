@@ -31,7 +31,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-pt queries/s + rays/s, 1M-tri, fwd & fwd+bwd, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = {2: 64, 4: 128, 8: 256}
 TRI_BYTES = 36  # three float3 vertices per triangle tested
 
 
@@ -54,21 +53,25 @@ def timed(fn, steps, warmup, stream=None):
     return s.elapsed_time(e) / steps  # ms
 
 
-def measured_traffic(args, n_tris, n_rays):
+def measured_traffic(args, n_tris, n_rays, node_format):
     """HBM bytes per closest-hit launch from the committed counter summary
     (profiles/rays_scene_traffic.json: FETCH_SIZE and WRITE_SIZE passes, each scaled by
-    what the same counter reads for a 256 MiB device copy), when it was measured on this
-    exact configuration; None otherwise."""
+    what the same counter reads for an elementwise kernel that reads and writes 256 MiB),
+    when it was measured on this exact configuration; None otherwise."""
     path = os.path.join(ROOT, "profiles", "rays_scene_traffic.json")
     try:
         m = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if (m["n_tris"], m["n_rays"], m["node_width"], m["stack_depth"], m["treelet_size"]) != \
-            (n_tris, n_rays, args.width, args.stack, args.treelet):
+    keys = ("n_tris", "n_rays", "node_width", "stack_depth", "treelet_size", "node_format",
+            "fetch_kb_raw", "write_kb_raw", "calib_kib", "calib_fetch_kb_raw", "calib_write_kb_raw")
+    if any(k not in m for k in keys):
         return None
-    rd = m["fetch_kb_raw"] * m["copy_kib"] / m["copy_fetch_kb_raw"]
-    wr = m["write_kb_raw"] * m["copy_kib"] / m["copy_write_kb_raw"]
+    if (m["n_tris"], m["n_rays"], m["node_width"], m["stack_depth"], m["treelet_size"],
+            m["node_format"]) != (n_tris, n_rays, args.width, args.stack, args.treelet, node_format):
+        return None
+    rd = m["fetch_kb_raw"] * m["calib_kib"] / m["calib_fetch_kb_raw"]
+    wr = m["write_kb_raw"] * m["calib_kib"] / m["calib_write_kb_raw"]
     return (rd + wr) * 1024.0
 
 
@@ -84,7 +87,8 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     n = rays.shape[0]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
+                    node_format=args.format)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
     log(f"[bench] GPU build {build_ms:.1f} ms, {info}")
@@ -105,7 +109,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, stats=st)
     torch.cuda.synchronize()
     nodes, tris, spills, _ = st.cpu().tolist()
-    bytes_closest = n * (32 + 16) + nodes * NODE_BYTES[args.width] + tris * TRI_BYTES
+    bytes_closest = n * (32 + 16) + nodes * info["node_bytes"] + tris * TRI_BYTES
     st.zero_()
     bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, stats=st)
     torch.cuda.synchronize()
@@ -134,7 +138,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         passes += 1
     cpu_rate = 2 * n * passes / cpu_s
     roof = {
-        "kernel": "k_ray<W=%d,closest>" % args.width,
+        "kernel": "k_ray<W=%d,%s,closest>" % (args.width, "q8" if info["node_format"] == 2 else "f32"),
         "bound": "hbm",
         "achieved": bytes_closest / (ms_closest * 1e-3) / 1e9,
         "peak": HBM_PEAK_GBS,
@@ -147,13 +151,14 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "kernel_ms": ms_closest,
     }
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic(args, int(F.shape[0]), n)
+    roof["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
     res = {
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
         "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
                    "passes": ["closest_hit", "any_hit"], "node_width": args.width,
-                   "leaf_size": 4, "stack_depth": args.stack, "treelet_size": args.treelet},
+                   "leaf_size": 4, "stack_depth": args.stack, "treelet_size": args.treelet,
+                   "node_format": info["node_format"]},
         "roofline": roof,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
                          "sample": f"{passes} full passes of the {n} closest-hit + {n} any-hit "
@@ -178,7 +183,8 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
     q = torch.from_numpy(Q).to(dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
+                    node_format=args.format)
     build_ms = (time.perf_counter() - t0) * 1e3
     n = q.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
@@ -188,7 +194,7 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
     bvh.closest_point(q, out=out, stack_depth=args.stack, stats=st)
     torch.cuda.synchronize()
     nodes, tris, _, _ = st.cpu().tolist()
-    byts = n * (12 + 20 + 12) + nodes * NODE_BYTES[args.width] + tris * TRI_BYTES
+    byts = n * (12 + 20 + 12) + nodes * bvh.info()["cpq_node_bytes"] + tris * TRI_BYTES
     del bvh
     return {"workload": name, "n_tris": int(F.shape[0]), "n_queries": n,
             "queries_per_s": n / (ms * 1e-3), "ms": ms, "gpu_build_ms": build_ms,
@@ -204,7 +210,8 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     q0 = torch.from_numpy(w["queries"]).to(dev)
     o0 = torch.from_numpy(w["ray_o"]).to(dev)
     d0 = torch.from_numpy(w["ray_d"]).to(dev)
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
+                    node_format=args.format)
     nq, nr = q0.shape[0], o0.shape[0]
 
     def fwd():
@@ -243,7 +250,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
-                        treelet_size=args.batch_treelet)
+                        treelet_size=args.batch_treelet, node_format=args.format)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
     res = batch.run()
@@ -253,7 +260,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     def plain():
         for i, m in enumerate(meshes):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width,
-                            treelet_size=args.batch_treelet)
+                            treelet_size=args.batch_treelet, node_format=args.format)
             p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack)
             rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack)
             plain_out[i] = (p, d, rp, rt)
@@ -278,13 +285,15 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
 def profile_mode(args, torch, gpurt, W, dev, ctx):
     """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
-    rays_scene, then the same number of 256 MiB device copies (a plain copy kernel of
-    known size, whose FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
+    rays_scene, then the same number of elementwise adds over 256 MiB (a plain streaming
+    kernel that reads 256 MiB and writes 256 MiB, whose FETCH_SIZE/WRITE_SIZE calibrate
+    the counters)."""
     w = W.rays_scene()
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
     rays = torch.from_numpy(w["rays"]).to(dev)
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet)
+    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
+                    node_format=args.format)
     n = rays.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
            torch.empty((n, 2), device=dev))
@@ -293,10 +302,11 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(args.profile_reps):
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)
     torch.cuda.synchronize()
     print(json.dumps({"profile": "rays_scene closest-hit", "launches": args.profile_reps,
-                      "rays": n, "copy_bytes": src.numel() * 4}), flush=True)
+                      "rays": n, "calib_bytes_read": src.numel() * 4,
+                      "calib_bytes_written": dst.numel() * 4}), flush=True)
 
 
 def main():
@@ -311,6 +321,9 @@ def main():
                          "treelet), 0 = off; the batched leg rebuilds every step and uses "
                          "--batch-treelet")
     ap.add_argument("--batch-treelet", type=int, default=0, choices=[0, 5, 6, 7])
+    ap.add_argument("--format", type=int, default=0, choices=[0, 1, 2],
+                    help="node encoding: 0 = library default, 1 = FP32 child boxes, "
+                         "2 = 8-bit child boxes")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: repeat full passes until this much CPU time")
     ap.add_argument("--no-extra", action="store_true")

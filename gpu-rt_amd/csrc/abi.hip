@@ -50,6 +50,31 @@ int compact_verts(const float* verts, size_t stride_floats, size_t n, float* out
     return GPURT_OK;
 }
 
+// node format used when gpurt_build_opts::node_format is 0
+constexpr int kDefaultFormat = kFormatMixed;
+
+int parse_build_opts(const gpurt_build_opts* o, BuildParams& p) {
+    p = BuildParams{};
+    if (o) {
+        if (o->node_width) p.width = o->node_width;
+        if (o->leaf_size) p.leaf_size = o->leaf_size;
+        p.treelet_size = o->treelet_size;
+        p.treelet_rounds = o->treelet_rounds;
+        p.format = o->node_format;
+    }
+    if (p.width != 2 && p.width != 4 && p.width != 8)
+        return set_error(GPURT_E_UNSUPPORTED, "node_width must be 2, 4 or 8");
+    if (p.leaf_size < 1 || p.leaf_size > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
+    if (p.treelet_size != 0 && (p.treelet_size < 5 || p.treelet_size > 7))
+        return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
+    if (p.treelet_rounds < 0 || p.treelet_rounds > 1)
+        return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0 or 1");
+    if (p.treelet_size && p.treelet_rounds == 0) p.treelet_rounds = 1;
+    if (p.format < 0 || p.format > 3) return set_error(GPURT_E_UNSUPPORTED, "node_format must be 0..3");
+    if (p.format == 0) p.format = kDefaultFormat;
+    return GPURT_OK;
+}
+
 }  // namespace gpurt_impl
 
 using namespace gpurt_impl;
@@ -116,26 +141,6 @@ static int check_mesh(const float* verts, size_t vstride_bytes, size_t n_verts,
     return GPURT_OK;
 }
 
-static int check_build_opts(const gpurt_build_opts* o, int& W, int& leaf, int& tsize, int& trounds) {
-    W = 4;
-    leaf = 4;
-    tsize = 0;
-    trounds = 0;
-    if (o) {
-        if (o->node_width) W = o->node_width;
-        if (o->leaf_size) leaf = o->leaf_size;
-        tsize = o->treelet_size;
-        trounds = o->treelet_rounds;
-    }
-    if (W != 2 && W != 4 && W != 8) return set_error(GPURT_E_UNSUPPORTED, "node_width must be 2, 4 or 8");
-    if (leaf < 1 || leaf > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
-    if (tsize != 0 && (tsize < 5 || tsize > 7))
-        return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
-    if (trounds < 0 || trounds > 1) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0 or 1");
-    if (tsize && trounds == 0) trounds = 1;
-    return GPURT_OK;
-}
-
 int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, size_t n_verts,
                     const uint32_t* tri_idx, size_t n_tris, const gpurt_build_opts* opts,
                     void* stream, gpurt_bvh* out) {
@@ -143,8 +148,8 @@ int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, siz
     *out = nullptr;
     int rc = check_mesh(verts, vstride_bytes, n_verts, tri_idx, n_tris);
     if (rc) return rc;
-    int W, leaf, tsize, trounds;
-    if ((rc = check_build_opts(opts, W, leaf, tsize, trounds))) return rc;
+    BuildParams bp;
+    if ((rc = parse_build_opts(opts, bp))) return rc;
     DeviceGuard g(ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     hipStream_t s = (hipStream_t)stream;
@@ -159,7 +164,7 @@ int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, siz
     auto* b = new gpurt_bvh_s();
     b->ctx = ctx;
     b->n_verts = (int64_t)n_verts;
-    rc = build_segments(ctx, &sd, 1, W, leaf, tsize, trounds, s, b);
+    rc = build_segments(ctx, &sd, 1, bp.width, bp.leaf_size, bp.treelet_size, bp.treelet_rounds, bp.format, s, b);
     if (rc) {
         delete b;
         return rc;
@@ -207,6 +212,7 @@ void gpurt_bvh_destroy(gpurt_bvh b) {
     DeviceGuard g(b->ctx->device);
     (void)hipDeviceSynchronize();
     if (b->nodes) (void)hipFree(b->nodes);
+    if (b->qnodes) (void)hipFree(b->qnodes);
     if (b->tris) (void)hipFree(b->tris);
     if (b->tri_idx) (void)hipFree(b->tri_idx);
     if (b->verts) (void)hipFree(b->verts);
@@ -220,12 +226,14 @@ int gpurt_bvh_get_info(gpurt_bvh b, gpurt_bvh_info* out) {
     out->n_tris = b->n_tris;
     out->n_verts = b->n_verts;
     out->n_nodes = b->n_nodes;
-    out->node_bytes = (int64_t)node_bytes(b->width);
+    out->node_bytes = (int64_t)(rays_use_q8(b->format) ? qnode_bytes(b->width) : node_bytes(b->width));
+    out->cpq_node_bytes = (int32_t)(cpq_use_q8(b->format) ? qnode_bytes(b->width) : node_bytes(b->width));
     out->tri_bytes = 48;
     out->depth = b->depth;
     out->node_width = b->width;
     out->leaf_size = b->leaf_size;
     out->sah_cost = b->sah_cost;
+    out->node_format = b->format;
     return GPURT_OK;
 }
 

@@ -2,13 +2,14 @@
 //
 // The builder is segmented by construction (the mesh id sits in the high bits of every
 // Morton key, so each mesh is its own subtree of one radix tree), so the BVHs of all
-// meshes are built by the same ~30 launches as one mesh.  Queries then run in ONE
-// persistent launch: every wave pulls a 64-query chunk (closest-point or closest-hit,
-// so the branch is wave-uniform) from a device work counter until the chunk list is
-// exhausted, and starts at its mesh's root (wide node m of the shared node array).
-// Chunks are ordered by decreasing mesh size so the tail of the launch is made of
-// cheap chunks.  The whole sequence -- build of every mesh plus the query launch --
-// is recorded once into a HIP graph and replayed by gpurt_batch_run.
+// meshes are built by the same ~30 launches as one mesh.  Queries then run in two
+// persistent launches (closest points, then rays) of the streaming traversal: a wave
+// takes 64-query chunks from a device work counter and hands their queries to its lanes
+// as lanes finish, until the chunk list is exhausted; a query starts at its mesh's root
+// (wide node m of the shared node array).  Chunks are ordered by decreasing mesh size so
+// the tail of a launch is made of cheap chunks.  The whole sequence -- build of every
+// mesh plus both query launches -- is recorded once into a HIP graph and replayed by
+// gpurt_batch_run.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -31,85 +32,155 @@ struct BatchMeshDev {
     uint32_t n_r;
 };
 
-constexpr uint32_t kRayChunk = 0x80000000u;  // chunk.x bit 31: ray chunk (else closest point)
-
-__global__ void k_reset_counter(uint32_t* c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+__global__ void k_reset_counters(uint32_t* c) {
+    if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0u;
 }
 
-template <int W, int S, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_batch_query(
-    const void* nodes_v, const float4* __restrict__ tris, const BatchMeshDev* __restrict__ meshes,
-    const uint2* __restrict__ chunks, uint32_t n_chunks, uint32_t* counter, uint2* spill,
-    uint32_t spill_slots, uint32_t* err) {
-    const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(nodes_v);
-    LaneStack<S, BS> st;
-    st.init(spill, spill_slots, blockIdx.x * BS + threadIdx.x, gridDim.x * BS);
-    const uint32_t lane = threadIdx.x & 63u;
-    unsigned long long dn = 0, dt = 0;
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = gatomic_add(counter, 1u);
-        // wave-uniform from here on: the chunk and mesh descriptors live in SGPRs
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= n_chunks) break;
-        const uint2 ch = chunks[c];
-        const uint32_t m = ch.x & ~kRayChunk;
-        const uint32_t i = ch.y + lane;
-        const BatchMeshDev md = meshes[m];
-        if (ch.x & kRayChunk) {
-            if (i < md.n_r) {
-                const float4 r0 = md.rays[2 * (size_t)i];
-                const float4 r1 = md.rays[2 * (size_t)i + 1];
-                RayHit h;
-                if (!ray_one<W, S, BS, false, false>(nodes, tris, m, v3(r0.x, r0.y, r0.z),
-                                                     v3(r1.x, r1.y, r1.z), r0.w, r1.w, st, h, dn, dt))
-                    gatomic_or(err, 1u);
-                const bool hit = h.id != 0xFFFFFFFFu;
-                md.r_prim[i] = hit ? (int32_t)h.id : -1;
-                md.r_t[i] = hit ? h.t : INFINITY;
-                if (md.r_uv) {
-                    md.r_uv[2 * (size_t)i] = hit ? h.u : 0.0f;
-                    md.r_uv[2 * (size_t)i + 1] = hit ? h.v : 0.0f;
+// Query source over a chunk list (traverse.h): a wave takes the next chunk (mesh m,
+// first query) from the device counter when its pool is empty and hands the chunk's
+// queries to the lanes that want one, in lane order.  Called by all lanes together, so
+// the pool stays wave-uniform.
+struct ChunkSource {
+    struct Handle {
+        uint32_t m, i;
+    };
+    const BatchMeshDev* meshes;
+    const uint2* chunks;
+    uint32_t n_chunks;
+    uint32_t* counter;
+    bool rays;
+    uint32_t pm = 0, pnext = 0, pend = 0;  // current chunk: mesh and [pnext, pend)
+    bool more = true;
+    __device__ __forceinline__ bool refill(bool has, Handle& h) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+        bool got = false;
+        uint64_t want = __ballot(!has);
+        while (want && more) {
+            if (pnext >= pend) {
+                uint32_t c = 0;
+                if (lane == 0) c = gatomic_add(counter, 1u);
+                c = __builtin_amdgcn_readfirstlane(c);
+                if (c >= n_chunks) {
+                    more = false;
+                    break;
                 }
+                const uint2 ch = chunks[c];
+                pm = ch.x;
+                pnext = ch.y;
+                const uint32_t nq = rays ? meshes[pm].n_r : meshes[pm].n_q;
+                pend = min(ch.y + 64u, nq);
             }
-        } else {
-            if (i < md.n_q) {
-                const V3 q = v3(md.q[3 * (size_t)i], md.q[3 * (size_t)i + 1], md.q[3 * (size_t)i + 2]);
-                CpqHit h;
-                if (!cpq_one<W, S, BS, false>(nodes, tris, m, q, st, h, dn, dt)) gatomic_or(err, 1u);
-                md.q_prim[i] = (int32_t)h.id;
-                md.q_pt[3 * (size_t)i] = h.cp.x;
-                md.q_pt[3 * (size_t)i + 1] = h.cp.y;
-                md.q_pt[3 * (size_t)i + 2] = h.cp.z;
-                md.q_dist[i] = (h.id != 0xFFFFFFFFu) ? sqrtf(h.d2) : INFINITY;
-                if (md.q_bary) {
-                    md.q_bary[3 * (size_t)i] = h.bary.x;
-                    md.q_bary[3 * (size_t)i + 1] = h.bary.y;
-                    md.q_bary[3 * (size_t)i + 2] = h.bary.z;
-                }
+            const uint32_t k = min(pend - pnext, (uint32_t)__popcll(want));
+            const uint32_t rank = (uint32_t)__popcll(want & below);
+            if (!has && !got && rank < k) {
+                h.m = pm;
+                h.i = pnext + rank;
+                got = true;
             }
+            pnext += k;
+            want = __ballot(!has && !got);
         }
+        return got;
     }
+    __device__ __forceinline__ uint32_t root(const Handle& h) const { return h.m; }
+    __device__ __forceinline__ void ray(const Handle& h, float4& r0, float4& r1) const {
+        const float4* r = meshes[h.m].rays;
+        r0 = r[2 * (size_t)h.i];
+        r1 = r[2 * (size_t)h.i + 1];
+    }
+    __device__ __forceinline__ V3 point(const Handle& h) const {
+        const float* q = meshes[h.m].q;
+        return v3(q[3 * (size_t)h.i], q[3 * (size_t)h.i + 1], q[3 * (size_t)h.i + 2]);
+    }
+};
+
+struct BatchArgs {
+    const void* nodes;
+    const float4* tris;
+    const BatchMeshDev* meshes;
+    const uint2* chunks;
+    uint32_t n_chunks;
+    uint32_t* counter;
+    uint2* spill;
+    uint32_t spill_slots;
+    uint32_t* err;
+};
+
+template <int W, bool Q, int S, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W, Q>::value))) void k_batch_cpq(
+    BatchArgs a) {
+    const NodeOf<W, Q>* __restrict__ nodes = reinterpret_cast<const NodeOf<W, Q>*>(a.nodes);
+    LaneStack<S, BS> st;
+    st.init(a.spill, a.spill_slots, blockIdx.x * BS + threadIdx.x, gridDim.x * BS);
+    ChunkSource src{a.meshes, a.chunks, a.n_chunks, a.counter, false};
+    auto out = [&](const ChunkSource::Handle& h, uint32_t id, float d2, V3 cp, V3 br) {
+        const BatchMeshDev& md = a.meshes[h.m];
+        const size_t i = h.i;
+        md.q_prim[i] = (int32_t)id;
+        md.q_pt[3 * i] = cp.x;
+        md.q_pt[3 * i + 1] = cp.y;
+        md.q_pt[3 * i + 2] = cp.z;
+        md.q_dist[i] = (id != 0xFFFFFFFFu) ? sqrtf(d2) : INFINITY;
+        if (md.q_bary) {
+            md.q_bary[3 * i] = br.x;
+            md.q_bary[3 * i + 1] = br.y;
+            md.q_bary[3 * i + 2] = br.z;
+        }
+    };
+    unsigned long long dn = 0, dt = 0;
+    if (!cpq_stream<NodeOf<W, Q>, S, BS, false>(nodes, a.tris, src, st, out, dn, dt)) gatomic_or(a.err, 1u);
 }
 
-template <int W, int S, int BS>
-static void launch_batch_query(unsigned grid, hipStream_t s, const void* nodes, const float4* tris,
-                               const BatchMeshDev* meshes, const uint2* chunks, uint32_t n_chunks,
-                               uint32_t* counter, uint2* spill, uint32_t spill_slots, uint32_t* err) {
-    hipLaunchKernelGGL((k_batch_query<W, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, nodes, tris, meshes, chunks,
-                       n_chunks, counter, spill, spill_slots, err);
+template <int W, bool Q, int S, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W, Q>::value))) void k_batch_ray(
+    BatchArgs a) {
+    const NodeOf<W, Q>* __restrict__ nodes = reinterpret_cast<const NodeOf<W, Q>*>(a.nodes);
+    LaneStack<S, BS> st;
+    st.init(a.spill, a.spill_slots, blockIdx.x * BS + threadIdx.x, gridDim.x * BS);
+    ChunkSource src{a.meshes, a.chunks, a.n_chunks, a.counter, true};
+    auto out = [&](const ChunkSource::Handle& h, uint32_t id, float t, float u, float v, bool) {
+        const BatchMeshDev& md = a.meshes[h.m];
+        const size_t i = h.i;
+        const bool hit = id != 0xFFFFFFFFu;
+        md.r_prim[i] = hit ? (int32_t)id : -1;
+        md.r_t[i] = hit ? t : INFINITY;
+        if (md.r_uv) {
+            md.r_uv[2 * i] = hit ? u : 0.0f;
+            md.r_uv[2 * i + 1] = hit ? v : 0.0f;
+        }
+    };
+    unsigned long long dn = 0, dt = 0;
+    if (!ray_stream<NodeOf<W, Q>, S, BS, false, false>(nodes, a.tris, src, st, out, dn, dt)) gatomic_or(a.err, 1u);
 }
 
-template <int W>
-static int launch_batch_w(int S, unsigned grid, hipStream_t s, const void* nodes, const float4* tris,
-                          const BatchMeshDev* meshes, const uint2* chunks, uint32_t n_chunks,
-                          uint32_t* counter, uint2* spill, uint32_t spill_slots, uint32_t* err) {
+template <int W, bool Q, int S, int BS>
+static void launch_batch_kind(bool rays, unsigned grid, hipStream_t s, const BatchArgs& a) {
+    if (rays)
+        hipLaunchKernelGGL((k_batch_ray<W, Q, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+    else
+        hipLaunchKernelGGL((k_batch_cpq<W, Q, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+}
+
+template <int W, bool Q>
+static int launch_batch_w(int S, bool rays, unsigned grid, hipStream_t s, const BatchArgs& a) {
     switch (S) {
-        case 8: launch_batch_query<W, 8, 256>(grid, s, nodes, tris, meshes, chunks, n_chunks, counter, spill, spill_slots, err); return 256;
-        case 16: launch_batch_query<W, 16, 256>(grid, s, nodes, tris, meshes, chunks, n_chunks, counter, spill, spill_slots, err); return 256;
-        case 32: launch_batch_query<W, 32, 128>(grid, s, nodes, tris, meshes, chunks, n_chunks, counter, spill, spill_slots, err); return 128;
-        case 64: launch_batch_query<W, 64, 64>(grid, s, nodes, tris, meshes, chunks, n_chunks, counter, spill, spill_slots, err); return 64;
+        case 8: launch_batch_kind<W, Q, 8, 256>(rays, grid, s, a); return 256;
+        case 16: launch_batch_kind<W, Q, 16, 256>(rays, grid, s, a); return 256;
+        case 32: launch_batch_kind<W, Q, 32, 128>(rays, grid, s, a); return 128;
+        case 64: launch_batch_kind<W, Q, 64, 64>(rays, grid, s, a); return 64;
+    }
+    return 0;
+}
+
+static int launch_batch(int W, bool q8, int S, bool rays, unsigned grid, hipStream_t s, const BatchArgs& a) {
+    switch (W * 2 + (q8 ? 1 : 0)) {
+        case 4: return launch_batch_w<2, false>(S, rays, grid, s, a);
+        case 5: return launch_batch_w<2, true>(S, rays, grid, s, a);
+        case 8: return launch_batch_w<4, false>(S, rays, grid, s, a);
+        case 9: return launch_batch_w<4, true>(S, rays, grid, s, a);
+        case 16: return launch_batch_w<8, false>(S, rays, grid, s, a);
+        case 17: return launch_batch_w<8, true>(S, rays, grid, s, a);
     }
     return 0;
 }
@@ -122,12 +193,13 @@ using namespace gpurt_impl;
 
 struct gpurt_batch_s {
     gpurt_ctx ctx = nullptr;
-    int width = 4, leaf_size = 4, stack = 8;
+    int width = 4, leaf_size = 4, stack = 8, block = 256, format = kFormatF32;
     BuildBufs build;
+    void* qnodes = nullptr;  // QNode<W>[build.capacity] when has_qnodes(format)
     BatchMeshDev* meshes_d = nullptr;
-    uint2* chunks_d = nullptr;
-    uint32_t n_chunks = 0;
-    uint32_t* counter_d = nullptr;
+    uint2* chunks_d = nullptr;    // closest-point chunks, then ray chunks
+    uint32_t n_cpq_chunks = 0, n_ray_chunks = 0;
+    uint32_t* counter_d = nullptr;  // [0]: closest-point chunks taken, [1]: ray chunks
     uint2* spill = nullptr;
     uint32_t spill_slots = 0;
     unsigned grid = 0;
@@ -142,6 +214,7 @@ struct gpurt_batch_s {
         if (chunks_d) (void)hipFree(chunks_d);
         if (counter_d) (void)hipFree(counter_d);
         if (spill) (void)hipFree(spill);
+        if (qnodes) (void)hipFree(qnodes);
     }
 };
 
@@ -164,14 +237,29 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
     if ((rc = build_enqueue_binary(b->build, treelet_size, treelet_rounds, s))) return rc;
     // all levels: empty levels exit at once, and the tree cannot be deeper (kMaxLevels)
     if ((rc = build_enqueue_levels(b->build, b->leaf_size, 0, kMaxLevels, s))) return rc;
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, b->counter_d);
-    int bs = 0;
-    switch (b->width) {
-        case 2: bs = launch_batch_w<2>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
-        case 4: bs = launch_batch_w<4>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
-        case 8: bs = launch_batch_w<8>(b->stack, b->grid, s, b->build.nodes, b->build.final_tris(), b->meshes_d, b->chunks_d, b->n_chunks, b->counter_d, b->spill, b->spill_slots, b->ctx->err); break;
+    if (has_qnodes(b->format)) {
+        // counters[0] holds the wide-node count the levels produced
+        launch_quantize(b->width, b->build.nodes, b->qnodes, 0, b->build.capacity, b->build.counters, s);
     }
-    if (!bs) return set_error(GPURT_E_UNSUPPORTED, "batch stack/width");
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->counter_d);
+    BatchArgs a{};
+    a.tris = b->build.final_tris();
+    a.meshes = b->meshes_d;
+    a.spill = b->spill;
+    a.spill_slots = b->spill_slots;
+    a.err = b->ctx->err;
+    for (int rays = 0; rays < 2; rays++) {
+        const uint32_t n = rays ? b->n_ray_chunks : b->n_cpq_chunks;
+        if (n == 0) continue;
+        const bool q8 = rays ? rays_use_q8(b->format) : cpq_use_q8(b->format);
+        a.nodes = q8 ? b->qnodes : b->build.nodes;
+        a.chunks = b->chunks_d + (rays ? b->n_cpq_chunks : 0u);
+        a.n_chunks = n;
+        a.counter = b->counter_d + rays;
+        const unsigned grid = std::max<size_t>(1, std::min<size_t>(b->grid, (size_t)n * 64 / b->block + 1));
+        if (!launch_batch(b->width, q8, b->stack, rays != 0, grid, s, a))
+            return set_error(GPURT_E_UNSUPPORTED, "batch stack/width");
+    }
     GPURT_HIP(hipGetLastError());
     return GPURT_OK;
 }
@@ -186,19 +274,10 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     *out = nullptr;
     if (!ctx || !meshes) return set_error(GPURT_E_INVALID, "null context or mesh array");
     if (n_meshes == 0 || n_meshes > (1u << 20)) return set_error(GPURT_E_INVALID, "n_meshes must be 1..2^20");
-    int W = 4, leaf = 4, tsize = 0, trounds = 0;
-    if (bopts) {
-        if (bopts->node_width) W = bopts->node_width;
-        if (bopts->leaf_size) leaf = bopts->leaf_size;
-        tsize = bopts->treelet_size;
-        trounds = bopts->treelet_rounds;
-    }
-    if (W != 2 && W != 4 && W != 8) return set_error(GPURT_E_UNSUPPORTED, "node_width must be 2, 4 or 8");
-    if (leaf < 1 || leaf > 8) return set_error(GPURT_E_UNSUPPORTED, "leaf_size must be 1..8");
-    if (tsize != 0 && (tsize < 5 || tsize > 7))
-        return set_error(GPURT_E_UNSUPPORTED, "treelet_size must be 0 or 5..7");
-    if (trounds < 0 || trounds > 1) return set_error(GPURT_E_UNSUPPORTED, "treelet_rounds must be 0 or 1");
-    if (tsize && !trounds) trounds = 1;
+    BuildParams bp;
+    int rc0 = parse_build_opts(bopts, bp);
+    if (rc0) return rc0;
+    const int W = bp.width, leaf = bp.leaf_size, tsize = bp.treelet_size, trounds = bp.treelet_rounds;
     int S = (qopts && qopts->stack_depth) ? qopts->stack_depth : (W == 8 ? 16 : 8);
     if (S != 8 && S != 16 && S != 32 && S != 64)
         return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 8, 16, 32 or 64");
@@ -242,6 +321,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->width = W;
     b->leaf_size = leaf;
     b->stack = S;
+    b->format = bp.format;
     int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {
         delete b;
@@ -253,26 +333,31 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return segs[x].n_tris > segs[y].n_tris; });
     std::vector<uint2> chunks;
-    for (uint32_t m : order) {
+    for (uint32_t m : order)
         for (uint32_t st = 0; st < md[m].n_q; st += 64) chunks.push_back(make_uint2(m, st));
-        for (uint32_t st = 0; st < md[m].n_r; st += 64) chunks.push_back(make_uint2(m | kRayChunk, st));
-    }
-    b->n_chunks = (uint32_t)chunks.size();
+    b->n_cpq_chunks = (uint32_t)chunks.size();
+    for (uint32_t m : order)
+        for (uint32_t st = 0; st < md[m].n_r; st += 64) chunks.push_back(make_uint2(m, st));
+    b->n_ray_chunks = (uint32_t)chunks.size() - b->n_cpq_chunks;
     const int BS = block_for_stack_b(S);
+    b->block = BS;
     const int waves_per_block = BS / 64;
-    const int reg_waves = 4 * WavesHint<4>::value;
-    int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8),
-                                 std::min(32, W == 8 ? 4 * WavesHint<8>::value : reg_waves) / waves_per_block);
+    // register-limited waves: the larger of the two kernels' needs (FP32 8-wide nodes
+    // need the most registers)
+    const bool any_f32 = !rays_use_q8(bp.format) || !cpq_use_q8(bp.format);
+    const int hint = W == 8 ? (any_f32 ? WavesHint<8, false>::value : WavesHint<8, true>::value)
+                            : WavesHint<4, false>::value;
+    int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8), std::min(32, 4 * hint) / waves_per_block);
     if (qopts && qopts->waves_per_cu > 0) blocks_per_cu = std::max(1, qopts->waves_per_cu / waves_per_block);
     blocks_per_cu = std::max(1, blocks_per_cu);
-    b->grid = (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)ctx->n_cu * blocks_per_cu,
-                                                             (size_t)b->n_chunks * 64 / BS + 1));
+    b->grid = (unsigned)std::max<size_t>(1, (size_t)ctx->n_cu * blocks_per_cu);
     // deepest possible wide tree (kMaxLevels levels) bounds the stack: no overflow
     const int need = (W - 1) * kMaxLevels + 1;
     b->spill_slots = need > S ? (uint32_t)(need - S) : 0u;
     bool fail = hipMalloc((void**)&b->meshes_d, sizeof(BatchMeshDev) * n_meshes) != hipSuccess ||
                 hipMalloc((void**)&b->chunks_d, sizeof(uint2) * std::max<size_t>(1, chunks.size())) != hipSuccess ||
                 hipMalloc((void**)&b->counter_d, 256) != hipSuccess ||
+                (has_qnodes(bp.format) && hipMalloc(&b->qnodes, qnode_bytes(W) * std::max<uint32_t>(1u, b->build.capacity)) != hipSuccess) ||
                 (b->spill_slots &&
                  hipMalloc((void**)&b->spill, (size_t)b->spill_slots * b->grid * BS * sizeof(uint2)) != hipSuccess);
     if (fail) {

@@ -935,7 +935,7 @@ int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level
 // Plain (single-mesh) build: enqueue, run the collapse in groups of levels until the
 // tree ends (host checks between groups), then hand the products to *out.
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, int leaf_size,
-                   int treelet_size, int treelet_rounds, hipStream_t s, gpurt_bvh_s* out) {
+                   int treelet_size, int treelet_rounds, int format, hipStream_t s, gpurt_bvh_s* out) {
     (void)ctx;
     BuildBufs b;
     int rc;
@@ -1002,6 +1002,12 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
         sah = (b.N > 1 && root_area > 0.0) ? sah / root_area : 0.0;
     }
     out->nodes = nodes;
+    out->format = format;
+    if (has_qnodes(format)) {
+        GPURT_HIP(hipMalloc(&out->qnodes, qnode_bytes(W) * std::max<uint32_t>(n_nodes, 1u)));
+        launch_quantize(W, nodes, out->qnodes, n_nodes, n_nodes, nullptr, s);
+        GPURT_HIP(hipGetLastError());
+    }
     if (b.repacked && h_counters[2] != b.N) return set_error(GPURT_E_DEVICE, "leaf re-pack lost triangles");
     out->tris = (float4*)b.release(b.final_tris());
     out->leaf_order = (uint32_t*)b.release(b.final_order());

@@ -5,6 +5,11 @@
 //                the node, so one node is 6*W floats of boxes + W child words + W aux
 //                words = 32*W bytes: 64 B for W=2, 128 B (one L2 line) for W=4, 256 B
 //                for W=8).  Boxes are inflated by `pad` so culls stay conservative.
+//   quantized    QNode<W>: the same tree with 8-bit child boxes relative to the node's
+//   nodes        own box (origin + per-axis power-of-two scale), 16 + 4*W + 6*W bytes
+//                rounded up to 16: 48 B for W=2, 64 B for W=4, 96 B for W=8.  Each
+//                decoded box contains the FP32 box it was made from (checked in FP32
+//                with the decoder's own arithmetic), so culls stay conservative.
 //   triangles    3 float4 per triangle in leaf order: (a.xyz, id), (b.xyz, 0),
 //                (c.xyz, 0) with id = the triangle's index in its mesh (float bits).
 //   child word   internal child: wide-node index (< 2^31);
@@ -41,6 +46,7 @@ __host__ __device__ inline uint32_t leaf_count(uint32_t c) { return ((c >> 28) &
 
 template <int W>
 struct alignas(32 * W) WNode {
+    static constexpr int kWidth = W;
     float lox[W], loy[W], loz[W];
     float hix[W], hiy[W], hiz[W];
     uint32_t child[W];
@@ -51,6 +57,31 @@ static_assert(sizeof(WNode<4>) == 128, "node size");
 static_assert(sizeof(WNode<8>) == 256, "node size");
 
 inline size_t node_bytes(int W) { return (size_t)32 * (size_t)W; }
+
+// Quantized wide node.  Child j's box on axis a decodes as
+//   lo = fmaf((float)qlo[a][j], s_a, origin_a),  hi = fmaf((float)qhi[a][j], s_a, origin_a)
+// with s_a = 2^(e_a - 127) (e_a = byte a of `exps`, 1..254).
+template <int W>
+struct alignas(16) QNode {
+    static constexpr int kWidth = W;
+    float ox, oy, oz;
+    uint32_t exps;
+    uint32_t child[W];
+    uint8_t q[6][W];  // qlo x, y, z, then qhi x, y, z
+};
+static_assert(sizeof(QNode<2>) == 48, "node size");
+static_assert(sizeof(QNode<4>) == 64, "node size");
+static_assert(sizeof(QNode<8>) == 96, "node size");
+
+inline size_t qnode_bytes(int W) { return W == 2 ? 48 : (W == 4 ? 64 : 96); }
+
+// Node formats of gpurt_build_opts::node_format (0 selects the library default, mixed).
+constexpr int kFormatF32 = 1;    // FP32 boxes for every query
+constexpr int kFormatQ8 = 2;     // 8-bit boxes for every query
+constexpr int kFormatMixed = 3;  // 8-bit boxes for rays, FP32 boxes for closest points
+inline bool has_qnodes(int f) { return f != kFormatF32; }
+inline bool rays_use_q8(int f) { return f == kFormatQ8 || f == kFormatMixed; }
+inline bool cpq_use_q8(int f) { return f == kFormatQ8; }
 
 // Atomics on global memory through a global-address-space pointer: no flat-address
 // (LDS / scratch aperture) checks are generated around them.
@@ -105,7 +136,9 @@ struct gpurt_bvh_s {
     float scale = 0.f;
     float pad = 0.f;
     float sah_cost = 0.f;
-    void* nodes = nullptr;        // WNode<W>[node_capacity]
+    void* nodes = nullptr;        // WNode<W>[node_capacity] (boxes in FP32; refit updates these)
+    void* qnodes = nullptr;       // QNode<W>[n_nodes] when has_qnodes(format)
+    int format = 1;
     float4* tris = nullptr;       // 3 * n_tris
     uint32_t* tri_idx = nullptr;  // 3 * n_tris, original order (for backward / refit)
     float* verts = nullptr;       // 3 * n_verts compact positions, original order
@@ -124,6 +157,12 @@ int hip_error(hipError_t e, const char* what);
         hipError_t e_ = (call);                                    \
         if (e_ != hipSuccess) return ::gpurt_impl::hip_error(e_, #call); \
     } while (0)
+
+// validated build options (abi.hip); zero fields of the C struct select defaults
+struct BuildParams {
+    int width = 4, leaf_size = 4, treelet_size = 0, treelet_rounds = 0, format = 0;
+};
+int parse_build_opts(const gpurt_build_opts* o, BuildParams& p);
 
 // builder (build.hip): device buffers of one build over one or more meshes
 struct BuildBufs {
@@ -175,7 +214,12 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
 int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level_end, hipStream_t s);
 int run_treelets(BuildBufs& b, int treelet_size, int rounds, int key_bits, int nbits, hipStream_t s);
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width, int leaf_size,
-                   int treelet_size, int treelet_rounds, hipStream_t stream, gpurt_bvh_s* out);
+                   int treelet_size, int treelet_rounds, int format, hipStream_t stream, gpurt_bvh_s* out);
+
+// quantized node copies (quantize.hip): nodes [0, n) with n = *n_dev when n_dev is given
+// (then `capacity` threads are launched), else n_host
+void launch_quantize(int width, const void* nodes, void* qnodes, uint32_t n_host, uint32_t capacity,
+                     const uint32_t* n_dev, hipStream_t s);
 
 // traversal launchers (query.hip)
 int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,

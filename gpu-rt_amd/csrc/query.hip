@@ -33,15 +33,16 @@ struct TravArgs {
     uint32_t* err;
 };
 
-template <int W, int S, int BS, bool STATS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_cpq(
+template <int W, bool Q, int S, int BS, bool STATS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W, Q>::value))) void k_cpq(
     TravArgs a) {
-    const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
+    const NodeOf<W, Q>* __restrict__ nodes = reinterpret_cast<const NodeOf<W, Q>*>(a.nodes);
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
+    StrideSource src{nullptr, a.q, a.n, gid, total};
     auto out = [&](uint32_t qi, uint32_t id, float d2, V3 cp, V3 br) {
         a.prim[qi] = (int32_t)id;
         a.pt[3 * (size_t)qi] = cp.x;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
             a.bary[3 * (size_t)qi + 2] = br.z;
         }
     };
-    if (!cpq_stream<W, S, BS, STATS>(nodes, a.tris, a.root, a.q, a.n, gid, total, st, out, n_nodes, n_tris))
+    if (!cpq_stream<NodeOf<W, Q>, S, BS, STATS>(nodes, a.tris, src, st, out, n_nodes, n_tris))
         gatomic_or(a.err, 1u);
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
@@ -63,15 +64,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
     }
 }
 
-template <int W, int S, int BS, bool ANY, bool STATS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>::value))) void k_ray(
+template <int W, bool Q, int S, int BS, bool ANY, bool STATS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W, Q>::value))) void k_ray(
     TravArgs a) {
-    const WNode<W>* __restrict__ nodes = reinterpret_cast<const WNode<W>*>(a.nodes);
+    const NodeOf<W, Q>* __restrict__ nodes = reinterpret_cast<const NodeOf<W, Q>*>(a.nodes);
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
+    StrideSource src{a.rays, nullptr, a.n, gid, total};
     auto out = [&](uint32_t ri, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
             a.occ[ri] = any ? 1 : 0;
@@ -85,8 +87,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
             }
         }
     };
-    if (!ray_stream<W, S, BS, ANY, STATS>(nodes, a.tris, a.root, a.rays, a.n, gid, total, st, out, n_nodes,
-                                          n_tris))
+    if (!ray_stream<NodeOf<W, Q>, S, BS, ANY, STATS>(nodes, a.tris, src, st, out, n_nodes, n_tris))
         gatomic_or(a.err, 1u);
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
@@ -101,29 +102,29 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W>
 
 enum class QKind { Cpq, Closest, Any };
 
-template <int W, int S, int BS, bool STATS>
+template <int W, bool Q, int S, int BS, bool STATS>
 static void launch_kind(QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (kind) {
         case QKind::Cpq:
-            hipLaunchKernelGGL((k_cpq<W, S, BS, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_cpq<W, Q, S, BS, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
         case QKind::Closest:
-            hipLaunchKernelGGL((k_ray<W, S, BS, false, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_ray<W, Q, S, BS, false, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
         case QKind::Any:
-            hipLaunchKernelGGL((k_ray<W, S, BS, true, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_ray<W, Q, S, BS, true, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
             break;
     }
 }
 
 // LDS per block is S * BS * 8 bytes: 16 KB for S = 8, 32 KB for the others.
-template <int W, bool STATS>
+template <int W, bool Q, bool STATS>
 static int dispatch_stack(int S, QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (S) {
-        case 8: launch_kind<W, 8, 256, STATS>(kind, a, grid, s); return 256;
-        case 16: launch_kind<W, 16, 256, STATS>(kind, a, grid, s); return 256;
-        case 32: launch_kind<W, 32, 128, STATS>(kind, a, grid, s); return 128;
-        case 64: launch_kind<W, 64, 64, STATS>(kind, a, grid, s); return 64;
+        case 8: launch_kind<W, Q, 8, 256, STATS>(kind, a, grid, s); return 256;
+        case 16: launch_kind<W, Q, 16, 256, STATS>(kind, a, grid, s); return 256;
+        case 32: launch_kind<W, Q, 32, 128, STATS>(kind, a, grid, s); return 128;
+        case 64: launch_kind<W, Q, 64, 64, STATS>(kind, a, grid, s); return 64;
     }
     return 0;
 }
@@ -144,7 +145,10 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
     // resident blocks per CU: limited by LDS (the stacks), by the 32 wave slots of a CU
     // and by registers (WavesHint waves per SIMD), unless the caller asks for waves/CU
     const int waves_per_block = BS / 64;
-    const int reg_waves = 4 * (b->width == 8 ? WavesHint<8>::value : WavesHint<4>::value);
+    const bool q8 = kind == QKind::Cpq ? cpq_use_q8(b->format) : rays_use_q8(b->format);
+    const int hint = b->width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
+                                   : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
+    const int reg_waves = 4 * hint;
     int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8), std::min(32, reg_waves) / waves_per_block);
     if (opts && opts->waves_per_cu > 0) blocks_per_cu = std::max(1, opts->waves_per_cu / waves_per_block);
     blocks_per_cu = std::max(blocks_per_cu, 1);
@@ -163,7 +167,6 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
             return set_error(GPURT_E_OOM, "traversal spill buffer allocation failed");
         }
     }
-    a.nodes = b->nodes;
     a.tris = b->tris;
     a.root = 0;
     a.n = (uint32_t)n;
@@ -173,13 +176,26 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
     a.err = ctx->err;
     const bool stats = a.stats != nullptr;
     int bs = 0;
-    switch (b->width) {
-        case 2: bs = stats ? dispatch_stack<2, true>(S, kind, a, grid, s) : dispatch_stack<2, false>(S, kind, a, grid, s); break;
-        case 4: bs = stats ? dispatch_stack<4, true>(S, kind, a, grid, s) : dispatch_stack<4, false>(S, kind, a, grid, s); break;
-        case 8: bs = stats ? dispatch_stack<8, true>(S, kind, a, grid, s) : dispatch_stack<8, false>(S, kind, a, grid, s); break;
-        default: return set_error(GPURT_E_UNSUPPORTED, "node width");
+    a.nodes = q8 ? b->qnodes : b->nodes;
+    switch (b->width * 4 + (q8 ? 2 : 0) + (stats ? 1 : 0)) {
+        case 8: bs = dispatch_stack<2, false, false>(S, kind, a, grid, s); break;
+        case 9: bs = dispatch_stack<2, false, true>(S, kind, a, grid, s); break;
+        case 10: bs = dispatch_stack<2, true, false>(S, kind, a, grid, s); break;
+        case 11: bs = dispatch_stack<2, true, true>(S, kind, a, grid, s); break;
+        case 16: bs = dispatch_stack<4, false, false>(S, kind, a, grid, s); break;
+        case 17: bs = dispatch_stack<4, false, true>(S, kind, a, grid, s); break;
+        case 18: bs = dispatch_stack<4, true, false>(S, kind, a, grid, s); break;
+        case 19: bs = dispatch_stack<4, true, true>(S, kind, a, grid, s); break;
+        case 32: bs = dispatch_stack<8, false, false>(S, kind, a, grid, s); break;
+        case 33: bs = dispatch_stack<8, false, true>(S, kind, a, grid, s); break;
+        case 34: bs = dispatch_stack<8, true, false>(S, kind, a, grid, s); break;
+        case 35: bs = dispatch_stack<8, true, true>(S, kind, a, grid, s); break;
+        default: break;
     }
-    (void)bs;
+    if (bs == 0) {
+        if (spill) (void)hipFreeAsync(spill, s);
+        return set_error(GPURT_E_UNSUPPORTED, "node width");
+    }
     hipError_t le = hipGetLastError();
     if (spill) (void)hipFreeAsync(spill, s);
     if (le != hipSuccess) return hip_error(le, "traversal launch");

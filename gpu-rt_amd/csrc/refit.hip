@@ -131,6 +131,8 @@ int launch_refit(gpurt_bvh_s* b, const float* verts, size_t vstride_floats, hipS
         case 4: refit_levels<4>(b, scale, s); break;
         case 8: refit_levels<8>(b, scale, s); break;
     }
+    if (has_qnodes(b->format)) launch_quantize(b->width, b->nodes, b->qnodes, (uint32_t)b->n_nodes,
+                                                (uint32_t)b->n_nodes, nullptr, s);
     hipError_t e = hipGetLastError();
     (void)hipFreeAsync(tmp, s);
     if (e != hipSuccess) return hip_error(e, "refit launch");

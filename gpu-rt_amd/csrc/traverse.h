@@ -1,13 +1,13 @@
-// traverse.h -- per-query BVH traversal shared by the plain kernels (query.hip) and the
-// persistent batch kernel (batch.hip).
+// traverse.h -- BVH traversal shared by the plain kernels (query.hip) and the persistent
+// batch kernels (batch.hip).
 //
 // One lane per query.  A lane keeps a stack of (node-or-leaf reference, lower bound)
 // pairs: the first S entries live in LDS laid out [slot][lane] (consecutive lanes hit
 // consecutive banks at every depth), deeper entries spill to a device buffer that the
 // launcher sizes from the tree depth, so no traversal can overflow.  A wide node is
-// fetched as 2*W float4 loads (W=4: one 128-byte line); children are bounded, sorted
-// by the bound with a sorting network, and pushed far-to-near so the nearest is taken
-// next without a stack round trip.  Culls compare against the best value with a small
+// fetched with unconditional dwordx4 loads (FP32 boxes: 7 for W=4; 8-bit boxes: 4);
+// children are bounded, sorted by the bound with a sorting network, and pushed
+// far-to-near so the nearest is taken next without a stack round trip.  Culls compare against the best value with a small
 // relative slack and boxes are inflated at build time, so a child that can hold an
 // equal-or-better primitive is never culled: results do not depend on the tree's
 // shape, and ties resolve to the lowest primitive id.
@@ -15,6 +15,8 @@
 
 #include "gpurt_internal.h"
 #include "tri_math.h"
+
+#include <type_traits>
 
 namespace gpurt_impl {
 
@@ -94,13 +96,88 @@ __device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&c)[W]) {
     }
 }
 
+// Child boxes and child words of one wide node, in registers.
+template <int W>
+struct ChildBoxes {
+    float lox[W], loy[W], loz[W];
+    float hix[W], hiy[W], hiz[W];
+    uint32_t child[W];
+};
+
+// FP32 node: the box and child words (7 * W dwords; all 16 dwords of a 2-wide node) as
+// unconditional dwordx4 loads issued together, before any use.
+template <int W>
+__device__ __forceinline__ void fetch_node(const WNode<W>* __restrict__ nodes, uint32_t i, ChildBoxes<W>& b) {
+    constexpr int NV = W == 2 ? 4 : 7 * W / 4;  // float4 loads
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float f[4 * NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const float4 v = p[k];
+        f[4 * k] = v.x;
+        f[4 * k + 1] = v.y;
+        f[4 * k + 2] = v.z;
+        f[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        b.lox[j] = f[j];
+        b.loy[j] = f[W + j];
+        b.loz[j] = f[2 * W + j];
+        b.hix[j] = f[3 * W + j];
+        b.hiy[j] = f[4 * W + j];
+        b.hiz[j] = f[5 * W + j];
+        b.child[j] = __float_as_uint(f[6 * W + j]);
+    }
+}
+
+// Quantized node: sizeof(QNode<W>) / 16 dwordx4 loads, then every box plane decodes as
+// fmaf(q, 2^(e-127), origin) -- the expression the builder checked it with.
+template <int W>
+__device__ __forceinline__ void fetch_node(const QNode<W>* __restrict__ nodes, uint32_t i, ChildBoxes<W>& b) {
+    constexpr int NW = (int)sizeof(QNode<W>) / 4;
+    constexpr int Q0 = 4 + W;  // first word of the 8-bit planes
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    uint32_t w[NW];
+#pragma unroll
+    for (int k = 0; k < NW / 4; k++) {
+        const uint4 v = p[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    const float ox = __uint_as_float(w[0]), oy = __uint_as_float(w[1]), oz = __uint_as_float(w[2]);
+    const float sx = __uint_as_float((w[3] & 0xFFu) << 23);
+    const float sy = __uint_as_float(((w[3] >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((w[3] >> 16) & 0xFFu) << 23);
+    auto qb = [&](int row, int j) -> float {
+        const int byte = row * W + j;
+        return (float)((w[Q0 + byte / 4] >> (8 * (byte % 4))) & 0xFFu);
+    };
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        b.child[j] = w[4 + j];
+        b.lox[j] = fmaf(qb(0, j), sx, ox);
+        b.loy[j] = fmaf(qb(1, j), sy, oy);
+        b.loz[j] = fmaf(qb(2, j), sz, oz);
+        b.hix[j] = fmaf(qb(3, j), sx, ox);
+        b.hiy[j] = fmaf(qb(4, j), sy, oy);
+        b.hiz[j] = fmaf(qb(5, j), sz, oz);
+    }
+}
+
+template <int W, bool Q>
+using NodeOf = typename std::conditional<Q, QNode<W>, WNode<W>>::type;
+
 // ------------------------------------------------------------------------------------
 // closest point
 // ------------------------------------------------------------------------------------
 
 // Resident waves per SIMD the register allocator targets: 8 (<= 64 VGPRs) for 2- and
-// 4-wide nodes; 8-wide nodes need more live registers for the child boxes.
-template <int W>
+// 4-wide nodes; 8-wide nodes need more live registers for the child boxes and sort
+// keys (forcing 64 VGPRs makes them spill), so they get what they need (5-7 waves).
+template <int W, bool Q = false>
 struct WavesHint {
     static constexpr int value = W == 8 ? 4 : 8;
 };
@@ -108,166 +185,183 @@ struct WavesHint {
 constexpr float kCullSlack = 1.0f + 0x1p-16f;  // relative slack on squared distances
 constexpr float kTSlack = 0x1p-20f;             // relative slack on ray box exits
 
-struct CpqHit {
-    uint32_t id;  // 0xFFFFFFFF: none
-    float d2;
-    V3 cp, bary;
-};
+// ------------------------------------------------------------------------------------
+// Streaming traversal with postponed leaves
+//
+// A lane runs one step per loop iteration, and in each iteration a wave runs either node
+// steps or leaf steps, never both (the two paths are long and would otherwise both be
+// issued whenever lanes disagree).  A lane that reaches a leaf parks it -- one parked
+// leaf per lane -- and keeps visiting nodes from its stack meanwhile; a lane that
+// reaches a second leaf waits.  The wave runs leaf steps when at least half of the
+// lanes with a query have parked a leaf, or when no lane has a node to visit
+// (postponed leaves with speculative traversal: Aila & Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009).  Extra visits cannot change the
+// answer: culls only compare against the best value found so far.
+//
+// Queries come from a source (Src) that every lane of the wave calls together at the
+// top of each iteration: a lane whose query is done writes its result and takes the
+// next one in the same iteration, so lanes do not wait for the wave's slowest query.
+// Every query ends (each node is visited at most once), and each iteration visits a
+// node, tests a parked leaf, pops an entry or ends a query.
+//
+// A source provides
+//   typename Handle;                         which query (index, or mesh and index)
+//   bool refill(bool has, Handle& h);        called by all lanes together; gives lanes
+//                                            with has == false their next query
+//   uint32_t root(const Handle&);            root node of the query's tree
+//   ray(const Handle&, float4&, float4&)     (rays) origin, tmin / direction, tmax
+//   V3 point(const Handle&)                  (closest point) the query point
+// ------------------------------------------------------------------------------------
 
-// Closest point of q on the subtree at `root`.  Returns false on stack overflow.
-template <int W, int S, int BS, bool STATS>
-__device__ __forceinline__ bool cpq_one(const WNode<W>* __restrict__ nodes,
-                                        const float4* __restrict__ tris, uint32_t root, V3 q,
-                                        LaneStack<S, BS>& st, CpqHit& h,
-                                        unsigned long long& n_nodes, unsigned long long& n_tris) {
-    float best = INFINITY, thr = INFINITY;
-    uint32_t bid = 0xFFFFFFFFu;
-    V3 bcp = v3(0.f, 0.f, 0.f), bb = v3(0.f, 0.f, 0.f);
-    st.sp = 0;
-    uint32_t cur = root;
-    bool have = true;
-    bool ok = true;
-    for (;;) {
-        if (!have) {
-            if (st.sp == 0) break;
-            uint2 e = st.pop();
-            if (__uint_as_float(e.y) > thr) continue;
-            cur = e.x;
-        }
-        have = false;
-        if (child_is_leaf(cur)) {
-            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; k++) {
-                const float4 A = tris[3 * (size_t)(f + k)];
-                const float4 B = tris[3 * (size_t)(f + k) + 1];
-                const float4 C = tris[3 * (size_t)(f + k) + 2];
-                V3 cp, br;
-                float d2 = closest_pt_tri(q, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z),
-                                          cp, br);
-                uint32_t id = __float_as_uint(A.w);
-                if (STATS) n_tris++;
-                if (d2 < best || (d2 == best && id < bid)) {
-                    best = d2;
-                    bid = id;
-                    bcp = cp;
-                    bb = br;
-                    thr = best * kCullSlack;
-                }
-            }
-            continue;
-        }
-        if (STATS) n_nodes++;
-        const WNode<W> nd = nodes[cur];
-        float d[W];
-        uint32_t c[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            float dx = fmaxf(fmaxf(nd.lox[j] - q.x, q.x - nd.hix[j]), 0.0f);
-            float dy = fmaxf(fmaxf(nd.loy[j] - q.y, q.y - nd.hiy[j]), 0.0f);
-            float dz = fmaxf(fmaxf(nd.loz[j] - q.z, q.z - nd.hiz[j]), 0.0f);
-            float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            c[j] = nd.child[j];
-            d[j] = (c[j] == kEmpty) ? INFINITY : dd;
-        }
-        sort_children<W>(d, c);
-#pragma unroll
-        for (int j = W - 1; j >= 1; j--) {
-            if (c[j] != kEmpty && d[j] <= thr) {
-                if (!st.push(c[j], d[j])) ok = false;
-            }
-        }
-        if (c[0] != kEmpty && d[0] <= thr) {
-            cur = c[0];
-            have = true;
-        }
-    }
-    h.id = bid;
-    h.d2 = best;
-    h.cp = bcp;
-    h.bary = bb;
-    return ok;
+constexpr uint32_t kLeafTrigger = 32;  // parked-leaf share (of 64 lanes) for a leaf step
+
+// Wave-uniform choice of the step kind: leaf steps when some lane has a parked leaf and
+// either no lane has a node to visit or at least kLeafTrigger/64 of the lanes with a
+// query have parked a leaf.
+__device__ __forceinline__ bool leaf_step_now(bool parked, bool node_ready) {
+    const uint64_t pk = __ballot(parked);
+    if (pk == 0) return false;
+    if (__ballot(node_ready) == 0) return true;
+    return (uint32_t)__popcll(pk) * 64u >= (uint32_t)__popcll(__ballot(true)) * kLeafTrigger;
 }
 
-struct RayHit {
-    uint32_t id;  // 0xFFFFFFFF: none
-    float t, u, v;
-    bool any;
+// Plain source: lane g takes queries g, g + stride, g + 2 stride, ... of one array.
+struct StrideSource {
+    using Handle = uint32_t;
+    const float4* rays;
+    const float* pts;
+    uint32_t n, next, stride;
+    __device__ __forceinline__ bool refill(bool has, Handle& h) {
+        if (has || next >= n) return false;
+        h = next;
+        next += stride;
+        return true;
+    }
+    __device__ __forceinline__ uint32_t root(Handle) const { return 0u; }
+    __device__ __forceinline__ void ray(Handle h, float4& r0, float4& r1) const {
+        r0 = rays[2 * (size_t)h];
+        r1 = rays[2 * (size_t)h + 1];
+    }
+    __device__ __forceinline__ V3 point(Handle h) const {
+        return v3(pts[3 * (size_t)h], pts[3 * (size_t)h + 1], pts[3 * (size_t)h + 2]);
+    }
 };
 
-// Closest (ANY=false) or any (ANY=true) hit of the ray on the subtree at `root`.
-// Returns false on stack overflow.
-template <int W, int S, int BS, bool ANY, bool STATS>
-__device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
-                                        const float4* __restrict__ tris, uint32_t root, V3 o, V3 d,
-                                        float tmin, float tmax, LaneStack<S, BS>& st, RayHit& h,
-                                        unsigned long long& n_nodes, unsigned long long& n_tris) {
-    float best = tmax;  // shrinks to the best hit
-    uint32_t bid = 0xFFFFFFFFu;
-    float bu = 0.f, bv = 0.f;
-    bool hit_any = false;
-    V3 inv;
-    {
-        float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
-        float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
-        float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
-        inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
-    }
-    const float oix = -o.x * inv.x, oiy = -o.y * inv.y, oiz = -o.z * inv.z;
-    st.sp = 0;
-    uint32_t cur = root;
-    bool have = true;
+template <typename NodeT, int S, int BS, bool ANY, bool STATS, typename Src, typename Out>
+__device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
+                                           Src& src, LaneStack<S, BS>& st, Out out,
+                                           unsigned long long& n_nodes, unsigned long long& n_tris) {
+    constexpr int W = NodeT::kWidth;
     bool ok = true;
+    typename Src::Handle h{};
+    bool has = false;
+    V3 o = v3(0.f, 0.f, 0.f), d = o, inv = o;
+    float oix = 0.f, oiy = 0.f, oiz = 0.f;
+    float tmin = 0.f, best = 0.f;
+    uint32_t bid = 0xFFFFFFFFu, bpos = 0, cur = 0, pleaf = kEmpty;
+    bool hit_any = false, have = false;
     for (;;) {
-        if (!have) {
-            if (st.sp == 0) break;
-            uint2 e = st.pop();
-            if (__uint_as_float(e.y) > best) continue;
-            cur = e.x;
+        if (has && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any))) {
+            // barycentrics of the winner, recomputed once with the same arithmetic
+            float bu = 0.f, bv = 0.f;
+            if (!ANY && bid != 0xFFFFFFFFu) {
+                const float4 A = tris[3 * (size_t)bpos];
+                const float4 B = tris[3 * (size_t)bpos + 1];
+                const float4 C = tris[3 * (size_t)bpos + 2];
+                float tt;
+                (void)ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+            }
+            out(h, bid, best, bu, bv, hit_any);
+            has = false;
         }
-        have = false;
-        if (child_is_leaf(cur)) {
-            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+        if (src.refill(has, h)) {
+            float4 r0, r1;
+            src.ray(h, r0, r1);
+            o = v3(r0.x, r0.y, r0.z);
+            d = v3(r1.x, r1.y, r1.z);
+            tmin = r0.w;
+            best = r1.w;
+            const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+            const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+            const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+            inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+            oix = -o.x * inv.x;
+            oiy = -o.y * inv.y;
+            oiz = -o.z * inv.z;
+            bid = 0xFFFFFFFFu;
+            hit_any = false;
+            st.sp = 0;
+            cur = src.root(h);
+            pleaf = kEmpty;
+            have = true;
+            has = true;
+        }
+        if (!__any(has)) break;
+        if (!has) continue;
+        if (!have && st.sp > 0) {
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) <= best) {
+                cur = e.x;
+                have = true;
+            }
+        }
+        if (have && pleaf == kEmpty && child_is_leaf(cur)) {
+            pleaf = cur;
+            have = false;
+        }
+        const bool node_ready = have && !child_is_leaf(cur);
+        if (leaf_step_now(pleaf != kEmpty, node_ready)) {
+            if (pleaf == kEmpty) continue;
+            const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {
                 const float4 A = tris[3 * (size_t)(f + k)];
                 const float4 B = tris[3 * (size_t)(f + k) + 1];
                 const float4 C = tris[3 * (size_t)(f + k) + 2];
                 if (STATS) n_tris++;
                 float t, u, v;
-                if (!ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), t, u, v))
-                    continue;
+                if (!ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), t, u, v)) continue;
                 if (!(t >= tmin && t <= best)) continue;
                 if (ANY) {
                     hit_any = true;
                     break;
                 }
-                uint32_t id = __float_as_uint(A.w);
+                const uint32_t id = __float_as_uint(A.w);
                 if (t < best || id < bid) {
                     best = t;
                     bid = id;
-                    bu = u;
-                    bv = v;
+                    bpos = f + k;
                 }
             }
-            if (ANY && hit_any) break;
+            pleaf = kEmpty;
+            if (ANY && hit_any) {
+                have = false;  // done: the result is written in the next iteration
+            } else if (have && child_is_leaf(cur)) {
+                pleaf = cur;  // the leaf this lane was waiting with
+                have = false;
+            }
             continue;
         }
+        if (!node_ready) continue;
         if (STATS) n_nodes++;
-        const WNode<W> nd = nodes[cur];
+        ChildBoxes<W> nd;
+        fetch_node(nodes, cur, nd);
         float tn[W];
         uint32_t c[W];
 #pragma unroll
         for (int j = 0; j < W; j++) {
+            // t = (b - o) / d as one FMA with o/d precomputed; its rounding is far below
+            // the build-time box inflation, so the test stays conservative
             const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
             const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
             const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
-            float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-            float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-            c[j] = nd.child[j];
-            bool hb = (c[j] != kEmpty) && (lo <= hi + fabsf(hi) * kTSlack);
+            const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+            const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+            const bool hb = (nd.child[j] != kEmpty) & (lo <= hi + fabsf(hi) * kTSlack);
             tn[j] = hb ? lo : INFINITY;
-            if (!hb) c[j] = kEmpty;
+            c[j] = hb ? nd.child[j] : kEmpty;
         }
         if (!ANY) sort_children<W>(tn, c);
+        have = false;
         // push all but the nearest hit child (far to near), then continue with it; the
         // pending entry is carried in scalars so every index stays static
         uint32_t pend = kEmpty;
@@ -286,182 +380,62 @@ __device__ __forceinline__ bool ray_one(const WNode<W>* __restrict__ nodes,
             have = true;
         }
     }
-    h.id = bid;
-    h.t = best;
-    h.u = bu;
-    h.v = bv;
-    h.any = hit_any;
     return ok;
 }
 
-}  // namespace gpurt_impl
-
-namespace gpurt_impl {
-
-// ------------------------------------------------------------------------------------
-// Streaming traversal: lane-level refill
-//
-// A lane runs one traversal step per loop iteration; when its query finishes it writes
-// the result and loads its next query (index += stride) in the same iteration, so lanes
-// of a wave do not wait for the wave's slowest query before taking new work (the
-// per-query loop above keeps a finished lane idle until all 64 finish).  Every query
-// ends (each node is visited at most once), a lane leaves when its indices run out,
-// and the loop ends when no lane of the wave is active.
-// ------------------------------------------------------------------------------------
-
-template <int W, int S, int BS, bool ANY, bool STATS, typename Out>
-__device__ __forceinline__ bool ray_stream(const WNode<W>* __restrict__ nodes,
-                                           const float4* __restrict__ tris, uint32_t root,
-                                           const float4* __restrict__ rays, uint32_t n, uint32_t first,
-                                           uint32_t stride, LaneStack<S, BS>& st, Out out,
+template <typename NodeT, int S, int BS, bool STATS, typename Src, typename Out>
+__device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
+                                           Src& src, LaneStack<S, BS>& st, Out out,
                                            unsigned long long& n_nodes, unsigned long long& n_tris) {
+    constexpr int W = NodeT::kWidth;
     bool ok = true;
-    uint32_t ri = first;
-    bool active = ri < n;
-    V3 o = v3(0.f, 0.f, 0.f), d = o, inv = o;
-    float oix = 0.f, oiy = 0.f, oiz = 0.f;
-    float tmin = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
-    uint32_t bid = 0xFFFFFFFFu, cur = root;
-    bool hit_any = false, have = false;
-    auto start = [&](uint32_t i) {
-        const float4 r0 = rays[2 * (size_t)i];
-        const float4 r1 = rays[2 * (size_t)i + 1];
-        o = v3(r0.x, r0.y, r0.z);
-        d = v3(r1.x, r1.y, r1.z);
-        tmin = r0.w;
-        best = r1.w;
-        const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
-        const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
-        const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
-        inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
-        oix = -o.x * inv.x;
-        oiy = -o.y * inv.y;
-        oiz = -o.z * inv.z;
-        bid = 0xFFFFFFFFu;
-        bu = bv = 0.f;
-        hit_any = false;
-        st.sp = 0;
-        cur = root;
-        have = true;
-    };
-    if (active) start(ri);
-    while (__any(active)) {
-        if (!active) continue;
-        if (!have) {
-            if (st.sp == 0 || (ANY && hit_any)) {
-                out(ri, bid, best, bu, bv, hit_any);
-                ri += stride;
-                if (ri < n) start(ri);
-                else active = false;
-                continue;
-            }
-            const uint2 e = st.pop();
-            if (__uint_as_float(e.y) > best) continue;
-            cur = e.x;
-        }
-        have = false;
-        if (child_is_leaf(cur)) {
-            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; k++) {
-                const float4 A = tris[3 * (size_t)(f + k)];
-                const float4 B = tris[3 * (size_t)(f + k) + 1];
-                const float4 C = tris[3 * (size_t)(f + k) + 2];
-                if (STATS) n_tris++;
-                float t, u, v;
-                if (!ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), t, u, v)) continue;
-                if (!(t >= tmin && t <= best)) continue;
-                if (ANY) {
-                    hit_any = true;
-                    break;
-                }
-                const uint32_t id = __float_as_uint(A.w);
-                if (t < best || id < bid) {
-                    best = t;
-                    bid = id;
-                    bu = u;
-                    bv = v;
-                }
-            }
-            continue;
-        }
-        if (STATS) n_nodes++;
-        const WNode<W> nd = nodes[cur];
-        float tn[W];
-        uint32_t c[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            // t = (b - o) / d as one FMA with o/d precomputed; its rounding is far below
-            // the build-time box inflation, so the test stays conservative
-            const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
-            const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
-            const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
-            const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-            const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-            c[j] = nd.child[j];
-            const bool hb = (c[j] != kEmpty) && (lo <= hi + fabsf(hi) * kTSlack);
-            tn[j] = hb ? lo : INFINITY;
-            if (!hb) c[j] = kEmpty;
-        }
-        if (!ANY) sort_children<W>(tn, c);
-        uint32_t pend = kEmpty;
-        float pend_t = 0.0f;
-#pragma unroll
-        for (int j = W - 1; j >= 0; j--) {
-            if (c[j] == kEmpty) continue;
-            if (pend != kEmpty) {
-                if (!st.push(pend, pend_t)) ok = false;
-            }
-            pend = c[j];
-            pend_t = tn[j];
-        }
-        if (pend != kEmpty) {
-            cur = pend;
-            have = true;
-        }
-    }
-    return ok;
-}
-
-template <int W, int S, int BS, bool STATS, typename Out>
-__device__ __forceinline__ bool cpq_stream(const WNode<W>* __restrict__ nodes,
-                                           const float4* __restrict__ tris, uint32_t root,
-                                           const float* __restrict__ qs, uint32_t n, uint32_t first,
-                                           uint32_t stride, LaneStack<S, BS>& st, Out out,
-                                           unsigned long long& n_nodes, unsigned long long& n_tris) {
-    bool ok = true;
-    uint32_t qi = first;
-    bool active = qi < n;
-    V3 q = v3(0.f, 0.f, 0.f), bcp = q, bb = q;
+    typename Src::Handle h{};
+    bool has = false;
+    V3 q = v3(0.f, 0.f, 0.f);
     float best = INFINITY, thr = INFINITY;
-    uint32_t bid = 0xFFFFFFFFu, cur = root;
+    uint32_t bid = 0xFFFFFFFFu, bpos = 0, cur = 0, pleaf = kEmpty;
     bool have = false;
-    auto start = [&](uint32_t i) {
-        q = v3(qs[3 * (size_t)i], qs[3 * (size_t)i + 1], qs[3 * (size_t)i + 2]);
-        best = thr = INFINITY;
-        bid = 0xFFFFFFFFu;
-        bcp = bb = v3(0.f, 0.f, 0.f);
-        st.sp = 0;
-        cur = root;
-        have = true;
-    };
-    if (active) start(qi);
-    while (__any(active)) {
-        if (!active) continue;
-        if (!have) {
-            if (st.sp == 0) {
-                out(qi, bid, best, bcp, bb);
-                qi += stride;
-                if (qi < n) start(qi);
-                else active = false;
-                continue;
+    for (;;) {
+        if (has && !have && pleaf == kEmpty && st.sp == 0) {
+            // the closest point and barycentrics of the winner, recomputed once with the
+            // same arithmetic (cheaper than carrying them through the traversal)
+            V3 cp = v3(0.f, 0.f, 0.f), br = cp;
+            if (bid != 0xFFFFFFFFu) {
+                const float4 A = tris[3 * (size_t)bpos];
+                const float4 B = tris[3 * (size_t)bpos + 1];
+                const float4 C = tris[3 * (size_t)bpos + 2];
+                (void)closest_pt_tri(q, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), cp, br);
             }
-            const uint2 e = st.pop();
-            if (__uint_as_float(e.y) > thr) continue;
-            cur = e.x;
+            out(h, bid, best, cp, br);
+            has = false;
         }
-        have = false;
-        if (child_is_leaf(cur)) {
-            const uint32_t f = leaf_first(cur), cnt = leaf_count(cur);
+        if (src.refill(has, h)) {
+            q = src.point(h);
+            best = thr = INFINITY;
+            bid = 0xFFFFFFFFu;
+            st.sp = 0;
+            cur = src.root(h);
+            pleaf = kEmpty;
+            have = true;
+            has = true;
+        }
+        if (!__any(has)) break;
+        if (!has) continue;
+        if (!have && st.sp > 0) {
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) <= thr) {
+                cur = e.x;
+                have = true;
+            }
+        }
+        if (have && pleaf == kEmpty && child_is_leaf(cur)) {
+            pleaf = cur;
+            have = false;
+        }
+        const bool node_ready = have && !child_is_leaf(cur);
+        if (leaf_step_now(pleaf != kEmpty, node_ready)) {
+            if (pleaf == kEmpty) continue;
+            const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {
                 const float4 A = tris[3 * (size_t)(f + k)];
                 const float4 B = tris[3 * (size_t)(f + k) + 1];
@@ -473,15 +447,21 @@ __device__ __forceinline__ bool cpq_stream(const WNode<W>* __restrict__ nodes,
                 if (d2 < best || (d2 == best && id < bid)) {
                     best = d2;
                     bid = id;
-                    bcp = cp;
-                    bb = br;
+                    bpos = f + k;
                     thr = best * kCullSlack;
                 }
             }
+            pleaf = kEmpty;
+            if (have && child_is_leaf(cur)) {
+                pleaf = cur;
+                have = false;
+            }
             continue;
         }
+        if (!node_ready) continue;
         if (STATS) n_nodes++;
-        const WNode<W> nd = nodes[cur];
+        ChildBoxes<W> nd;
+        fetch_node(nodes, cur, nd);
         float dd[W];
         uint32_t c[W];
 #pragma unroll
@@ -493,6 +473,7 @@ __device__ __forceinline__ bool cpq_stream(const WNode<W>* __restrict__ nodes,
             dd[j] = (c[j] == kEmpty) ? INFINITY : fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         }
         sort_children<W>(dd, c);
+        have = false;
 #pragma unroll
         for (int j = W - 1; j >= 1; j--) {
             if (c[j] != kEmpty && dd[j] <= thr) {

@@ -105,7 +105,7 @@ class Bvh:
     """
 
     def __init__(self, ctx, verts, tris, node_width=4, leaf_size=4, treelet_size=0,
-                 treelet_rounds=0):
+                 treelet_rounds=0, node_format=0):
         lib = _lib.load()
         self.ctx = ctx
         dev = ctx.device
@@ -118,7 +118,7 @@ class Bvh:
         self.n_verts = verts.shape[0]
         self.n_tris = tris.shape[0]
         self._tris = tris  # indices are needed by no kernel after the build; kept for users
-        opts = BuildOpts(node_width, leaf_size, treelet_size, treelet_rounds)
+        opts = BuildOpts(node_width, leaf_size, treelet_size, treelet_rounds, node_format)
         h = ctypes.c_void_p()
         _lib.check(lib.gpurt_bvh_build(ctx.handle, _ptr(verts), verts.shape[1] * 4,
                                        verts.shape[0], _ptr(tris), tris.shape[0],
@@ -234,7 +234,7 @@ class Batch:
     """
 
     def __init__(self, ctx, meshes, node_width=4, leaf_size=4, treelet_size=0, stack_depth=0,
-                 waves_per_cu=0, with_bary=True, with_uv=True):
+                 waves_per_cu=0, with_bary=True, with_uv=True, node_format=0):
         lib = _lib.load()
         self.ctx = ctx
         dev = ctx.device
@@ -275,7 +275,7 @@ class Batch:
             bm.r_prim, bm.r_t = out["r_prim"].data_ptr(), out["r_t"].data_ptr()
             bm.r_uv = out["r_uv"].data_ptr() if with_uv else None
         self._arr = arr
-        bo = BuildOpts(node_width, leaf_size, treelet_size, 0)
+        bo = BuildOpts(node_width, leaf_size, treelet_size, 0, node_format)
         qo = QueryOpts(stack_depth, waves_per_cu, None)
         torch.cuda.synchronize(dev)  # inputs must be resident before the first (eager) run
         h = ctypes.c_void_p()

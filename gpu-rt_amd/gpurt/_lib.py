@@ -41,7 +41,8 @@ class GpurtError(RuntimeError):
 
 class BuildOpts(ctypes.Structure):
     _fields_ = [("node_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
-                ("treelet_size", ctypes.c_int32), ("treelet_rounds", ctypes.c_int32)]
+                ("treelet_size", ctypes.c_int32), ("treelet_rounds", ctypes.c_int32),
+                ("node_format", ctypes.c_int32)]
 
 
 class QueryOpts(ctypes.Structure):
@@ -54,7 +55,8 @@ class BvhInfo(ctypes.Structure):
                 ("n_nodes", ctypes.c_int64), ("node_bytes", ctypes.c_int64),
                 ("tri_bytes", ctypes.c_int64), ("depth", ctypes.c_int32),
                 ("node_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
-                ("sah_cost", ctypes.c_float)]
+                ("sah_cost", ctypes.c_float), ("node_format", ctypes.c_int32),
+                ("cpq_node_bytes", ctypes.c_int32)]
 
 
 class BatchMesh(ctypes.Structure):

@@ -1,7 +1,7 @@
 """Autotuner for the traversal and build parameters.
 
-Sweeps node width {2, 4, 8}, LDS stack depth {8, 16, 32, 64}, resident waves per CU and
-treelet size over a sample of the caller's own queries, times every candidate with HIP
+Sweeps node width {2, 4, 8}, node encoding (FP32 or 8-bit child boxes), LDS stack depth
+{8, 16, 32, 64}, resident waves per CU and treelet size over a sample of the caller's own queries, times every candidate with HIP
 events on the current stream, and keeps the fastest.  Answers are independent of every
 parameter (the traversal is exact for any tree), so only speed is compared.  Winners
 are cached in a JSON file keyed by the device name and a coarse signature of the
@@ -27,6 +27,7 @@ DEFAULT_SPACE = {
     "stack_depth": (8, 16, 32, 64),
     "waves_per_cu": (0, 16, 24),
     "treelet_size": (0, 6),
+    "node_format": (1, 2),
 }
 
 
@@ -110,11 +111,11 @@ def tune_queries(ctx, verts, tris, queries=None, rays=None, space=None, sample=2
     q = _sample(queries, sample)
     r = _sample(rays, sample)
     table = []
-    for W, T in itertools.product(space["node_width"], space["treelet_size"]):
+    for W, T, NF in itertools.product(space["node_width"], space["treelet_size"], space["node_format"]):
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
-        bvh = gpurt.Bvh(ctx, verts, tris, node_width=W, treelet_size=T)
+        bvh = gpurt.Bvh(ctx, verts, tris, node_width=W, treelet_size=T, node_format=NF)
         ev1.record()
         torch.cuda.synchronize()
         build_ms = ev0.elapsed_time(ev1)
@@ -125,12 +126,13 @@ def tune_queries(ctx, verts, tris, queries=None, rays=None, space=None, sample=2
                 if r is not None:
                     bvh.trace_closest(r, stack_depth=S, waves_per_cu=WV)
             ms = _time_ms(run)
-            table.append({"node_width": W, "treelet_size": T, "stack_depth": S, "waves_per_cu": WV,
-                          "query_ms": ms, "build_ms": build_ms,
+            table.append({"node_width": W, "treelet_size": T, "node_format": NF, "stack_depth": S,
+                          "waves_per_cu": WV, "query_ms": ms, "build_ms": build_ms,
                           "objective_ms": ms + builds_per_call * build_ms})
         bvh.close()
     best = min(table, key=lambda e: e["objective_ms"])
-    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"]},
+    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"],
+                        "node_format": best["node_format"]},
               "query": {"stack_depth": best["stack_depth"], "waves_per_cu": best["waves_per_cu"]},
               "objective_ms": best["objective_ms"], "key": key, "table": table}
     if use_cache:
@@ -153,15 +155,18 @@ def tune_batch(ctx, meshes, space=None, use_cache=True, path=None, reps=3):
         if hit:
             return dict(hit, cached=True)
     table = []
-    for W, T, S, WV in itertools.product(space["node_width"], space["treelet_size"],
-                                         space["stack_depth"], space["waves_per_cu"]):
-        b = gpurt.Batch(ctx, meshes, node_width=W, treelet_size=T, stack_depth=S, waves_per_cu=WV)
+    for W, T, NF, S, WV in itertools.product(space["node_width"], space["treelet_size"],
+                                             space["node_format"], space["stack_depth"],
+                                             space["waves_per_cu"]):
+        b = gpurt.Batch(ctx, meshes, node_width=W, treelet_size=T, stack_depth=S, waves_per_cu=WV,
+                        node_format=NF)
         ms = _time_ms(b.run, reps)
         b.close()
-        table.append({"node_width": W, "treelet_size": T, "stack_depth": S, "waves_per_cu": WV,
-                      "replay_ms": ms})
+        table.append({"node_width": W, "treelet_size": T, "node_format": NF, "stack_depth": S,
+                      "waves_per_cu": WV, "replay_ms": ms})
     best = min(table, key=lambda e: e["replay_ms"])
-    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"]},
+    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"],
+                        "node_format": best["node_format"]},
               "query": {"stack_depth": best["stack_depth"], "waves_per_cu": best["waves_per_cu"]},
               "objective_ms": best["replay_ms"], "key": key, "table": table}
     if use_cache:

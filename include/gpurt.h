@@ -73,6 +73,12 @@ typedef struct gpurt_build_opts {
     int32_t leaf_size;      /* max triangles per leaf, 1..8 (0 = 4)                   */
     int32_t treelet_size;   /* leaves per treelet for SAH restructuring, 0 = off, 5..7 */
     int32_t treelet_rounds; /* bottom-up restructuring rounds: 0 or 1 (0 = 1 when on) */
+    int32_t node_format;    /* traversed node encoding (answers are identical for all):
+                               1 = FP32 child boxes (32*W bytes per node),
+                               2 = 8-bit child boxes relative to the node's box
+                                   (48/64/96 bytes for W = 2/4/8),
+                               3 = both: 8-bit for rays, FP32 for closest points,
+                               0 = library default (3)                                */
 } gpurt_build_opts;
 
 typedef struct gpurt_query_opts {
@@ -94,6 +100,9 @@ typedef struct gpurt_bvh_info {
     int32_t node_width;
     int32_t leaf_size;
     float sah_cost;     /* SAH cost of the wide tree, relative to the root area        */
+    int32_t node_format; /* resolved gpurt_build_opts::node_format (1, 2 or 3)          */
+    int32_t cpq_node_bytes; /* bytes per node traversed by closest-point queries
+                               (node_bytes: by rays)                                    */
 } gpurt_bvh_info;
 
 /* ---- context --------------------------------------------------------------------- */

@@ -3,6 +3,8 @@ errors are reported as status codes before any device work (no GPU needed)."""
 import ctypes
 import os
 import re
+import subprocess
+import tempfile
 
 import pytest
 
@@ -26,10 +28,29 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_layouts_match_header():
-    assert ctypes.sizeof(_lib.BuildOpts) == 16
-    assert ctypes.sizeof(_lib.QueryOpts) == 16
-    assert ctypes.sizeof(_lib.BvhInfo) == 56
+def _c_layout(struct, fields):
+    """sizeof and member offsets of a struct of include/gpurt.h, from the C compiler."""
+    body = "\n".join(f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src = (f'#include <stdio.h>\n#include "gpurt.h"\nint main(void){{{body}'
+           f'printf("%zu\\n", sizeof({struct})); return 0;}}')
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "probe.c"), os.path.join(d, "probe")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        vals = [int(x) for x in subprocess.run([exe], check=True, capture_output=True,
+                                               text=True).stdout.split()]
+    return vals[-1], vals[:-1]
+
+
+@pytest.mark.parametrize("cls,struct", [(_lib.BuildOpts, "gpurt_build_opts"),
+                                        (_lib.QueryOpts, "gpurt_query_opts"),
+                                        (_lib.BvhInfo, "gpurt_bvh_info"),
+                                        (_lib.BatchMesh, "gpurt_batch_mesh")])
+def test_struct_layouts_match_header(cls, struct):
+    fields = [f for f, _ in cls._fields_]
+    size, offs = _c_layout(struct, fields)
+    assert ctypes.sizeof(cls) == size
+    assert [getattr(cls, f).offset for f in fields] == offs
 
 
 def test_null_arguments_return_invalid_without_device_work():

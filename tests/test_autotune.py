@@ -35,9 +35,10 @@ def test_tuned_queries_stay_exact(ctx, dev, tmp_path):
     R = W.pack_rays(rng.uniform(-1.5, 1.5, (8192, 3)), W.random_dirs(rng, 8192), 1e-5, 1e7)
     tr = torch.from_numpy(R).to(dev)
     path = str(tmp_path / "tune.json")
-    space = {"node_width": (2, 4), "stack_depth": (8, 16), "waves_per_cu": (0,), "treelet_size": (0, 5)}
+    space = {"node_width": (2, 4), "stack_depth": (8, 16), "waves_per_cu": (0,), "treelet_size": (0, 5),
+             "node_format": (1, 2)}
     best = autotune.tune_queries(ctx, tv, tf, queries=tq, rays=tr, space=space, path=path)
-    assert not best["cached"] and len(best["table"]) == 8
+    assert not best["cached"] and len(best["table"]) == 16
     again = autotune.tune_queries(ctx, tv, tf, queries=tq, rays=tr, space=space, path=path)
     assert again["cached"] and again["build"] == best["build"]
     bvh = gpurt.Bvh(ctx, tv, tf, **best["build"])
@@ -57,7 +58,8 @@ def test_tuned_batch(ctx, dev, tmp_path):
         meshes.append({"verts": torch.from_numpy(v.astype(np.float32)).to(dev),
                        "tris": torch.from_numpy(f.astype(np.int32)).to(dev),
                        "queries": torch.from_numpy(q).to(dev)})
-    space = {"node_width": (4, 8), "stack_depth": (8,), "waves_per_cu": (0,), "treelet_size": (0,)}
+    space = {"node_width": (4, 8), "stack_depth": (8,), "waves_per_cu": (0,), "treelet_size": (0,),
+             "node_format": (2,)}
     best = autotune.tune_batch(ctx, meshes, space=space, path=str(tmp_path / "t.json"))
     res = gpurt.Batch(ctx, meshes, **best["build"], **best["query"]).run()
     torch.cuda.synchronize()

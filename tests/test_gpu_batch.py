@@ -39,12 +39,13 @@ def _dev(meshes, dev):
     return [{k: torch.from_numpy(v).to(dev) for k, v in m.items()} for m in meshes]
 
 
+@pytest.mark.parametrize("fmt", [1, 2])
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_batch_matches_oracle_and_plain(ctx, dev, width):
+def test_batch_matches_oracle_and_plain(ctx, dev, width, fmt):
     host = _small_meshes()
     dm = _dev(host, dev)
-    print("batch width", width, "meshes", len(host), flush=True)
-    batch = gpurt.Batch(ctx, dm, node_width=width)
+    print("batch width", width, "format", fmt, "meshes", len(host), flush=True)
+    batch = gpurt.Batch(ctx, dm, node_width=width, node_format=fmt)
     res = batch.run()
     torch.cuda.synchronize()
     for i, (h, r) in enumerate(zip(host, res)):
@@ -58,7 +59,7 @@ def test_batch_matches_oracle_and_plain(ctx, dev, width):
         assert np.array_equal(r["r_t"].cpu().numpy(), rt), i
         assert np.array_equal(r["r_uv"].cpu().numpy(), ruv), i
         # the plain per-mesh path gives the same answers
-        bvh = gpurt.Bvh(ctx, dm[i]["verts"], dm[i]["tris"], node_width=width)
+        bvh = gpurt.Bvh(ctx, dm[i]["verts"], dm[i]["tris"], node_width=width, node_format=fmt)
         p, _, d, _ = bvh.closest_point(dm[i]["queries"])
         assert torch.equal(p, r["q_prim"]) and torch.equal(d, r["q_dist"])
 

@@ -4,6 +4,10 @@ Integer outputs (primitive ids, occlusion) and every FP32 output of the forward 
 are compared BIT-EXACTLY with the oracle's FP32 brute force: the kernels and the oracle
 perform the same rounded operations, and the traversal never culls a subtree that may
 hold an equal-or-better primitive, so the answer cannot depend on the tree.
+
+Both traversed node encodings are covered: FP32 child boxes (node_format 1) and 8-bit
+child boxes relative to the node (node_format 2), whose decoded boxes contain the FP32
+ones, so they change the work done but never the answers.
 """
 import numpy as np
 import pytest
@@ -27,7 +31,7 @@ def _np(*ts):
 
 def _cpq_check(ctx, dev, V, F, Q, **kw):
     bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **{k: v for k, v in kw.items() if k in
-                                                      ("node_width", "leaf_size")})
+                                                      ("node_width", "leaf_size", "node_format")})
     qkw = {k: v for k, v in kw.items() if k in ("stack_depth", "waves_per_cu")}
     prim, pt, dist, bary = _np(*bvh.closest_point(_t(Q, dev), **qkw))
     ep, ept, ed, eb = O.cpq_brute_f32(V, F, Q)
@@ -46,23 +50,29 @@ def test_cpq_small_full_workload_bitexact(ctx, dev):
     assert info["n_tris"] == 5120 and info["leaf_size"] == 4 and info["node_width"] == 4
 
 
+FORMATS = [1, 2]
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("width", [2, 4, 8])
-@pytest.mark.parametrize("stack", [16, 32, 64])
-def test_cpq_widths_and_stacks(ctx, dev, width, stack):
+@pytest.mark.parametrize("stack", [8, 16, 32, 64])
+def test_cpq_widths_and_stacks(ctx, dev, width, stack, fmt):
     w = W.cpq_small(n_queries=4096)
-    print("cpq width", width, "stack", stack, flush=True)
-    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], node_width=width, stack_depth=stack)
+    print("cpq width", width, "stack", stack, "format", fmt, flush=True)
+    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], node_width=width, stack_depth=stack,
+               node_format=fmt)
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("leaf", [1, 2, 8])
-def test_cpq_leaf_sizes(ctx, dev, leaf):
+def test_cpq_leaf_sizes(ctx, dev, leaf, fmt):
     w = W.cpq_small(n_queries=4096)
-    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], leaf_size=leaf)
+    _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], leaf_size=leaf, node_format=fmt)
 
 
 def _ray_check(ctx, dev, V, F, R, RA=None, **kw):
     bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **{k: v for k, v in kw.items() if k in
-                                                      ("node_width", "leaf_size")})
+                                                      ("node_width", "leaf_size", "node_format")})
     qkw = {k: v for k, v in kw.items() if k in ("stack_depth",)}
     prim, t, uv = _np(*bvh.trace_closest(_t(R, dev), **qkw))
     ep, et, euv = O.ray_closest_brute_f32(V, F, R)
@@ -75,11 +85,13 @@ def _ray_check(ctx, dev, V, F, R, RA=None, **kw):
     return prim
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_rays_small_scene_bitexact(ctx, dev, width):
+def test_rays_small_scene_bitexact(ctx, dev, width, fmt):
     w = W.rays_scene(n_rays=8192, grid=101, n_spheres=6)
-    print("rays", w["meta"], "width", width, flush=True)
-    prim = _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], node_width=width)
+    print("rays", w["meta"], "width", width, "format", fmt, flush=True)
+    prim = _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], node_width=width,
+                      node_format=fmt)
     assert 0.2 < (prim >= 0).mean() < 0.9
 
 
@@ -97,24 +109,51 @@ def test_vertex_strides(ctx, dev, stride):
     assert np.array_equal(prim, ep) and np.array_equal(dist, ed)
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("n_tris", [1, 2, 3, 5, 17])
-def test_tiny_meshes(ctx, dev, n_tris):
+def test_tiny_meshes(ctx, dev, n_tris, fmt):
     rng = np.random.default_rng(n_tris)
     V = rng.normal(size=(3 * n_tris, 3)).astype(np.float32)
     F = np.arange(3 * n_tris, dtype=np.int32).reshape(-1, 3)
     Q = rng.normal(size=(777, 3)).astype(np.float32)
-    _cpq_check(ctx, dev, V, F, Q)
+    _cpq_check(ctx, dev, V, F, Q, node_format=fmt)
     o = rng.normal(size=(777, 3)) * 3
     d = W.random_dirs(rng, 777)
-    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7))
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7), node_format=fmt)
 
 
-def test_exact_ties_go_to_lowest_index(ctx, dev):
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_quantized_nodes_are_smaller_and_exact(ctx, dev, width):
+    """8-bit child boxes: smaller nodes, boxes that can only be looser (so never fewer
+    node visits than FP32 boxes on the same tree), and bit-identical answers."""
+    w = W.rays_scene(n_rays=8192, grid=101, n_spheres=6)
+    V, F, R = _t(w["verts"], dev), _t(w["tris"], dev), _t(w["rays"], dev)
+    f32 = gpurt.Bvh(ctx, V, F, node_width=width, node_format=1)
+    q8 = gpurt.Bvh(ctx, V, F, node_width=width, node_format=2)
+    i32, i8 = f32.info(), q8.info()
+    assert i32["node_format"] == 1 and i8["node_format"] == 2
+    assert i32["node_bytes"] == 32 * width and i8["node_bytes"] == {2: 48, 4: 64, 8: 96}[width]
+    assert i32["n_nodes"] == i8["n_nodes"]
+    s32 = torch.zeros(4, dtype=torch.int64, device=dev)
+    s8 = torch.zeros(4, dtype=torch.int64, device=dev)
+    a = _np(*f32.trace_closest(R, stats=s32))
+    b = _np(*q8.trace_closest(R, stats=s8))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    n32, t32 = s32[:2].tolist()
+    n8, t8 = s8[:2].tolist()
+    assert n8 >= n32 and t8 >= t32
+    # the looser boxes cost little: at most a few percent more visits here
+    assert n8 <= 1.2 * n32, (n8, n32)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_exact_ties_go_to_lowest_index(ctx, dev, fmt):
     a, b, c = [0, 0, 0], [1, 0, 0], [0, 1, 0]
     V = np.array([a, b, c], np.float32)
     F = np.array([[2, 0, 1], [0, 1, 2], [1, 2, 0]] * 5, np.int32)  # 15 copies of one triangle
     Q = np.array([[0.25, 0.25, 1], [-1, -1, 0], [0.5, 0.5, 0], [3, 3, 3]], np.float32)
-    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), leaf_size=2)
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), leaf_size=2, node_format=fmt)
     prim, = _np(bvh.closest_point(_t(Q, dev))[0])
     assert prim.tolist() == [0, 0, 0, 0]
     R = W.pack_rays(np.array([[0.2, 0.3, 1.0], [0.2, 0.3, -1.0]]),
@@ -123,21 +162,23 @@ def test_exact_ties_go_to_lowest_index(ctx, dev):
     assert p.tolist() == [0, 0] and t.tolist() == [1.0, 1.0]
 
 
-def test_shared_vertices_and_edges_of_a_closed_mesh(ctx, dev):
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_shared_vertices_and_edges_of_a_closed_mesh(ctx, dev, fmt):
     """Queries exactly at vertices and on edges of an icosphere: exact FP32 ties
     between neighbouring triangles resolve to the lowest index."""
     v, f = W.icosphere(3)
     V, F = v.astype(np.float32), f.astype(np.int32)
     mid = ((V[F[:, 0]] + V[F[:, 1]]) * np.float32(0.5)).astype(np.float32)
     Q = np.concatenate([V, mid, V * np.float32(1.5), mid * np.float32(0.5)]).astype(np.float32)
-    _cpq_check(ctx, dev, V, F, Q)
+    _cpq_check(ctx, dev, V, F, Q, node_format=fmt)
     # rays aimed at vertices and edge midpoints from outside
     o = np.concatenate([V, mid]) * 3.0
     d = -o / np.linalg.norm(o, axis=1, keepdims=True)
-    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7))
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, d, 0.0, 1e7), node_format=fmt)
 
 
-def test_degenerate_triangles(ctx, dev):
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_degenerate_triangles(ctx, dev, fmt):
     rng = np.random.default_rng(5)
     w = W.cpq_small(n_queries=3000)
     V, F = w["verts"], w["tris"].copy()
@@ -146,9 +187,9 @@ def test_degenerate_triangles(ctx, dev):
     F[k[100:200], 2] = F[k[100:200], 0]
     F[k[200:], 1] = F[k[200:], 0]
     F[k[200:], 2] = F[k[200:], 0]            # a point triangle
-    _cpq_check(ctx, dev, V, F, w["queries"])
+    _cpq_check(ctx, dev, V, F, w["queries"], node_format=fmt)
     o = rng.uniform(-1.5, 1.5, (3000, 3))
-    _ray_check(ctx, dev, V, F, W.pack_rays(o, W.random_dirs(rng, 3000), 1e-5, 1e7))
+    _ray_check(ctx, dev, V, F, W.pack_rays(o, W.random_dirs(rng, 3000), 1e-5, 1e7), node_format=fmt)
 
 
 def test_ray_interval_bounds(ctx, dev):
@@ -177,6 +218,8 @@ def test_bad_options_are_rejected(ctx, dev):
     w = W.cpq_small(n_queries=16)
     with pytest.raises(gpurt.GpurtError):
         gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev), node_width=3)
+    with pytest.raises(gpurt.GpurtError):
+        gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev), node_format=7)
     bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
     with pytest.raises(gpurt.GpurtError):
         bvh.closest_point(_t(w["queries"], dev), stack_depth=24)
@@ -204,14 +247,16 @@ def test_stats_count_visits(ctx, dev):
     assert tris < 0.05 * 4096 * 5120
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_refit_after_moving_vertices(ctx, dev, width):
+def test_refit_after_moving_vertices(ctx, dev, width, fmt):
     """Refit (same triangles, moved vertices) answers exactly like brute force on the
-    moved mesh, for closest points and rays, and twice in a row."""
+    moved mesh, for closest points and rays, and twice in a row (quantized nodes are
+    re-encoded from the refitted boxes)."""
     w = W.cpq_small(n_queries=3000)
     V, F, Q = w["verts"], w["tris"], w["queries"]
     tv = _t(V, dev)
-    bvh = gpurt.Bvh(ctx, tv, _t(F, dev), node_width=width)
+    bvh = gpurt.Bvh(ctx, tv, _t(F, dev), node_width=width, node_format=fmt)
     rng = np.random.default_rng(9)
     o = rng.uniform(-1.5, 1.5, (3000, 3))
     R = W.pack_rays(o, W.random_dirs(rng, 3000), 1e-5, 1e7)
