@@ -68,7 +68,8 @@ def measured_traffic(args, n_tris, n_rays, node_format):
     if any(k not in m for k in keys):
         return None
     if (m["n_tris"], m["n_rays"], m["node_width"], m["stack_depth"], m["treelet_size"],
-            m["node_format"]) != (n_tris, n_rays, args.width, args.stack, args.treelet, node_format):
+            m["node_format"], m.get("leaf_size", 4), m.get("waves_per_cu", 0)) != \
+            (n_tris, n_rays, args.width, args.stack, args.treelet, node_format, args.leaf, args.waves):
         return None
     rd = m["fetch_kb_raw"] * m["calib_kib"] / m["calib_fetch_kb_raw"]
     wr = m["write_kb_raw"] * m["calib_kib"] / m["calib_write_kb_raw"]
@@ -88,7 +89,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
-                    node_format=args.format)
+                    node_format=args.format, leaf_size=args.leaf)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
     log(f"[bench] GPU build {build_ms:.1f} ms, {info}")
@@ -98,20 +99,20 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     occ = torch.empty(n, dtype=torch.uint8, device=dev)
 
     def step():
-        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack)
-        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack)
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves)
+        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves)
 
     ms = timed(step, args.steps, args.warmup)
-    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack),
+    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves),
                        args.steps, 1)
-    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack), args.steps, 1)
+    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves), args.steps, 1)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
-    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, stats=st)
+    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
     torch.cuda.synchronize()
     nodes, tris, spills, _ = st.cpu().tolist()
     bytes_closest = n * (32 + 16) + nodes * info["node_bytes"] + tris * TRI_BYTES
     st.zero_()
-    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, stats=st)
+    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
     torch.cuda.synchronize()
     nodes_a, tris_a, spills_a, _ = st.cpu().tolist()
     hit_rate = float((prim >= 0).float().mean().item())
@@ -157,7 +158,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "ms_per_step": ms,
         "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
                    "passes": ["closest_hit", "any_hit"], "node_width": args.width,
-                   "leaf_size": 4, "stack_depth": args.stack, "treelet_size": args.treelet,
+                   "leaf_size": args.leaf, "stack_depth": args.stack, "treelet_size": args.treelet,
                    "node_format": info["node_format"]},
         "roofline": roof,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
@@ -184,14 +185,14 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
-                    node_format=args.format)
+                    node_format=args.format, leaf_size=args.leaf)
     build_ms = (time.perf_counter() - t0) * 1e3
     n = q.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
            torch.empty(n, device=dev), torch.empty((n, 3), device=dev))
-    ms = timed(lambda: bvh.closest_point(q, out=out, stack_depth=args.stack), args.steps, 2)
+    ms = timed(lambda: bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves), args.steps, 2)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
-    bvh.closest_point(q, out=out, stack_depth=args.stack, stats=st)
+    bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
     torch.cuda.synchronize()
     nodes, tris, _, _ = st.cpu().tolist()
     byts = n * (12 + 20 + 12) + nodes * bvh.info()["cpq_node_bytes"] + tris * TRI_BYTES
@@ -211,7 +212,7 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     o0 = torch.from_numpy(w["ray_o"]).to(dev)
     d0 = torch.from_numpy(w["ray_d"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
-                    node_format=args.format)
+                    node_format=args.format, leaf_size=args.leaf)
     nq, nr = q0.shape[0], o0.shape[0]
 
     def fwd():
@@ -250,6 +251,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
+                        waves_per_cu=args.waves, leaf_size=args.leaf,
                         treelet_size=args.batch_treelet, node_format=args.format)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
@@ -259,10 +261,10 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
     def plain():
         for i, m in enumerate(meshes):
-            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width,
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
                             treelet_size=args.batch_treelet, node_format=args.format)
-            p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack)
-            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack)
+            p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves)
+            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves)
             plain_out[i] = (p, d, rp, rt)
             bvh.close()
 
@@ -293,12 +295,12 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     tf = torch.from_numpy(w["tris"]).to(dev)
     rays = torch.from_numpy(w["rays"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
-                    node_format=args.format)
+                    node_format=args.format, leaf_size=args.leaf)
     n = rays.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
            torch.empty((n, 2), device=dev))
     for _ in range(args.profile_reps):
-        bvh.trace_closest(rays, out=out, stack_depth=args.stack)
+        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(args.profile_reps):
@@ -316,6 +318,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
     ap.add_argument("--stack", type=int, default=8, choices=[8, 16, 32, 64])
+    ap.add_argument("--leaf", type=int, default=4, choices=range(1, 9), metavar="1..8",
+                    help="max triangles per leaf")
+    ap.add_argument("--waves", type=int, default=0, help="resident waves per CU (0 = library choice)")
     ap.add_argument("--treelet", type=int, default=7, choices=[0, 5, 6, 7],
                     help="SAH treelet restructuring of the single-mesh BVHs (leaves per "
                          "treelet), 0 = off; the batched leg rebuilds every step and uses "

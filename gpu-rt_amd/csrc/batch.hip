@@ -4,7 +4,7 @@
 // Morton key, so each mesh is its own subtree of one radix tree), so the BVHs of all
 // meshes are built by the same ~30 launches as one mesh.  Queries then run in two
 // persistent launches (closest points, then rays) of the streaming traversal: a wave
-// takes 64-query chunks from a device work counter and hands their queries to its lanes
+// takes 64-query chunks from device work queues and hands their queries to its lanes
 // as lanes finish, until the chunk list is exhausted; a query starts at its mesh's root
 // (wide node m of the shared node array).  Chunks are ordered by decreasing mesh size so
 // the tail of a launch is made of cheap chunks.  The whole sequence -- build of every
@@ -32,14 +32,17 @@ struct BatchMeshDev {
     uint32_t n_r;
 };
 
+// Clears the queue counters of both query kinds (2 * kQueues, kQueueStride apart).
 __global__ void k_reset_counters(uint32_t* c) {
-    if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * kQueues) c[i * kQueueStride] = 0u;
 }
 
-// Query source over a chunk list (traverse.h): a wave takes the next chunk (mesh m,
-// first query) from the device counter when its pool is empty and hands the chunk's
-// queries to the lanes that want one, in lane order.  Called by all lanes together, so
-// the pool stays wave-uniform.
+// Query source over a chunk list (traverse.h): the chunks are dealt round-robin to
+// nq = min(kQueues, blocks) queues with a device counter each (as the plain kernels'
+// QueueSource), and block b's waves take the next chunk (mesh m, first query) of queue
+// b % nq when their pool is empty, handing the chunk's queries to the lanes that want
+// one, in lane order.  Called by all lanes together, so the pool stays wave-uniform.
 struct ChunkSource {
     struct Handle {
         uint32_t m, i;
@@ -47,25 +50,28 @@ struct ChunkSource {
     const BatchMeshDev* meshes;
     const uint2* chunks;
     uint32_t n_chunks;
-    uint32_t* counter;
+    uint32_t* counters;
     bool rays;
     uint32_t pm = 0, pnext = 0, pend = 0;  // current chunk: mesh and [pnext, pend)
     bool more = true;
     __device__ __forceinline__ bool refill(bool has, Handle& h) {
         const uint32_t lane = threadIdx.x & 63u;
         const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+        const uint32_t nq = min(kQueues, gridDim.x);  // every queue has blocks serving it
+        const uint32_t q = blockIdx.x % nq;
         bool got = false;
         uint64_t want = __ballot(!has);
         while (want && more) {
             if (pnext >= pend) {
                 uint32_t c = 0;
-                if (lane == 0) c = gatomic_add(counter, 1u);
+                if (lane == 0) c = gatomic_add(counters + q * kQueueStride, 1u);
                 c = __builtin_amdgcn_readfirstlane(c);
-                if (c >= n_chunks) {
+                const uint64_t j = (uint64_t)c * nq + q;
+                if (j >= n_chunks) {
                     more = false;
                     break;
                 }
-                const uint2 ch = chunks[c];
+                const uint2 ch = chunks[j];
                 pm = ch.x;
                 pnext = ch.y;
                 const uint32_t nq = rays ? meshes[pm].n_r : meshes[pm].n_q;
@@ -199,7 +205,7 @@ struct gpurt_batch_s {
     BatchMeshDev* meshes_d = nullptr;
     uint2* chunks_d = nullptr;    // closest-point chunks, then ray chunks
     uint32_t n_cpq_chunks = 0, n_ray_chunks = 0;
-    uint32_t* counter_d = nullptr;  // [0]: closest-point chunks taken, [1]: ray chunks
+    uint32_t* counter_d = nullptr;  // kQueues closest-point queue counters, then kQueues for rays
     uint2* spill = nullptr;
     uint32_t spill_slots = 0;
     unsigned grid = 0;
@@ -241,7 +247,7 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
         // counters[0] holds the wide-node count the levels produced
         launch_quantize(b->width, b->build.nodes, b->qnodes, 0, b->build.capacity, b->build.counters, s);
     }
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->counter_d);
+    hipLaunchKernelGGL(k_reset_counters, dim3(cdiv(2 * kQueues, 256)), dim3(256), 0, s, b->counter_d);
     BatchArgs a{};
     a.tris = b->build.final_tris();
     a.meshes = b->meshes_d;
@@ -255,7 +261,7 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
         a.nodes = q8 ? b->qnodes : b->build.nodes;
         a.chunks = b->chunks_d + (rays ? b->n_cpq_chunks : 0u);
         a.n_chunks = n;
-        a.counter = b->counter_d + rays;
+        a.counter = b->counter_d + (size_t)rays * kQueues * kQueueStride;
         const unsigned grid = std::max<size_t>(1, std::min<size_t>(b->grid, (size_t)n * 64 / b->block + 1));
         if (!launch_batch(b->width, q8, b->stack, rays != 0, grid, s, a))
             return set_error(GPURT_E_UNSUPPORTED, "batch stack/width");
@@ -356,7 +362,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->spill_slots = need > S ? (uint32_t)(need - S) : 0u;
     bool fail = hipMalloc((void**)&b->meshes_d, sizeof(BatchMeshDev) * n_meshes) != hipSuccess ||
                 hipMalloc((void**)&b->chunks_d, sizeof(uint2) * std::max<size_t>(1, chunks.size())) != hipSuccess ||
-                hipMalloc((void**)&b->counter_d, 256) != hipSuccess ||
+                hipMalloc((void**)&b->counter_d, sizeof(uint32_t) * 2 * kQueues * kQueueStride) != hipSuccess ||
                 (has_qnodes(bp.format) && hipMalloc(&b->qnodes, qnode_bytes(W) * std::max<uint32_t>(1u, b->build.capacity)) != hipSuccess) ||
                 (b->spill_slots &&
                  hipMalloc((void**)&b->spill, (size_t)b->spill_slots * b->grid * BS * sizeof(uint2)) != hipSuccess);

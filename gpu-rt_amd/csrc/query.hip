@@ -29,6 +29,7 @@ struct TravArgs {
     int32_t* prim;
     uint2* spill;
     uint32_t spill_slots;
+    uint32_t* counter;  // kQueues work-queue positions (64-query chunks), zero at launch
     unsigned long long* stats;
     uint32_t* err;
 };
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
-    StrideSource src{nullptr, a.q, a.n, gid, total};
+    QueueSource src{nullptr, a.q, a.n, gid, total, a.counter};
     auto out = [&](uint32_t qi, uint32_t id, float d2, V3 cp, V3 br) {
         a.prim[qi] = (int32_t)id;
         a.pt[3 * (size_t)qi] = cp.x;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
-    StrideSource src{a.rays, nullptr, a.n, gid, total};
+    QueueSource src{a.rays, nullptr, a.n, gid, total, a.counter};
     auto out = [&](uint32_t ri, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
             a.occ[ri] = any ? 1 : 0;
@@ -94,6 +95,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
         atomicAdd(&a.stats[1], n_tris);
         atomicAdd(&a.stats[2], (unsigned long long)st.spills);
     }
+}
+
+__global__ void k_clear_counters(uint32_t* c) {
+    if (threadIdx.x < kQueues) c[threadIdx.x * kQueueStride] = 0u;
 }
 
 // ------------------------------------------------------------------------------------
@@ -159,19 +164,24 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
     // part beyond the LDS slots spills to a stream-ordered allocation owned by this call
     const int need = (b->width - 1) * b->depth + 1;
     uint32_t spill_slots = need > S ? (uint32_t)(need - S) : 0u;
-    void* spill = nullptr;
-    if (spill_slots) {
-        size_t bytes = (size_t)spill_slots * total * sizeof(uint2);
-        if (hipMallocAsync(&spill, bytes, s) != hipSuccess) {
-            (void)hipGetLastError();
-            return set_error(GPURT_E_OOM, "traversal spill buffer allocation failed");
-        }
+    // one stream-ordered allocation owned by this call: the queue counters and the
+    // stack entries beyond the LDS slots
+    constexpr size_t kCounterBytes = (size_t)kQueues * kQueueStride * sizeof(uint32_t);
+    void* scratch = nullptr;
+    const size_t bytes = kCounterBytes + (size_t)spill_slots * total * sizeof(uint2);
+    if (hipMallocAsync(&scratch, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GPURT_E_OOM, "traversal scratch allocation failed");
     }
+    uint32_t* counter = (uint32_t*)scratch;
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, counter);
+    void* spill = spill_slots ? (void*)((char*)scratch + kCounterBytes) : nullptr;
     a.tris = b->tris;
     a.root = 0;
     a.n = (uint32_t)n;
     a.spill = (uint2*)spill;
     a.spill_slots = spill_slots;
+    a.counter = counter;
     a.stats = opts ? (unsigned long long*)opts->stats : nullptr;
     a.err = ctx->err;
     const bool stats = a.stats != nullptr;
@@ -193,11 +203,11 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
         default: break;
     }
     if (bs == 0) {
-        if (spill) (void)hipFreeAsync(spill, s);
+        (void)hipFreeAsync(scratch, s);
         return set_error(GPURT_E_UNSUPPORTED, "node width");
     }
     hipError_t le = hipGetLastError();
-    if (spill) (void)hipFreeAsync(spill, s);
+    (void)hipFreeAsync(scratch, s);
     if (le != hipSuccess) return hip_error(le, "traversal launch");
     return GPURT_OK;
 }

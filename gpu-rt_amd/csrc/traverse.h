@@ -39,24 +39,39 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t spill_rsrc(void* spill, uint32
 
 template <int S, int BS>
 struct LaneStack {
+    // Only sp (and the spill count of counting kernels) is per lane: the wave's LDS
+    // column and spill base are wave-uniform (scalar registers), and the lane's own
+    // offsets are rebuilt from its lane id when needed, so no vector register holds an
+    // address for the whole traversal.
     __amdgpu_buffer_rsrc_t spill;
-    uint32_t lane, base, cap;  // base: byte offset of this lane's spill slots
+    uint32_t wave_lds;    // first LDS column of this wave (wave index in block * 64)
+    uint32_t wave_gid;    // global lane index of this wave's lane 0
+    uint32_t slots, cap;  // spill slots per lane; LDS + spill slots per lane
     int sp;
     uint32_t spills;  // pushes that went to the spill buffer (reported by counting kernels)
     __device__ __forceinline__ void init(void* spill_buf, uint32_t spill_slots, uint32_t gid, uint32_t total) {
+        (void)gid;
         spill = spill_rsrc(spill_buf, spill_slots, total);
-        lane = threadIdx.x;
-        base = gid * spill_slots * (uint32_t)sizeof(uint2);
+        const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+        wave_lds = w;
+        wave_gid = blockIdx.x * BS + w;
+        slots = spill_slots;
         cap = S + spill_slots;
         sp = 0;
         spills = 0;
     }
+    __device__ __forceinline__ uint32_t lds_index(int slot) const {
+        return (uint32_t)slot * BS + wave_lds + __lane_id();
+    }
+    __device__ __forceinline__ uint32_t spill_off(int slot) const {
+        return ((wave_gid + __lane_id()) * slots + (uint32_t)(slot - S)) * (uint32_t)sizeof(uint2);
+    }
     __device__ __forceinline__ bool push(uint32_t ref, float key) {
         if ((uint32_t)sp >= cap) return false;
         if (sp < S) {
-            lds_stack[sp * BS + lane] = make_uint2(ref, __float_as_uint(key));
+            lds_stack[lds_index(sp)] = make_uint2(ref, __float_as_uint(key));
         } else {
-            const uint32_t off = base + (uint32_t)(sp - S) * (uint32_t)sizeof(uint2);
+            const uint32_t off = spill_off(sp);
             spills++;
             __builtin_amdgcn_raw_buffer_store_b32(ref, spill, off, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(key), spill, off + 4, 0, 0);
@@ -66,8 +81,8 @@ struct LaneStack {
     }
     __device__ __forceinline__ uint2 pop() {
         sp--;
-        if (sp < S) return lds_stack[sp * BS + lane];
-        const uint32_t off = base + (uint32_t)(sp - S) * (uint32_t)sizeof(uint2);
+        if (sp < S) return lds_stack[lds_index(sp)];
+        const uint32_t off = spill_off(sp);
         return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(spill, off, 0, 0),
                           __builtin_amdgcn_raw_buffer_load_b32(spill, off + 4, 0, 0));
     }
@@ -236,6 +251,76 @@ struct StrideSource {
         h = next;
         next += stride;
         return true;
+    }
+    __device__ __forceinline__ uint32_t root(Handle) const { return 0u; }
+    __device__ __forceinline__ void ray(Handle h, float4& r0, float4& r1) const {
+        r0 = rays[2 * (size_t)h];
+        r1 = rays[2 * (size_t)h + 1];
+    }
+    __device__ __forceinline__ V3 point(Handle h) const {
+        return v3(pts[3 * (size_t)h], pts[3 * (size_t)h + 1], pts[3 * (size_t)h + 2]);
+    }
+};
+
+// Plain source with load balancing: lane g takes query g first; the queries after the
+// first `total` are cut into 64-query chunks dealt round-robin to nq = min(kQueues,
+// blocks) queues (chunk j belongs to queue j % nq), each with its own device counter
+// (zero at launch, 256 bytes apart).  Block b's waves take chunks of queue b % nq, so
+// every queue is served by at least one block: a wave takes a
+// chunk when lanes want work and its pool is empty, and gives the chunk's queries to
+// those lanes in lane order.  Lanes that get cheap queries take more of them, so no
+// lane idles while others still hold several; many counters keep the device-wide
+// atomics from serializing on one address (one shared counter costs ~0.1 ms per
+// million queries).  refill is called by all lanes together, so the pool stays
+// wave-uniform.
+constexpr uint32_t kQueues = 256;
+constexpr uint32_t kQueueStride = 64;  // uint32 words between counters
+
+struct QueueSource {
+    using Handle = uint32_t;
+    const float4* rays;
+    const float* pts;
+    uint32_t n, first, total;
+    uint32_t* counters;
+    bool started = false;
+    uint32_t pnext = 0, pend = 0;  // wave's pool: queries [pnext, pend) not handed out
+    bool more = true;
+    __device__ __forceinline__ bool refill(bool has, Handle& h) {
+        if (!started) {  // first call (all lanes together): the static first query only
+            started = true;
+            more = total < n;
+            h = first;
+            return first < n;
+        }
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+        const uint32_t nq = min(kQueues, gridDim.x);  // every queue has blocks serving it
+        const uint32_t q = blockIdx.x % nq;
+        bool got = false;
+        uint64_t want = __ballot(!has);
+        while (want && more) {
+            if (pnext >= pend) {
+                uint32_t c = 0;
+                if (lane == 0) c = gatomic_add(counters + q * kQueueStride, 1u);
+                c = __builtin_amdgcn_readfirstlane(c);
+                const uint64_t beg = (uint64_t)total + ((uint64_t)c * nq + q) * 64u;
+                if (beg >= n) {
+                    more = false;
+                    break;
+                }
+                pnext = (uint32_t)beg;
+                pend = beg + 64u < n ? (uint32_t)(beg + 64u) : n;
+            }
+            const uint32_t k = min(pend - pnext, (uint32_t)__popcll(want));
+            const uint32_t rank = (uint32_t)__popcll(want & below);
+            if (!has && !got && rank < k) {
+                h = pnext + rank;
+                got = true;
+            }
+            pnext += k;
+            want = __ballot(!has && !got);
+        }
+        return got;
     }
     __device__ __forceinline__ uint32_t root(Handle) const { return 0u; }
     __device__ __forceinline__ void ray(Handle h, float4& r0, float4& r1) const {
