@@ -241,9 +241,10 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
 
 def batched_bench(args, torch, gpurt, W, dev, ctx):
     """256 meshes (log-uniform 1K..524K triangles), each with its own closest-point
-    queries and rays: one batch replay (all builds + one persistent query launch, a HIP
-    graph) against the plain path (per mesh: build, closest-point launch, closest-hit
-    launch), both rebuilding every BVH each step."""
+    queries and rays: one batch replay (all builds + the persistent query launches, a
+    HIP graph) against the plain path (per mesh: build, closest-point launch, closest-hit
+    launch), both rebuilding every BVH each step; then the same with the backward pass
+    (batched backward graph vs the plain per-mesh backward launches)."""
     t0 = time.time()
     w = W.batched()
     log(f"[bench] batched generated in {time.time() - t0:.1f}s: {w['meta']}")
@@ -273,6 +274,40 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     parity = all(torch.equal(plain_out[i][0], r["q_prim"]) and torch.equal(plain_out[i][1], r["q_dist"])
                  and torch.equal(plain_out[i][2], r["r_prim"]) and torch.equal(plain_out[i][3], r["r_t"])
                  for i, r in enumerate(res))
+
+    # forward + backward: gradients w.r.t. query points, ray origins/directions and all
+    # vertices (distance + hit-t terms), batched graph vs the plain per-mesh path
+    grads = batch.enable_backward()
+
+    def batched_fb():
+        batch.run()
+        batch.backward()
+
+    ms_bfb = timed(batched_fb, max(3, args.steps // 4), 1)
+    plain_g = {}
+
+    def plain_fb():
+        for i, m in enumerate(meshes):
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
+                            treelet_size=args.batch_treelet, node_format=args.format)
+            p, pt, d, b = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves)
+            rp, rt, ruv = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves)
+            g = grads[i]
+            gq = torch.empty_like(m["queries"])
+            go = torch.empty((rp.shape[0], 3), device=dev)
+            gd = torch.empty((rp.shape[0], 3), device=dev)
+            gv = torch.zeros((m["verts"].shape[0], 3), device=dev)
+            bvh.closest_point_bwd(m["queries"], p, b, pt, d, g["g_dist"], gq, gv)
+            bvh.trace_bwd(m["rays"], rp, rt, ruv, g["g_t"], go, gd, gv)
+            plain_g[i] = (gq, go, gv)
+            bvh.close()
+
+    ms_pfb = timed(plain_fb, 2, 1)
+    torch.cuda.synchronize()
+    parity_bwd = all(torch.equal(plain_g[i][0], g["g_q"]) and torch.equal(plain_g[i][1], g["g_origin"])
+                     and float((plain_g[i][2] - g["g_verts"]).abs().max()) <=
+                     1e-5 * (float(plain_g[i][2].abs().max()) + 1e-6)
+                     for i, g in enumerate(grads))
     nq = sum(m["queries"].shape[0] for m in meshes)
     nr = sum(m["rays"].shape[0] for m in meshes)
     batch.close()
@@ -280,9 +315,14 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "n_queries": nq, "n_rays": nr,
             "batched_units_per_s": (nq + nr) / (ms_b * 1e-3), "batched_ms": ms_b,
             "plain_units_per_s": (nq + nr) / (ms_p * 1e-3), "plain_ms": ms_p,
-            "batched_over_plain": ms_p / ms_b, "batch_create_s": create_s,
-            "parity_batched_vs_plain": bool(parity), "treelet_size": args.batch_treelet,
-            "note": "each step rebuilds all 256 BVHs and answers every query"}
+            "batched_over_plain": ms_p / ms_b,
+            "fwd_bwd_batched_units_per_s": (nq + nr) / (ms_bfb * 1e-3), "fwd_bwd_batched_ms": ms_bfb,
+            "fwd_bwd_plain_units_per_s": (nq + nr) / (ms_pfb * 1e-3), "fwd_bwd_plain_ms": ms_pfb,
+            "fwd_bwd_batched_over_plain": ms_pfb / ms_bfb, "batch_create_s": create_s,
+            "parity_batched_vs_plain": bool(parity), "parity_bwd_batched_vs_plain": bool(parity_bwd),
+            "treelet_size": args.batch_treelet,
+            "note": "each step rebuilds all 256 BVHs and answers every query; fwd+bwd adds the "
+                    "gradients of sum(dist) and sum(t) w.r.t. queries, rays and vertices"}
 
 
 def profile_mode(args, torch, gpurt, W, dev, ctx):

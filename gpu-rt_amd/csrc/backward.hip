@@ -1,25 +1,13 @@
-// backward.hip -- gradients of unsigned distance and ray hit distance.
+// backward.hip -- gradients of unsigned distance and ray hit distance (plain path).
 //
 // Both kernels reuse the forward pass's saved primitive ids and barycentrics (no
 // re-traversal) and scatter vertex gradients with hardware FP32 atomic adds
 // (global_atomic_add_f32), so their cost is one gather of the triangle plus three
-// 12-byte atomic read-modify-writes per query.
-//
-//   distance  d = |q - p|, p = sum_j b_j v_j the closest point (envelope theorem, the
-//             minimiser over the convex triangle is unique):
-//             dd/dq = (q - p)/d,  dd/dv_j = -b_j (q - p)/d
-//   hit t     t = n.(v0 - o)/(n.d), n = (v1-v0) x (v2-v0), b = (1-u-v, u, v) fixed:
-//             dt/do = -n/(n.d),  dt/dd = -t n/(n.d),  dt/dv_j = b_j n/(n.d)
-#include "gpurt_internal.h"
-#include "tri_math.h"
+// 12-byte atomic read-modify-writes per query.  The per-query math is in backward.h.
+#include "backward.h"
 
 namespace gpurt_impl {
 
-// The offset q - p is re-formed relative to the triangle's first vertex,
-// (q - v0) - (b1 (v1 - v0) + b2 (v2 - v0)), like the forward pass, instead of from the
-// rounded closest point: its error then scales with the triangle, not the coordinates,
-// which keeps the direction accurate for queries very close to the surface.  Without
-// barycentrics it falls back to q - pt.
 __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __restrict__ q,
                                                     const int32_t* __restrict__ prim,
                                                     const float* __restrict__ bary,
@@ -29,48 +17,9 @@ __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __r
                                                     const uint32_t* __restrict__ tri_idx,
                                                     uint32_t n_tris, const float* __restrict__ verts,
                                                     float* __restrict__ g_q, float* g_verts) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int32_t p = prim[i];
-    const float d = dist[i];
-    float gx = 0.f, gy = 0.f, gz = 0.f;
-    // ids outside the mesh (not produced by the forward pass) contribute nothing
-    const bool live = p >= 0 && (uint32_t)p < n_tris && d > 0.0f && d < INFINITY;
-    if (live) {
-        const float s = g[i] / d;
-        const V3 qq = v3(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2]);
-        V3 df;
-        if (bary) {
-            const size_t i0 = tri_idx[3 * (size_t)p], i1 = tri_idx[3 * (size_t)p + 1], i2 = tri_idx[3 * (size_t)p + 2];
-            const V3 a = v3(verts[3 * i0], verts[3 * i0 + 1], verts[3 * i0 + 2]);
-            const V3 e1 = sub(v3(verts[3 * i1], verts[3 * i1 + 1], verts[3 * i1 + 2]), a);
-            const V3 e2 = sub(v3(verts[3 * i2], verts[3 * i2 + 1], verts[3 * i2 + 2]), a);
-            const float b1 = bary[3 * (size_t)i + 1], b2 = bary[3 * (size_t)i + 2];
-            const V3 ap = sub(qq, a);
-            df = v3(ap.x - (e1.x * b1 + e2.x * b2), ap.y - (e1.y * b1 + e2.y * b2), ap.z - (e1.z * b1 + e2.z * b2));
-        } else {
-            df = sub(qq, v3(pt[3 * (size_t)i], pt[3 * (size_t)i + 1], pt[3 * (size_t)i + 2]));
-        }
-        gx = df.x * s;
-        gy = df.y * s;
-        gz = df.z * s;
-    }
-    if (g_q) {
-        g_q[3 * (size_t)i] = gx;
-        g_q[3 * (size_t)i + 1] = gy;
-        g_q[3 * (size_t)i + 2] = gz;
-    }
-    if (g_verts && live) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float b = bary[3 * (size_t)i + j];
-            if (b == 0.0f) continue;
-            const size_t v = tri_idx[3 * (size_t)p + j];
-            unsafeAtomicAdd(&g_verts[3 * v], -b * gx);
-            unsafeAtomicAdd(&g_verts[3 * v + 1], -b * gy);
-            unsafeAtomicAdd(&g_verts[3 * v + 2], -b * gz);
-        }
-    }
+    cpq_bwd_one(i, q, prim, bary, pt, dist, g, tri_idx, n_tris, verts, 3, g_q, g_verts);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ray_bwd(uint32_t n, const float4* __restrict__ rays,
@@ -83,54 +32,9 @@ __global__ __launch_bounds__(kBlock) void k_ray_bwd(uint32_t n, const float4* __
                                                     const float* __restrict__ verts,
                                                     float* __restrict__ g_o,
                                                     float* __restrict__ g_d, float* g_verts) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int32_t p = prim[i];
-    V3 go = v3(0.f, 0.f, 0.f), gd = v3(0.f, 0.f, 0.f);
-    V3 nrm = v3(0.f, 0.f, 0.f);
-    float s = 0.f;
-    uint32_t vi[3] = {0, 0, 0};
-    bool live = false;
-    if (p >= 0 && (uint32_t)p < n_tris) {
-        const float4 r1 = rays[2 * (size_t)i + 1];
-        V3 pv[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            vi[j] = tri_idx[3 * (size_t)p + j];
-            pv[j] = v3(verts[3 * (size_t)vi[j]], verts[3 * (size_t)vi[j] + 1], verts[3 * (size_t)vi[j] + 2]);
-        }
-        nrm = cross(sub(pv[1], pv[0]), sub(pv[2], pv[0]));
-        const float nd = dot(nrm, v3(r1.x, r1.y, r1.z));
-        if (nd != 0.0f) {
-            live = true;
-            s = g[i] / nd;
-            go = v3(-nrm.x * s, -nrm.y * s, -nrm.z * s);
-            const float ts = t[i] * s;
-            gd = v3(-nrm.x * ts, -nrm.y * ts, -nrm.z * ts);
-        }
-    }
-    if (g_o) {
-        g_o[3 * (size_t)i] = go.x;
-        g_o[3 * (size_t)i + 1] = go.y;
-        g_o[3 * (size_t)i + 2] = go.z;
-    }
-    if (g_d) {
-        g_d[3 * (size_t)i] = gd.x;
-        g_d[3 * (size_t)i + 1] = gd.y;
-        g_d[3 * (size_t)i + 2] = gd.z;
-    }
-    if (g_verts && live) {
-        const float u = uv[2 * (size_t)i], v = uv[2 * (size_t)i + 1];
-        const float bw[3] = {(1.0f - u) - v, u, v};
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float w = bw[j] * s;
-            if (w == 0.0f) continue;
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j]], nrm.x * w);
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j] + 1], nrm.y * w);
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j] + 2], nrm.z * w);
-        }
-    }
+    ray_bwd_one(i, rays, prim, t, uv, g, tri_idx, n_tris, verts, 3, g_o, g_d, g_verts);
 }
 
 int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim,
@@ -149,8 +53,7 @@ int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_
     if (n == 0) return GPURT_OK;
     hipLaunchKernelGGL(k_ray_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n,
                        reinterpret_cast<const float4*>(rays), prim, t, uv, g, b->tri_idx,
-                       (uint32_t)b->n_tris, b->verts,
-                       g_o, g_d, g_verts);
+                       (uint32_t)b->n_tris, b->verts, g_o, g_d, g_verts);
     GPURT_HIP(hipGetLastError());
     return GPURT_OK;
 }

@@ -14,6 +14,7 @@
 #include <numeric>
 #include <vector>
 
+#include "backward.h"
 #include "traverse.h"
 
 namespace gpurt_impl {
@@ -193,6 +194,58 @@ static int launch_batch(int W, bool q8, int S, bool rays, unsigned grid, hipStre
 
 static int block_for_stack_b(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 
+// ------------------------------------------------------------------------------------
+// batched backward: gradients of every mesh's distances and hit distances, from the
+// forward outputs the batch wrote (ids, barycentrics, points, distances, hit t, (u,v))
+// ------------------------------------------------------------------------------------
+
+struct BatchGradDev {
+    const float* verts;
+    uint64_t vstride;  // floats
+    const uint32_t* tri_idx;
+    uint32_t n_tris, n_verts;
+    const float* g_dist;
+    float* g_q;
+    const float* g_t;
+    float* g_origin;
+    float* g_dir;
+    float* g_verts;
+};
+
+constexpr uint32_t kZeroSeg = 4096;  // floats cleared per block of the zeroing kernel
+
+// Clears every requested vertex gradient: segment k = (mesh, first float) covers up to
+// kZeroSeg floats of that mesh's n_verts x 3 array.
+__global__ __launch_bounds__(256) void k_batch_zero_vgrads(const BatchGradDev* __restrict__ grads,
+                                                           const uint2* __restrict__ segs) {
+    const uint2 sg = segs[blockIdx.x];
+    const BatchGradDev& gd = grads[sg.x];
+    const uint32_t end = min(sg.y + kZeroSeg, 3u * gd.n_verts);
+    for (uint32_t i = sg.y + threadIdx.x; i < end; i += 256) gd.g_verts[i] = 0.0f;
+}
+
+// One wave per 64-query chunk of the forward chunk list.
+template <bool RAYS>
+__global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restrict__ meshes,
+                                                   const BatchGradDev* __restrict__ grads,
+                                                   const uint2* __restrict__ chunks, uint32_t n_chunks) {
+    const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (c >= n_chunks) return;
+    const uint2 ch = chunks[c];
+    const BatchMeshDev& md = meshes[ch.x];
+    const BatchGradDev& gd = grads[ch.x];
+    const uint32_t i = ch.y + (threadIdx.x & 63u);
+    if (RAYS) {
+        if (i >= md.n_r || !gd.g_t) return;
+        ray_bwd_one(i, md.rays, md.r_prim, md.r_t, md.r_uv, gd.g_t, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride,
+                    gd.g_origin, gd.g_dir, gd.g_verts);
+    } else {
+        if (i >= md.n_q || !gd.g_dist) return;
+        cpq_bwd_one(i, md.q, md.q_prim, md.q_bary, md.q_pt, md.q_dist, gd.g_dist, gd.tri_idx, gd.n_tris,
+                    gd.verts, gd.vstride, gd.g_q, gd.g_verts);
+    }
+}
+
 }  // namespace gpurt_impl
 
 using namespace gpurt_impl;
@@ -212,7 +265,24 @@ struct gpurt_batch_s {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // backward (gpurt_batch_set_grads): its own recorded graph
+    std::vector<gpurt_batch_mesh> host_meshes;
+    BatchGradDev* grads_d = nullptr;
+    uint2* zero_segs_d = nullptr;
+    hipGraph_t bgraph = nullptr;
+    hipGraphExec_t bexec = nullptr;
+    void drop_backward() {
+        if (bexec) (void)hipGraphExecDestroy(bexec);
+        if (bgraph) (void)hipGraphDestroy(bgraph);
+        if (grads_d) (void)hipFree(grads_d);
+        if (zero_segs_d) (void)hipFree(zero_segs_d);
+        bexec = nullptr;
+        bgraph = nullptr;
+        grads_d = nullptr;
+        zero_segs_d = nullptr;
+    }
     ~gpurt_batch_s() {
+        drop_backward();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
@@ -328,6 +398,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->leaf_size = leaf;
     b->stack = S;
     b->format = bp.format;
+    b->host_meshes.assign(meshes, meshes + n_meshes);
     int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {
         delete b;
@@ -423,6 +494,79 @@ int gpurt_batch_run(gpurt_batch b, void* stream) {
     DevGuard dg(b->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     GPURT_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return GPURT_OK;
+}
+
+int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
+    if (!b || !grads) return set_error(GPURT_E_INVALID, "null batch or gradient array");
+    const size_t n_meshes = b->host_meshes.size();
+    std::vector<BatchGradDev> gd(n_meshes);
+    std::vector<uint2> segs;
+    for (size_t m = 0; m < n_meshes; m++) {
+        const gpurt_batch_mesh& hm = b->host_meshes[m];
+        const gpurt_batch_grad& g = grads[m];
+        const std::string at = " (mesh " + std::to_string(m) + ")";
+        if (g.g_dist && hm.n_q && g.g_verts && !hm.q_bary)
+            return set_error(GPURT_E_INVALID, "vertex gradients of distances need the saved barycentrics" + at);
+        if (g.g_t && hm.n_r && g.g_verts && !hm.r_uv)
+            return set_error(GPURT_E_INVALID, "vertex gradients of hit t need the saved (u,v)" + at);
+        if (g.g_verts && 3 * (uint64_t)hm.n_verts >= 0xFFFFFFFFull)
+            return set_error(GPURT_E_INVALID, "too many vertices for a batched vertex gradient" + at);
+        gd[m] = BatchGradDev{hm.verts, hm.vstride_bytes / 4, hm.tri_idx, (uint32_t)hm.n_tris, (uint32_t)hm.n_verts,
+                             hm.n_q ? g.g_dist : nullptr, g.g_q, hm.n_r ? g.g_t : nullptr, g.g_origin, g.g_dir,
+                             g.g_verts};
+        if (g.g_verts)
+            for (uint64_t f = 0; f < 3 * (uint64_t)hm.n_verts; f += kZeroSeg) segs.push_back(make_uint2((uint32_t)m, (uint32_t)f));
+    }
+    DevGuard dg(b->ctx->device);
+    if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    b->drop_backward();
+    if (hipMalloc((void**)&b->grads_d, sizeof(BatchGradDev) * n_meshes) != hipSuccess ||
+        (!segs.empty() && hipMalloc((void**)&b->zero_segs_d, sizeof(uint2) * segs.size()) != hipSuccess)) {
+        (void)hipGetLastError();
+        b->drop_backward();
+        return set_error(GPURT_E_OOM, "batch gradient allocation failed");
+    }
+    if (hipMemcpy(b->grads_d, gd.data(), sizeof(BatchGradDev) * n_meshes, hipMemcpyHostToDevice) != hipSuccess ||
+        (!segs.empty() &&
+         hipMemcpy(b->zero_segs_d, segs.data(), sizeof(uint2) * segs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+        int rc = hip_error(hipGetLastError(), "batch gradient setup");
+        b->drop_backward();
+        return rc;
+    }
+    if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        int rc = hip_error(hipGetLastError(), "hipStreamBeginCapture");
+        b->drop_backward();
+        return rc;
+    }
+    if (!segs.empty())
+        hipLaunchKernelGGL(k_batch_zero_vgrads, dim3((unsigned)segs.size()), dim3(256), 0, b->cap, b->grads_d,
+                           b->zero_segs_d);
+    if (b->n_cpq_chunks)
+        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
+                           b->grads_d, b->chunks_d, b->n_cpq_chunks);
+    if (b->n_ray_chunks)
+        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
+                           b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks);
+    hipError_t el = hipGetLastError();
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->cap, &g);
+    b->bgraph = g;
+    int rc = GPURT_OK;
+    if (el != hipSuccess) rc = hip_error(el, "batch backward launch");
+    else if (ec != hipSuccess) rc = hip_error(ec, "hipStreamEndCapture");
+    else if (hipGraphInstantiate(&b->bexec, b->bgraph, nullptr, nullptr, 0) != hipSuccess)
+        rc = hip_error(hipGetLastError(), "hipGraphInstantiate");
+    if (rc) b->drop_backward();
+    return rc;
+}
+
+int gpurt_batch_run_backward(gpurt_batch b, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null batch");
+    if (!b->bexec) return set_error(GPURT_E_INVALID, "gpurt_batch_set_grads was not called");
+    DevGuard dg(b->ctx->device);
+    if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    GPURT_HIP(hipGraphLaunch(b->bexec, (hipStream_t)stream));
     return GPURT_OK;
 }
 

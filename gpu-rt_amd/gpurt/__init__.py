@@ -275,6 +275,7 @@ class Batch:
             bm.r_prim, bm.r_t = out["r_prim"].data_ptr(), out["r_t"].data_ptr()
             bm.r_uv = out["r_uv"].data_ptr() if with_uv else None
         self._arr = arr
+        self._meshes = meshes
         bo = BuildOpts(node_width, leaf_size, treelet_size, 0, node_format)
         qo = QueryOpts(stack_depth, waves_per_cu, None)
         torch.cuda.synchronize(dev)  # inputs must be resident before the first (eager) run
@@ -286,6 +287,43 @@ class Batch:
     def run(self):
         _lib.check(_lib.load().gpurt_batch_run(self._h, _stream(self.ctx.device)))
         return self.results
+
+    def enable_backward(self, query_grads=True, vertex_grads=True, distance=True, hit_t=True):
+        """Allocate gradient buffers and record the batched backward (gpurt_batch_set_grads).
+
+        grads[i] holds mesh i's inputs g_dist (n_q), g_t (n_r) -- upstream gradients of
+        q_dist and r_t, ones by default -- and outputs g_q (n_q,3), g_origin, g_dir
+        (n_r,3), g_verts (n_verts,3) (None where not requested).  backward() fills the
+        outputs from the last run()."""
+        dev = self.ctx.device
+        arr = (_lib.BatchGrad * len(self.results))()
+        self.grads = []
+        for i, (r, m) in enumerate(zip(self.results, self._meshes)):
+            nq, nr = r["q_prim"].shape[0], r["r_prim"].shape[0]
+            if vertex_grads and distance and nq and r["q_bary"] is None:
+                raise ValueError("vertex gradients of distances need with_bary=True")
+            if vertex_grads and hit_t and nr and r["r_uv"] is None:
+                raise ValueError("vertex gradients of hit t need with_uv=True")
+            g = {
+                "g_dist": torch.ones(nq, device=dev) if distance else None,
+                "g_q": torch.empty((nq, 3), device=dev) if (distance and query_grads) else None,
+                "g_t": torch.ones(nr, device=dev) if hit_t else None,
+                "g_origin": torch.empty((nr, 3), device=dev) if (hit_t and query_grads) else None,
+                "g_dir": torch.empty((nr, 3), device=dev) if (hit_t and query_grads) else None,
+                "g_verts": torch.empty((m["verts"].shape[0], 3), device=dev) if vertex_grads else None,
+            }
+            self.grads.append(g)
+            for k, v in g.items():
+                setattr(arr[i], k, v.data_ptr() if v is not None else None)
+        self._garr = arr
+        torch.cuda.synchronize(dev)
+        _lib.check(_lib.load().gpurt_batch_set_grads(self._h, arr))
+        return self.grads
+
+    def backward(self):
+        """Run the recorded backward graph (after run()); returns the gradient dicts."""
+        _lib.check(_lib.load().gpurt_batch_run_backward(self._h, _stream(self.ctx.device)))
+        return self.grads
 
     def close(self):
         if getattr(self, "_h", None):

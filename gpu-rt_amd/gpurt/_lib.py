@@ -29,7 +29,8 @@ EXPORTS = (
     "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_bvh_build", "gpurt_bvh_refit",
     "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_closest_point", "gpurt_trace_closest",
     "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd", "gpurt_batch_create",
-    "gpurt_batch_run", "gpurt_batch_destroy", "gpurt_last_error",
+    "gpurt_batch_run", "gpurt_batch_set_grads", "gpurt_batch_run_backward", "gpurt_batch_destroy",
+    "gpurt_last_error",
 )
 
 
@@ -70,6 +71,12 @@ class BatchMesh(ctypes.Structure):
                 ("r_uv", ctypes.c_void_p)]
 
 
+class BatchGrad(ctypes.Structure):
+    _fields_ = [("g_dist", ctypes.c_void_p), ("g_q", ctypes.c_void_p), ("g_t", ctypes.c_void_p),
+                ("g_origin", ctypes.c_void_p), ("g_dir", ctypes.c_void_p),
+                ("g_verts", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -98,6 +105,8 @@ def load():
         "gpurt_batch_create": (I, [P, ctypes.POINTER(BatchMesh), S, ctypes.POINTER(BuildOpts),
                                    ctypes.POINTER(QueryOpts), ctypes.POINTER(P)]),
         "gpurt_batch_run": (I, [P, P]),
+        "gpurt_batch_set_grads": (I, [P, ctypes.POINTER(BatchGrad)]),
+        "gpurt_batch_run_backward": (I, [P, P]),
         "gpurt_batch_destroy": (V, [P]),
         "gpurt_last_error": (ctypes.c_char_p, []),
     }

@@ -207,6 +207,28 @@ GPURT_API int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, 
                        gpurt_batch* out);
 /* Run the batch (graph launch on `stream`): rebuild every BVH and answer every query. */
 GPURT_API int gpurt_batch_run(gpurt_batch batch, void* stream);
+
+/* Gradient buffers of one batch mesh (all device pointers, each nullable). */
+typedef struct gpurt_batch_grad {
+    const float* g_dist;  /* n_q upstream gradients of q_dist (null: no distance term)   */
+    float* g_q;           /* n_q x 3 gradient w.r.t. the query points                    */
+    const float* g_t;     /* n_r upstream gradients of r_t (null: no hit-distance term)   */
+    float* g_origin;      /* n_r x 3 gradient w.r.t. the ray origins                      */
+    float* g_dir;         /* n_r x 3 gradient w.r.t. the ray directions                   */
+    float* g_verts;       /* n_verts x 3 gradient w.r.t. the vertex positions, overwritten
+                             with the sum of the distance and hit-distance terms          */
+} gpurt_batch_grad;
+
+/* Record the batched backward pass (one gpurt_batch_grad per mesh, in the order of
+ * gpurt_batch_create) into a second HIP graph: it clears the vertex gradients, then
+ * forms every mesh's gradients from the outputs of the last gpurt_batch_run (saved ids,
+ * barycentrics, points, distances, hit t, (u,v)) without re-traversal, scatter-adding
+ * vertex terms with FP32 atomics.  Vertex gradients need the forward's q_bary / r_uv.
+ * No counterpart in the reference (it has no gradients); it is the batched form of
+ * gpurt_closest_point_bwd + gpurt_trace_bwd.  Calling it again replaces the buffers. */
+GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* grads);
+/* Run the recorded backward graph on `stream` (after gpurt_batch_run on the same stream). */
+GPURT_API int gpurt_batch_run_backward(gpurt_batch batch, void* stream);
 GPURT_API void gpurt_batch_destroy(gpurt_batch batch);
 
 /* ---- errors ----------------------------------------------------------------------- */

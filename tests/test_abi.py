@@ -45,7 +45,8 @@ def _c_layout(struct, fields):
 @pytest.mark.parametrize("cls,struct", [(_lib.BuildOpts, "gpurt_build_opts"),
                                         (_lib.QueryOpts, "gpurt_query_opts"),
                                         (_lib.BvhInfo, "gpurt_bvh_info"),
-                                        (_lib.BatchMesh, "gpurt_batch_mesh")])
+                                        (_lib.BatchMesh, "gpurt_batch_mesh"),
+                                        (_lib.BatchGrad, "gpurt_batch_grad")])
 def test_struct_layouts_match_header(cls, struct):
     fields = [f for f, _ in cls._fields_]
     size, offs = _c_layout(struct, fields)

@@ -121,3 +121,61 @@ def test_batch_with_treelets_matches_oracle(ctx, dev):
         rp, rt, _ = O.ray_closest_brute_f32(h["verts"], h["tris"], h["rays"])
         assert np.array_equal(r["r_prim"].cpu().numpy(), rp)
         assert np.array_equal(r["r_t"].cpu().numpy(), rt)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_batch_backward_matches_plain_backward(ctx, dev, fmt):
+    """The recorded batched backward gives the plain per-mesh backward's gradients:
+    query/origin/direction gradients bit for bit (same per-query arithmetic), vertex
+    gradients up to the order of the FP32 atomic sums; a second replay clears and
+    recomputes the vertex gradients."""
+    host = _small_meshes(seed=11)
+    dm = _dev(host, dev)
+    batch = gpurt.Batch(ctx, dm, node_format=fmt)
+    res = batch.run()
+    grads = batch.enable_backward()
+    gen = torch.Generator(device=dev).manual_seed(3)
+    for g in grads:
+        g["g_dist"].uniform_(0.5, 2.0, generator=gen)
+        g["g_t"].uniform_(0.5, 2.0, generator=gen)
+    for rep in range(2):
+        batch.run()
+        batch.backward()
+        torch.cuda.synchronize()
+        for i, (m, r, g) in enumerate(zip(dm, res, grads)):
+            bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_format=fmt)
+            prim, pt, dist, bary = bvh.closest_point(m["queries"])
+            rp, rt, ruv = bvh.trace_closest(m["rays"])
+            assert torch.equal(prim, r["q_prim"]) and torch.equal(rp, r["r_prim"])
+            gq = torch.empty_like(m["queries"])
+            go = torch.empty((rp.shape[0], 3), device=dev)
+            gd = torch.empty((rp.shape[0], 3), device=dev)
+            gv = torch.zeros((m["verts"].shape[0], 3), device=dev)
+            bvh.closest_point_bwd(m["queries"], prim, bary, pt, dist, g["g_dist"], gq, gv)
+            bvh.trace_bwd(m["rays"], rp, rt, ruv, g["g_t"], go, gd, gv)
+            torch.cuda.synchronize()
+            assert torch.equal(gq, g["g_q"]), (rep, i)
+            assert torch.equal(go, g["g_origin"]) and torch.equal(gd, g["g_dir"]), (rep, i)
+            scale = float(gv.abs().max()) + 1e-6
+            assert float((gv - g["g_verts"]).abs().max()) <= 1e-5 * scale, (rep, i)
+            # FP64 closed form of the distance term at the batch's own argmin
+            V = m["verts"].cpu().numpy().astype(np.float64)
+            F = m["tris"].cpu().numpy()
+            Q = m["queries"].cpu().numpy().astype(np.float64)
+            eq, _ = O.cpq_grad_f64(V, F, Q, r["q_prim"].cpu().numpy(), r["q_pt"].cpu().numpy().astype(np.float64),
+                                   r["q_bary"].cpu().numpy().astype(np.float64),
+                                   g["g_dist"].cpu().numpy().astype(np.float64))
+            ok = r["q_dist"].cpu().numpy() > 1e-3
+            np.testing.assert_allclose(g["g_q"].cpu().numpy()[ok], eq[ok], atol=1e-3, rtol=1e-3)
+
+
+def test_batch_backward_needs_saved_barycentrics(ctx, dev):
+    host = _small_meshes(seed=2)[:3]
+    batch = gpurt.Batch(ctx, _dev(host, dev), with_bary=False)
+    with pytest.raises(ValueError):
+        batch.enable_backward()
+    g = batch.enable_backward(vertex_grads=False)
+    batch.run()
+    batch.backward()
+    torch.cuda.synchronize()
+    assert g[0]["g_verts"] is None and g[0]["g_q"].shape[1] == 3
