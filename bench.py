@@ -76,6 +76,33 @@ def measured_traffic(args, n_tris, n_rays, node_format):
     return (rd + wr) * 1024.0
 
 
+def fp64_rays(O, V, F, R, prim, t, every):
+    """FP32 closest hits against the oracle's FP64 brute force on every `every`-th ray:
+    id agreement, and the relative error of t where the ids agree."""
+    idx = np.arange(0, R.shape[0], every)
+    ep, et, _ = O.ray_closest_brute_f64(V.astype(np.float64), F, R[idx].astype(np.float64))
+    p, tt = prim[idx], t[idx].astype(np.float64)
+    same = (p == ep)
+    hit = same & (ep >= 0)
+    rel = np.abs(tt[hit] - et[hit]) / np.maximum(np.abs(et[hit]), 1e-30)
+    return {"sample": int(idx.size), "id_agreement": float(same.mean()),
+            "t_max_rel_err": float(rel.max()) if rel.size else 0.0,
+            "t_p999_rel_err": float(np.quantile(rel, 0.999)) if rel.size else 0.0}
+
+
+def fp64_cpq(O, V, F, Q, prim, dist, every):
+    """FP32 closest points against the oracle's FP64 brute force on every `every`-th
+    query: id agreement, and the relative error of the distance where the ids agree."""
+    idx = np.arange(0, Q.shape[0], every)
+    ep, _, ed, _ = O.cpq_brute_f64(V.astype(np.float64), F, Q[idx].astype(np.float64))
+    p, d = prim[idx], dist[idx].astype(np.float64)
+    same = p == ep
+    rel = np.abs(d[same] - ed[same]) / np.maximum(ed[same], 1e-30)
+    return {"sample": int(idx.size), "id_agreement": float(same.mean()),
+            "dist_max_rel_err": float(rel.max()) if rel.size else 0.0,
+            "dist_p999_rel_err": float(np.quantile(rel, 0.999)) if rel.size else 0.0}
+
+
 def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     t0 = time.time()
     w = W.rays_scene()
@@ -126,6 +153,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     eo = cbvh.trace_any(w["rays_any"])
     parity = bool(np.array_equal(prim.cpu().numpy(), ep) and np.array_equal(t.cpu().numpy(), et)
                   and np.array_equal(uv.cpu().numpy(), euv) and np.array_equal(occ.cpu().numpy(), eo))
+    fp64 = fp64_rays(O, V, F, w["rays"], prim.cpu().numpy(), t.cpu().numpy(), 1024)
 
     # CPU baseline: the oracle LBVH on every host core, full passes over the same rays
     # (closest hit + any hit) repeated until about --cpu-seconds of CPU work
@@ -171,13 +199,13 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
                        "hit_rate": hit_rate, "occluded_rate": occ_rate,
                        "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
-                       "bvh": info, "parity_all_rays_vs_cpu_lbvh": parity},
+                       "bvh": info, "parity_all_rays_vs_cpu_lbvh": parity, "fp64_check": fp64},
     }
     del bvh
     return res
 
 
-def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
+def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0):
     V, F, Q = w["verts"], w["tris"], w["queries"]
     tv = torch.from_numpy(V).to(dev)
     tf = torch.from_numpy(F).to(dev)
@@ -197,10 +225,11 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx):
     nodes, tris, _, _ = st.cpu().tolist()
     byts = n * (12 + 20 + 12) + nodes * bvh.info()["cpq_node_bytes"] + tris * TRI_BYTES
     del bvh
+    fp64 = fp64_cpq(O, V, F, Q, out[0].cpu().numpy(), out[2].cpu().numpy(), fp64_every) if fp64_every else None
     return {"workload": name, "n_tris": int(F.shape[0]), "n_queries": n,
             "queries_per_s": n / (ms * 1e-3), "ms": ms, "gpu_build_ms": build_ms,
             "nodes_per_query": nodes / n, "tris_per_query": tris / n,
-            "algorithmic_GBps": byts / (ms * 1e-3) / 1e9}
+            "algorithmic_GBps": byts / (ms * 1e-3) / 1e9, "fp64_check": fp64}
 
 
 def diff_bench(args, torch, gpurt, W, dev, ctx):
@@ -403,7 +432,7 @@ def main():
     extra = {"rays": r.pop("extra_rays")}
     if not args.no_extra:
         log("[bench] cpq_small")
-        extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx)
+        extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx, O, 1)
         log("[bench] diff fwd / fwd+bwd")
         extra["diff"] = diff_bench(args, torch, gpurt, W, dev, ctx)
         if not args.no_batched:
@@ -411,7 +440,7 @@ def main():
             extra["batched"] = batched_bench(args, torch, gpurt, W, dev, ctx)
         if args.large:
             log("[bench] cpq_large")
-            extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx)
+            extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx, O, 1024)
     line = {
         "metric": METRIC,
         "value": r["value"],
