@@ -126,20 +126,20 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     occ = torch.empty(n, dtype=torch.uint8, device=dev)
 
     def step():
-        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves)
-        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves)
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
 
     ms = timed(step, args.steps, args.warmup)
-    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves),
+    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger),
                        args.steps, 1)
-    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves), args.steps, 1)
+    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger), args.steps, 1)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
-    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
+    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
     torch.cuda.synchronize()
     nodes, tris, spills, _ = st.cpu().tolist()
     bytes_closest = n * (32 + 16) + nodes * info["node_bytes"] + tris * TRI_BYTES
     st.zero_()
-    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
+    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
     torch.cuda.synchronize()
     nodes_a, tris_a, spills_a, _ = st.cpu().tolist()
     hit_rate = float((prim >= 0).float().mean().item())
@@ -187,7 +187,8 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
                    "passes": ["closest_hit", "any_hit"], "node_width": args.width,
                    "leaf_size": args.leaf, "stack_depth": args.stack, "treelet_size": args.treelet,
-                   "node_format": info["node_format"]},
+                   "node_format": info["node_format"], "waves_per_cu": args.waves,
+                   "leaf_trigger": args.leaf_trigger},
         "roofline": roof,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
                          "sample": f"{passes} full passes of the {n} closest-hit + {n} any-hit "
@@ -218,9 +219,9 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0):
     n = q.shape[0]
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
            torch.empty(n, device=dev), torch.empty((n, 3), device=dev))
-    ms = timed(lambda: bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves), args.steps, 2)
+    ms = timed(lambda: bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger), args.steps, 2)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
-    bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves, stats=st)
+    bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
     torch.cuda.synchronize()
     nodes, tris, _, _ = st.cpu().tolist()
     byts = n * (12 + 20 + 12) + nodes * bvh.info()["cpq_node_bytes"] + tris * TRI_BYTES
@@ -281,7 +282,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
-                        waves_per_cu=args.waves, leaf_size=args.leaf,
+                        waves_per_cu=args.waves, leaf_size=args.leaf, leaf_trigger=args.leaf_trigger,
                         treelet_size=args.batch_treelet, node_format=args.format)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
@@ -293,8 +294,8 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
         for i, m in enumerate(meshes):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
                             treelet_size=args.batch_treelet, node_format=args.format)
-            p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves)
-            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves)
+            p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
             plain_out[i] = (p, d, rp, rt)
             bvh.close()
 
@@ -319,8 +320,8 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
         for i, m in enumerate(meshes):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
                             treelet_size=args.batch_treelet, node_format=args.format)
-            p, pt, d, b = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves)
-            rp, rt, ruv = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves)
+            p, pt, d, b = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+            rp, rt, ruv = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
             g = grads[i]
             gq = torch.empty_like(m["queries"])
             go = torch.empty((rp.shape[0], 3), device=dev)
@@ -369,7 +370,7 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
            torch.empty((n, 2), device=dev))
     for _ in range(args.profile_reps):
-        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves)
+        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(args.profile_reps):
@@ -390,6 +391,9 @@ def main():
     ap.add_argument("--leaf", type=int, default=4, choices=range(1, 9), metavar="1..8",
                     help="max triangles per leaf")
     ap.add_argument("--waves", type=int, default=0, help="resident waves per CU (0 = library choice)")
+    ap.add_argument("--leaf-trigger", type=int, default=0,
+                    help="lanes (of 64) with a postponed leaf that switch a wave to leaf tests "
+                         "(0 = library default)")
     ap.add_argument("--treelet", type=int, default=7, choices=[0, 5, 6, 7],
                     help="SAH treelet restructuring of the single-mesh BVHs (leaves per "
                          "treelet), 0 = off; the batched leg rebuilds every step and uses "

@@ -244,6 +244,8 @@ static int check_opts(const gpurt_query_opts* o) {
         return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 8, 16, 32 or 64");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 64)
         return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
+    if (o->leaf_trigger < 0 || o->leaf_trigger > 64)
+        return set_error(GPURT_E_UNSUPPORTED, "leaf_trigger must be 0..64");
     return GPURT_OK;
 }
 

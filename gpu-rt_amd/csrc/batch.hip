@@ -112,6 +112,7 @@ struct BatchArgs {
     uint2* spill;
     uint32_t spill_slots;
     uint32_t* err;
+    uint32_t leaf_trigger;
 };
 
 template <int W, bool Q, int S, int BS>
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
         }
     };
     unsigned long long dn = 0, dt = 0;
-    if (!cpq_stream<NodeOf<W, Q>, S, BS, false>(nodes, a.tris, src, st, out, dn, dt)) gatomic_or(a.err, 1u);
+    if (!cpq_stream<NodeOf<W, Q>, S, BS, false>(nodes, a.tris, src, a.leaf_trigger, st, out, dn, dt)) gatomic_or(a.err, 1u);
 }
 
 template <int W, bool Q, int S, int BS>
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
         }
     };
     unsigned long long dn = 0, dt = 0;
-    if (!ray_stream<NodeOf<W, Q>, S, BS, false, false>(nodes, a.tris, src, st, out, dn, dt)) gatomic_or(a.err, 1u);
+    if (!ray_stream<NodeOf<W, Q>, S, BS, false, false>(nodes, a.tris, src, a.leaf_trigger, st, out, dn, dt)) gatomic_or(a.err, 1u);
 }
 
 template <int W, bool Q, int S, int BS>
@@ -253,6 +254,7 @@ using namespace gpurt_impl;
 struct gpurt_batch_s {
     gpurt_ctx ctx = nullptr;
     int width = 4, leaf_size = 4, stack = 8, block = 256, format = kFormatF32;
+    uint32_t leaf_trigger = 32;
     BuildBufs build;
     void* qnodes = nullptr;  // QNode<W>[build.capacity] when has_qnodes(format)
     BatchMeshDev* meshes_d = nullptr;
@@ -324,6 +326,7 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
     a.spill = b->spill;
     a.spill_slots = b->spill_slots;
     a.err = b->ctx->err;
+    a.leaf_trigger = b->leaf_trigger;
     for (int rays = 0; rays < 2; rays++) {
         const uint32_t n = rays ? b->n_ray_chunks : b->n_cpq_chunks;
         if (n == 0) continue;
@@ -359,6 +362,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
         return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 8, 16, 32 or 64");
     if (qopts && (qopts->waves_per_cu < 0 || qopts->waves_per_cu > 64))
         return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
+    if (qopts && (qopts->leaf_trigger < 0 || qopts->leaf_trigger > 64))
+        return set_error(GPURT_E_UNSUPPORTED, "leaf_trigger must be 0..64");
 
     std::vector<SegDesc> segs(n_meshes);
     std::vector<BatchMeshDev> md(n_meshes);
@@ -398,6 +403,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->leaf_size = leaf;
     b->stack = S;
     b->format = bp.format;
+    if (qopts && qopts->leaf_trigger) b->leaf_trigger = (uint32_t)qopts->leaf_trigger;
     b->host_meshes.assign(meshes, meshes + n_meshes);
     int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {

@@ -207,7 +207,7 @@ constexpr float kTSlack = 0x1p-20f;             // relative slack on ray box exi
 // steps or leaf steps, never both (the two paths are long and would otherwise both be
 // issued whenever lanes disagree).  A lane that reaches a leaf parks it -- one parked
 // leaf per lane -- and keeps visiting nodes from its stack meanwhile; a lane that
-// reaches a second leaf waits.  The wave runs leaf steps when at least half of the
+// reaches a second leaf waits.  The wave runs leaf steps when a share (trig/64) of the
 // lanes with a query have parked a leaf, or when no lane has a node to visit
 // (postponed leaves with speculative traversal: Aila & Laine, "Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009).  Extra visits cannot change the
@@ -228,16 +228,14 @@ constexpr float kTSlack = 0x1p-20f;             // relative slack on ray box exi
 //   V3 point(const Handle&)                  (closest point) the query point
 // ------------------------------------------------------------------------------------
 
-constexpr uint32_t kLeafTrigger = 32;  // parked-leaf share (of 64 lanes) for a leaf step
-
 // Wave-uniform choice of the step kind: leaf steps when some lane has a parked leaf and
-// either no lane has a node to visit or at least kLeafTrigger/64 of the lanes with a
-// query have parked a leaf.
-__device__ __forceinline__ bool leaf_step_now(bool parked, bool node_ready) {
+// either no lane has a node to visit or at least trig/64 of the lanes with a query have
+// parked a leaf (trig = 1..64).
+__device__ __forceinline__ bool leaf_step_now(bool parked, bool node_ready, uint32_t trig) {
     const uint64_t pk = __ballot(parked);
     if (pk == 0) return false;
     if (__ballot(node_ready) == 0) return true;
-    return (uint32_t)__popcll(pk) * 64u >= (uint32_t)__popcll(__ballot(true)) * kLeafTrigger;
+    return (uint32_t)__popcll(pk) * 64u >= (uint32_t)__popcll(__ballot(true)) * trig;
 }
 
 // Plain source: lane g takes queries g, g + stride, g + 2 stride, ... of one array.
@@ -334,7 +332,7 @@ struct QueueSource {
 
 template <typename NodeT, int S, int BS, bool ANY, bool STATS, typename Src, typename Out>
 __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
-                                           Src& src, LaneStack<S, BS>& st, Out out,
+                                           Src& src, uint32_t trig, LaneStack<S, BS>& st, Out out,
                                            unsigned long long& n_nodes, unsigned long long& n_tris) {
     constexpr int W = NodeT::kWidth;
     bool ok = true;
@@ -395,7 +393,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
             have = false;
         }
         const bool node_ready = have && !child_is_leaf(cur);
-        if (leaf_step_now(pleaf != kEmpty, node_ready)) {
+        if (leaf_step_now(pleaf != kEmpty, node_ready, trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {
@@ -470,7 +468,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
 
 template <typename NodeT, int S, int BS, bool STATS, typename Src, typename Out>
 __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
-                                           Src& src, LaneStack<S, BS>& st, Out out,
+                                           Src& src, uint32_t trig, LaneStack<S, BS>& st, Out out,
                                            unsigned long long& n_nodes, unsigned long long& n_tris) {
     constexpr int W = NodeT::kWidth;
     bool ok = true;
@@ -518,7 +516,7 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
             have = false;
         }
         const bool node_ready = have && !child_is_leaf(cur);
-        if (leaf_step_now(pleaf != kEmpty, node_ready)) {
+        if (leaf_step_now(pleaf != kEmpty, node_ready, trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {

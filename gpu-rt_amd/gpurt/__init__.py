@@ -145,7 +145,7 @@ class Bvh:
         _lib.check(_lib.load().gpurt_bvh_refit(self._h, _ptr(verts), verts.shape[1] * 4,
                                                _stream(self.device)))
 
-    def _qopts(self, stack_depth, waves_per_cu, stats):
+    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger=0):
         """stats: optional int64 device tensor of 4 counters (wide nodes visited,
         triangles tested, stack spills, reserved), accumulated by the counting kernel
         variant."""
@@ -155,10 +155,11 @@ class Bvh:
                 raise ValueError("stats must be a contiguous int64 tensor of >= 4 elements "
                                  "on the BVH's device")
         return QueryOpts(stack_depth, waves_per_cu,
-                         ctypes.c_void_p(stats.data_ptr()) if stats is not None else None)
+                         ctypes.c_void_p(stats.data_ptr()) if stats is not None else None,
+                         leaf_trigger, 0)
 
     def closest_point(self, q, with_bary=True, stack_depth=0, waves_per_cu=0, stats=None,
-                      out=None):
+                      out=None, leaf_trigger=0):
         dev = self.device
         _need(q, torch.float32, (3,), "q", dev)
         n = q.shape[0]
@@ -169,14 +170,14 @@ class Bvh:
             bary = torch.empty((n, 3), dtype=torch.float32, device=dev) if with_bary else None
         else:
             prim, pt, dist, bary = out
-        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
         _lib.check(_lib.load().gpurt_closest_point(self._h, _ptr(q), n, _ptr(prim), _ptr(pt),
                                                    _ptr(dist), _ptr(bary), ctypes.byref(opts),
                                                    _stream(dev)))
         return prim, pt, dist, bary
 
     def trace_closest(self, rays, with_uv=True, stack_depth=0, waves_per_cu=0, stats=None,
-                      out=None):
+                      out=None, leaf_trigger=0):
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
@@ -186,17 +187,17 @@ class Bvh:
             uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
         else:
             prim, t, uv = out
-        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
         _lib.check(_lib.load().gpurt_trace_closest(self._h, _ptr(rays), n, _ptr(prim), _ptr(t),
                                                    _ptr(uv), ctypes.byref(opts), _stream(dev)))
         return prim, t, uv
 
-    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None):
+    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0):
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
         occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
-        opts = self._qopts(stack_depth, waves_per_cu, stats)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
         _lib.check(_lib.load().gpurt_trace_any(self._h, _ptr(rays), n, _ptr(occ),
                                                ctypes.byref(opts), _stream(dev)))
         return occ
@@ -234,7 +235,7 @@ class Batch:
     """
 
     def __init__(self, ctx, meshes, node_width=4, leaf_size=4, treelet_size=0, stack_depth=0,
-                 waves_per_cu=0, with_bary=True, with_uv=True, node_format=0):
+                 waves_per_cu=0, with_bary=True, with_uv=True, node_format=0, leaf_trigger=0):
         lib = _lib.load()
         self.ctx = ctx
         dev = ctx.device
@@ -277,7 +278,7 @@ class Batch:
         self._arr = arr
         self._meshes = meshes
         bo = BuildOpts(node_width, leaf_size, treelet_size, 0, node_format)
-        qo = QueryOpts(stack_depth, waves_per_cu, None)
+        qo = QueryOpts(stack_depth, waves_per_cu, None, leaf_trigger, 0)
         torch.cuda.synchronize(dev)  # inputs must be resident before the first (eager) run
         h = ctypes.c_void_p()
         _lib.check(lib.gpurt_batch_create(ctx.handle, arr, len(meshes), ctypes.byref(bo),

@@ -88,6 +88,10 @@ typedef struct gpurt_query_opts {
     uint64_t* stats;      /* optional DEVICE uint64[4], accumulated: wide nodes visited,
                              triangles tested, stack entries spilled past the LDS
                              part, reserved (slower counting kernel variant)          */
+    int32_t leaf_trigger; /* lanes (of a 64-lane wave) holding a postponed leaf at which
+                             the wave switches to leaf tests, 1..64 (0 = library
+                             default); changes speed only                             */
+    int32_t reserved;
 } gpurt_query_opts;
 
 typedef struct gpurt_bvh_info {
