@@ -221,6 +221,25 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width
 void launch_quantize(int width, const void* nodes, void* qnodes, uint32_t n_host, uint32_t capacity,
                      const uint32_t* n_dev, hipStream_t s);
 
+// traversal launch planning (query.hip): stack depth, block size, grid, spill slots and
+// leaf trigger for one launch of a query kind, and the per-call scratch (queue counters
+// cleared on the stream + stack spill buffer)
+enum class QKind { Cpq, Closest, Any };
+struct TravPlan {
+    int S = 8, BS = 256;
+    unsigned grid = 1;
+    uint32_t total = 256, spill_slots = 0, leaf_trigger = 32;
+};
+struct TravScratch {
+    void* mem = nullptr;
+    uint32_t* counters = nullptr;
+    uint2* spill = nullptr;
+};
+int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
+                   size_t n, TravPlan& p);
+int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t);
+void free_traversal_scratch(TravScratch& t, hipStream_t s);
+
 // traversal launchers (query.hip)
 int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,
                float* bary, const gpurt_query_opts* opts, hipStream_t s);

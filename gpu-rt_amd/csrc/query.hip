@@ -106,8 +106,6 @@ __global__ void k_clear_counters(uint32_t* c) {
 // launch plumbing
 // ------------------------------------------------------------------------------------
 
-enum class QKind { Cpq, Closest, Any };
-
 template <int W, bool Q, int S, int BS, bool STATS>
 static void launch_kind(QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (kind) {
@@ -145,82 +143,96 @@ static int default_waves_per_cu(QKind kind) { return kind == QKind::Any ? 24 : 0
 
 static int block_for_stack(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 
-static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpurt_query_opts* opts,
-                     hipStream_t s) {
-    if (n == 0) return GPURT_OK;
+int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
+                   size_t n, TravPlan& p) {
     if (n > 0xFFFFFFFFull - 1024u) return set_error(GPURT_E_INVALID, "too many queries in one call");
     // default: the smallest LDS stack (most resident waves); 8-wide nodes push more
     // entries per visit, so they default to 16
-    int S = opts && opts->stack_depth ? opts->stack_depth : (b->width == 8 ? 16 : 8);
-    if (S != 8 && S != 16 && S != 32 && S != 64)
+    p.S = opts && opts->stack_depth ? opts->stack_depth : (width == 8 ? 16 : 8);
+    if (p.S != 8 && p.S != 16 && p.S != 32 && p.S != 64)
         return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 8, 16, 32 or 64");
-    const int BS = block_for_stack(S);
-    gpurt_ctx ctx = b->ctx;
+    p.BS = block_for_stack(p.S);
     // resident blocks per CU: limited by LDS (the stacks), by the 32 wave slots of a CU
     // and by registers (WavesHint waves per SIMD), unless the caller asks for waves/CU
-    const int waves_per_block = BS / 64;
-    const bool q8 = kind == QKind::Cpq ? cpq_use_q8(b->format) : rays_use_q8(b->format);
-    const int hint = b->width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
-                                   : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
-    const int reg_waves = 4 * hint;
-    int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8), std::min(32, reg_waves) / waves_per_block);
+    const int waves_per_block = p.BS / 64;
+    const int hint = width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
+                                : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
+    int blocks_per_cu = std::min(ctx->lds_per_cu / (p.S * p.BS * 8), std::min(32, 4 * hint) / waves_per_block);
     if (opts && opts->waves_per_cu > 0)
         blocks_per_cu = std::max(1, opts->waves_per_cu / waves_per_block);
     else if (default_waves_per_cu(kind) > 0)
         blocks_per_cu = std::min(blocks_per_cu, std::max(1, default_waves_per_cu(kind) / waves_per_block));
     blocks_per_cu = std::max(blocks_per_cu, 1);
-    unsigned grid = (unsigned)std::min<size_t>(cdiv(n, BS), (size_t)ctx->n_cu * blocks_per_cu);
-    if (grid == 0) grid = 1;
-    const uint32_t total = grid * BS;
-    // worst case stack: (W-1) entries per level above the last plus W for the last; the
-    // part beyond the LDS slots spills to a stream-ordered allocation owned by this call
-    const int need = (b->width - 1) * b->depth + 1;
-    uint32_t spill_slots = need > S ? (uint32_t)(need - S) : 0u;
-    // one stream-ordered allocation owned by this call: the queue counters and the
-    // stack entries beyond the LDS slots
+    p.grid = (unsigned)std::min<size_t>(cdiv(n, p.BS), (size_t)ctx->n_cu * blocks_per_cu);
+    if (p.grid == 0) p.grid = 1;
+    p.total = p.grid * p.BS;
+    // the part of the worst-case stack beyond the LDS slots spills to device memory
+    p.spill_slots = stack_need > p.S ? (uint32_t)(stack_need - p.S) : 0u;
+    p.leaf_trigger = opts && opts->leaf_trigger ? (uint32_t)opts->leaf_trigger : default_leaf_trigger(kind);
+    return GPURT_OK;
+}
+
+int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t) {
+    // one stream-ordered allocation owned by the call: the queue counters and the stack
+    // entries beyond the LDS slots
     constexpr size_t kCounterBytes = (size_t)kQueues * kQueueStride * sizeof(uint32_t);
-    void* scratch = nullptr;
-    const size_t bytes = kCounterBytes + (size_t)spill_slots * total * sizeof(uint2);
-    if (hipMallocAsync(&scratch, bytes, s) != hipSuccess) {
+    const size_t bytes = kCounterBytes + (size_t)p.spill_slots * p.total * sizeof(uint2);
+    if (hipMallocAsync(&t.mem, bytes, s) != hipSuccess) {
         (void)hipGetLastError();
+        t.mem = nullptr;
         return set_error(GPURT_E_OOM, "traversal scratch allocation failed");
     }
-    uint32_t* counter = (uint32_t*)scratch;
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, counter);
-    void* spill = spill_slots ? (void*)((char*)scratch + kCounterBytes) : nullptr;
+    t.counters = (uint32_t*)t.mem;
+    t.spill = p.spill_slots ? (uint2*)((char*)t.mem + kCounterBytes) : nullptr;
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, t.counters);
+    return GPURT_OK;
+}
+
+void free_traversal_scratch(TravScratch& t, hipStream_t s) {
+    if (t.mem) (void)hipFreeAsync(t.mem, s);
+    t.mem = nullptr;
+}
+
+static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpurt_query_opts* opts,
+                     hipStream_t s) {
+    if (n == 0) return GPURT_OK;
+    const bool q8 = kind == QKind::Cpq ? cpq_use_q8(b->format) : rays_use_q8(b->format);
+    // worst case stack: (W-1) entries per level above the last plus W for the last
+    TravPlan p;
+    int rc = plan_traversal(b->ctx, b->width, q8, kind, (b->width - 1) * b->depth + 1, opts, n, p);
+    if (rc) return rc;
+    TravScratch t;
+    if ((rc = alloc_traversal_scratch(p, s, t))) return rc;
     a.tris = b->tris;
     a.root = 0;
     a.n = (uint32_t)n;
-    a.spill = (uint2*)spill;
-    a.spill_slots = spill_slots;
-    a.counter = counter;
+    a.spill = t.spill;
+    a.spill_slots = p.spill_slots;
+    a.counter = t.counters;
     a.stats = opts ? (unsigned long long*)opts->stats : nullptr;
-    a.err = ctx->err;
-    a.leaf_trigger = opts && opts->leaf_trigger ? (uint32_t)opts->leaf_trigger : default_leaf_trigger(kind);
+    a.err = b->ctx->err;
+    a.leaf_trigger = p.leaf_trigger;
     const bool stats = a.stats != nullptr;
     int bs = 0;
     a.nodes = q8 ? b->qnodes : b->nodes;
     switch (b->width * 4 + (q8 ? 2 : 0) + (stats ? 1 : 0)) {
-        case 8: bs = dispatch_stack<2, false, false>(S, kind, a, grid, s); break;
-        case 9: bs = dispatch_stack<2, false, true>(S, kind, a, grid, s); break;
-        case 10: bs = dispatch_stack<2, true, false>(S, kind, a, grid, s); break;
-        case 11: bs = dispatch_stack<2, true, true>(S, kind, a, grid, s); break;
-        case 16: bs = dispatch_stack<4, false, false>(S, kind, a, grid, s); break;
-        case 17: bs = dispatch_stack<4, false, true>(S, kind, a, grid, s); break;
-        case 18: bs = dispatch_stack<4, true, false>(S, kind, a, grid, s); break;
-        case 19: bs = dispatch_stack<4, true, true>(S, kind, a, grid, s); break;
-        case 32: bs = dispatch_stack<8, false, false>(S, kind, a, grid, s); break;
-        case 33: bs = dispatch_stack<8, false, true>(S, kind, a, grid, s); break;
-        case 34: bs = dispatch_stack<8, true, false>(S, kind, a, grid, s); break;
-        case 35: bs = dispatch_stack<8, true, true>(S, kind, a, grid, s); break;
+        case 8: bs = dispatch_stack<2, false, false>(p.S, kind, a, p.grid, s); break;
+        case 9: bs = dispatch_stack<2, false, true>(p.S, kind, a, p.grid, s); break;
+        case 10: bs = dispatch_stack<2, true, false>(p.S, kind, a, p.grid, s); break;
+        case 11: bs = dispatch_stack<2, true, true>(p.S, kind, a, p.grid, s); break;
+        case 16: bs = dispatch_stack<4, false, false>(p.S, kind, a, p.grid, s); break;
+        case 17: bs = dispatch_stack<4, false, true>(p.S, kind, a, p.grid, s); break;
+        case 18: bs = dispatch_stack<4, true, false>(p.S, kind, a, p.grid, s); break;
+        case 19: bs = dispatch_stack<4, true, true>(p.S, kind, a, p.grid, s); break;
+        case 32: bs = dispatch_stack<8, false, false>(p.S, kind, a, p.grid, s); break;
+        case 33: bs = dispatch_stack<8, false, true>(p.S, kind, a, p.grid, s); break;
+        case 34: bs = dispatch_stack<8, true, false>(p.S, kind, a, p.grid, s); break;
+        case 35: bs = dispatch_stack<8, true, true>(p.S, kind, a, p.grid, s); break;
         default: break;
     }
-    if (bs == 0) {
-        (void)hipFreeAsync(scratch, s);
-        return set_error(GPURT_E_UNSUPPORTED, "node width");
-    }
     hipError_t le = hipGetLastError();
-    (void)hipFreeAsync(scratch, s);
+    free_traversal_scratch(t, s);
+    if (bs == 0) return set_error(GPURT_E_UNSUPPORTED, "node width");
     if (le != hipSuccess) return hip_error(le, "traversal launch");
     return GPURT_OK;
 }

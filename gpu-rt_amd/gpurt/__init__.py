@@ -338,6 +338,94 @@ class Batch:
             pass
 
 
+class Tlas:
+    """Two-level acceleration structure over instances of BLASes (gpurt_tlas_build);
+    mirrors VK::Accel(blas, T) (src/vk/vulkan.cpp:777-856).
+
+    instances: list of (bvh, transform) or (bvh, transform, custom_index[, mask]) with
+    transform a row-major 3x4 (or 4x4, last row ignored) object-to-world matrix; the
+    custom index defaults to the instance's position (as Accel::recreate sets it) and
+    the mask to 0xFF.  All BVHs need the same node width and ray node format and must
+    stay alive (this object keeps references)."""
+
+    def __init__(self, ctx, instances, treelet_size=0):
+        lib = _lib.load()
+        self.ctx = ctx
+        arr = (_lib.Instance * len(instances))()
+        self._blas = []
+        for i, inst in enumerate(instances):
+            bvh, xf = inst[0], inst[1]
+            custom = inst[2] if len(inst) > 2 and inst[2] is not None else i
+            mask = inst[3] if len(inst) > 3 else 0xFF
+            m = torch.as_tensor(xf, dtype=torch.float64).reshape(-1)
+            if m.numel() == 16:
+                m = m.reshape(4, 4)[:3].reshape(-1)
+            if m.numel() != 12:
+                raise ValueError("transform must be 3x4 or 4x4")
+            arr[i].transform[:] = [float(x) for x in m.tolist()]
+            arr[i].custom_index = int(custom)
+            arr[i].mask = int(mask)
+            arr[i].blas = bvh.handle
+            self._blas.append(bvh)
+        self.n_instances = len(instances)
+        opts = BuildOpts(0, 0, treelet_size, 0, 0)
+        h = ctypes.c_void_p()
+        _lib.check(lib.gpurt_tlas_build(ctx.handle, arr, len(instances), ctypes.byref(opts),
+                                        _stream(ctx.device), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def device(self):
+        return self.ctx.device
+
+    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger):
+        if stats is not None and (stats.dtype != torch.int64 or stats.numel() < 4):
+            raise ValueError("stats must be an int64 tensor of >= 4 elements")
+        return QueryOpts(stack_depth, waves_per_cu,
+                         ctypes.c_void_p(stats.data_ptr()) if stats is not None else None,
+                         leaf_trigger, 0)
+
+    def trace_closest(self, rays, with_uv=True, stack_depth=0, waves_per_cu=0, stats=None,
+                      out=None, leaf_trigger=0):
+        """Returns (obj, prim, t, uv): obj = custom index of the hit instance (-1: miss)."""
+        dev = self.device
+        _need(rays, torch.float32, (8,), "rays", dev)
+        n = rays.shape[0]
+        if out is None:
+            obj = torch.empty(n, dtype=torch.int32, device=dev)
+            prim = torch.empty(n, dtype=torch.int32, device=dev)
+            t = torch.empty(n, dtype=torch.float32, device=dev)
+            uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
+        else:
+            obj, prim, t, uv = out
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        _lib.check(_lib.load().gpurt_tlas_trace_closest(self._h, _ptr(rays), n, _ptr(obj), _ptr(prim),
+                                                        _ptr(t), _ptr(uv), ctypes.byref(opts),
+                                                        _stream(dev)))
+        return obj, prim, t, uv
+
+    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0):
+        dev = self.device
+        _need(rays, torch.float32, (8,), "rays", dev)
+        n = rays.shape[0]
+        occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        _lib.check(_lib.load().gpurt_tlas_trace_any(self._h, _ptr(rays), n, _ptr(occ),
+                                                    ctypes.byref(opts), _stream(dev)))
+        return occ
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().gpurt_tlas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class _ClosestPointDistance(torch.autograd.Function):
     """d_i = min over triangles |q_i - p| ; backward via gpurt_closest_point_bwd."""
 

@@ -30,6 +30,7 @@ EXPORTS = (
     "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_closest_point", "gpurt_trace_closest",
     "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd", "gpurt_batch_create",
     "gpurt_batch_run", "gpurt_batch_set_grads", "gpurt_batch_run_backward", "gpurt_batch_destroy",
+    "gpurt_tlas_build", "gpurt_tlas_destroy", "gpurt_tlas_trace_closest", "gpurt_tlas_trace_any",
     "gpurt_last_error",
 )
 
@@ -72,6 +73,11 @@ class BatchMesh(ctypes.Structure):
                 ("r_uv", ctypes.c_void_p)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("transform", ctypes.c_float * 12), ("custom_index", ctypes.c_uint32),
+                ("mask", ctypes.c_uint32), ("blas", ctypes.c_void_p)]
+
+
 class BatchGrad(ctypes.Structure):
     _fields_ = [("g_dist", ctypes.c_void_p), ("g_q", ctypes.c_void_p), ("g_t", ctypes.c_void_p),
                 ("g_origin", ctypes.c_void_p), ("g_dir", ctypes.c_void_p),
@@ -108,6 +114,11 @@ def load():
         "gpurt_batch_run": (I, [P, P]),
         "gpurt_batch_set_grads": (I, [P, ctypes.POINTER(BatchGrad)]),
         "gpurt_batch_run_backward": (I, [P, P]),
+        "gpurt_tlas_build": (I, [P, ctypes.POINTER(Instance), S, ctypes.POINTER(BuildOpts), P,
+                                 ctypes.POINTER(P)]),
+        "gpurt_tlas_destroy": (V, [P]),
+        "gpurt_tlas_trace_closest": (I, [P, P, S, P, P, P, P, ctypes.POINTER(QueryOpts), P]),
+        "gpurt_tlas_trace_any": (I, [P, P, S, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_batch_destroy": (V, [P]),
         "gpurt_last_error": (ctypes.c_char_p, []),
     }

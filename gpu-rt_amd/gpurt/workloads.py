@@ -220,6 +220,52 @@ def rays_scene(n_rays=1 << 20, grid=708, n_spheres=64, sphere_level=5):
                      "grid": grid, "n_spheres": n_spheres}}
 
 
+def random_rotation(rng):
+    """Uniform random rotation (QR of a Gaussian matrix with the sign fixed)."""
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def instanced_scene(n_rays=1 << 20, grid=708, n_spheres=64, sphere_level=5, seed=2):
+    """rays_scene as the reference lays it out: one terrain mesh and ONE icosphere mesh,
+    placed by 1 + n_spheres instances (object-to-world 3x4 transforms: identity for the
+    terrain; rotation * radius + position for each sphere, the way GPURT::build_accel
+    places objects by scale * pose), with the same incoherent rays (origins uniform in the
+    scene box, uniform directions; any-hit tmax U[1, 50])."""
+    rng = np.random.default_rng(seed)
+    tv, tf = terrain(grid, rng)
+    hmin, hmax = tv[:, 1].min(), tv[:, 1].max()
+    sv, sf = icosphere(sphere_level)
+    radii = rng.uniform(0.5, 3.0, n_spheres)
+    cx = rng.uniform(-45.0, 45.0, n_spheres)
+    cz = rng.uniform(-45.0, 45.0, n_spheres)
+    cy = rng.uniform(hmin, hmax + 8.0, n_spheres)
+    xfs = [np.concatenate([np.eye(3), np.zeros((3, 1))], 1)]
+    for k in range(n_spheres):
+        M = random_rotation(rng) * radii[k]
+        xfs.append(np.concatenate([M, np.array([[cx[k]], [cy[k]], [cz[k]]])], 1))
+    xfs = np.asarray(xfs, np.float32)
+    lo = np.minimum(tv.min(0), (np.array([cx, cy, cz]).T - radii[:, None]).min(0))
+    hi = np.maximum(tv.max(0), (np.array([cx, cy, cz]).T + radii[:, None]).max(0))
+    r = np.random.default_rng(seed + 1)
+    o = r.uniform(lo, hi, (n_rays, 3))
+    d = random_dirs(r, n_rays)
+    rays = pack_rays(o, d, 1e-5, 1e7)
+    rays_any = pack_rays(o, d, 1e-5, 0.0)
+    rays_any[:, 7] = r.uniform(1.0, 50.0, n_rays)
+    meshes = [_as_f32(tv, tf), _as_f32(sv, sf)]
+    mesh_of = np.array([0] + [1] * n_spheres, np.int32)
+    n_tris = int(tf.shape[0] + n_spheres * sf.shape[0])
+    return {"meshes": meshes, "xf": xfs.reshape(-1, 12), "mesh_of": mesh_of, "rays": rays,
+            "rays_any": rays_any,
+            "meta": {"workload": "instanced_scene", "n_instances": int(n_spheres + 1),
+                     "n_blas_tris": int(tf.shape[0] + sf.shape[0]), "n_tris_instanced": n_tris,
+                     "n_rays": int(n_rays), "grid": grid, "n_spheres": n_spheres}}
+
+
 def near_surface_queries(v, f, rng, n, frac_surface=0.8, sigma=0.005, box=2.0):
     """80% surface samples + N(0, sigma) normal offsets, 20% uniform in a box: the cube
     [-box, box]^3 for a number, or an explicit (lo, hi) pair."""

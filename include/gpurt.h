@@ -235,6 +235,51 @@ GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* g
 GPURT_API int gpurt_batch_run_backward(gpurt_batch batch, void* stream);
 GPURT_API void gpurt_batch_destroy(gpurt_batch batch);
 
+/* ---- two-level instancing (TLAS) ----------------------------------------------------- */
+
+typedef struct gpurt_tlas_s* gpurt_tlas;
+
+/* One instance: a BLAS placed by an object-to-world transform.  Mirrors
+ * VkAccelerationStructureInstanceKHR as filled by Accel::recreate(blas, T)
+ * (src/vk/vulkan.cpp:777-856): transform = the row-major 3x4 VkTransformMatrixKHR
+ * (T[i].T(), vulkan.cpp:795-796), custom_index = instanceCustomIndex (i, line 798; reported
+ * as a hit's object id, gl_InstanceCustomIndexEXT in rt.rchit:14), mask = 8-bit visibility
+ * mask (0xFF there; rays use cull mask 0xFF as traceRayEXT does in rt.rgen:258,278, so an
+ * instance with mask 0 is never hit).  Face culling is off for every instance
+ * (TRIANGLE_FACING_CULL_DISABLE, line 802), as for BLAS queries. */
+typedef struct gpurt_instance {
+    float transform[12];
+    uint32_t custom_index;
+    uint32_t mask;
+    gpurt_bvh blas;
+} gpurt_instance;
+
+/* Build a TLAS over n instances (replaces Accel(const std::vector<Drop<Accel>>&,
+ * const std::vector<Mat4>&), vulkan.h:259, and its recreate, vulkan.cpp:777-856).  Every
+ * BLAS must have the same node width and ray node format; the BLASes must outlive the
+ * TLAS, and a BLAS refit requires rebuilding the TLAS (as a Vulkan TLAS build does).
+ * Rays are traversed in world space through the TLAS and in object space inside an
+ * instance (o' = M^-1 (o - c), d' = M^-1 d with the FP32 rounding of the FP64 inverse of
+ * the 3x3 part M; c the translation), so t is the same parameter in both spaces.
+ * Results are exact (equal to FP32 brute force over all instances, ties to the lowest
+ * instance index, then the lowest primitive id) for ray origins within 2^8 times the
+ * scene's extent; transforms must be invertible.  Returns when the TLAS is complete. */
+GPURT_API int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n,
+                               const gpurt_build_opts* opts, void* stream, gpurt_tlas* out);
+GPURT_API void gpurt_tlas_destroy(gpurt_tlas tlas);
+
+/* Closest hit through the TLAS: per ray obj (custom_index of the hit instance, -1 on a
+ * miss), prim (triangle index in that instance's mesh, -1 on a miss), t, uv (nullable).
+ * Replaces traceRayEXT(TLAS, ...) in trace_ray (rt.rgen:257-270) with the payload of
+ * rt.rchit:11-16 / rt.rmiss:9-11. */
+GPURT_API int gpurt_tlas_trace_closest(gpurt_tlas tlas, const gpurt_ray* rays, size_t n, int32_t* obj,
+                                       int32_t* prim, float* t, float* uv, const gpurt_query_opts* opts,
+                                       void* stream);
+/* Any hit through the TLAS (visibility, rt.rgen:272-291): occluded[i] = 1 if some
+ * instance has a hit with t in [tmin, tmax]. */
+GPURT_API int gpurt_tlas_trace_any(gpurt_tlas tlas, const gpurt_ray* rays, size_t n, uint8_t* occluded,
+                                   const gpurt_query_opts* opts, void* stream);
+
 /* ---- errors ----------------------------------------------------------------------- */
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -24,6 +24,10 @@
  *     *_brute_f32         global (value, lowest index) minimum over all triangles;
  *                         any-hit = "some triangle has t in [tmin, tmax]"
  *                         (rt.rgen:272-291, rt.rmiss:9-11).
+ *     orc_inst_*          two-level instancing (the TLAS of vulkan.cpp:777-856): each
+ *                         instance's mesh is tested with the object-space ray
+ *                         o' = M^-1 (o - c), d' = M^-1 d (M^-1 = FP32 rounding of the FP64
+ *                         cofactor inverse); minimum over (t, instance, triangle).
  *     orc_lbvh_*          CPU LBVH: 30-bit Morton codes of AABB centroids over the
  *                         centroid bounds, stable sort, Karras (HPG 2012) radix-tree
  *                         hierarchy, leaves of <= leaf_size primitives; closest-first
@@ -326,6 +330,114 @@ ORC_API void orc_ray_any_brute_f32(const float* V, const uint32_t* F, int64_t nt
                                    int64_t nr, uint8_t* occ) {
     ray_bf32_ctx c = {V, F, nt, R, NULL, NULL, NULL, occ};
     parallel_for(nr, 16, ray_any_bf32_range, &c);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* two-level instancing                                                                  */
+/* ------------------------------------------------------------------------------------ */
+
+/* FP32 inverse of the 3x3 part of a row-major 3x4 object-to-world transform: cofactors
+ * over the determinant in FP64, then rounded.  Returns 0 if singular or not finite. */
+ORC_API int orc_invert3(const float* t, float* minv) {
+    const double m00 = t[0], m01 = t[1], m02 = t[2];
+    const double m10 = t[4], m11 = t[5], m12 = t[6];
+    const double m20 = t[8], m21 = t[9], m22 = t[10];
+    const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+    if (!(det != 0.0) || !isfinite(det)) return 0;
+    const double inv[9] = {(m11 * m22 - m12 * m21) / det, (m02 * m21 - m01 * m22) / det, (m01 * m12 - m02 * m11) / det,
+                           (m12 * m20 - m10 * m22) / det, (m00 * m22 - m02 * m20) / det, (m02 * m10 - m00 * m12) / det,
+                           (m10 * m21 - m11 * m20) / det, (m01 * m20 - m00 * m21) / det, (m00 * m11 - m01 * m10) / det};
+    for (int k = 0; k < 9; k++) {
+        if (!isfinite(inv[k])) return 0;
+        minv[k] = (float)inv[k];
+    }
+    return 1;
+}
+
+/* object-space ray: x = o - c, o' = rows of minv . x, d' = rows of minv . d, each row
+ * summed left to right with every product rounded */
+static inline void to_object_f32(const float* minv, const float* t, const float* o, const float* d, float* oo,
+                                 float* od) {
+    const float x[3] = {o[0] - t[3], o[1] - t[7], o[2] - t[11]};
+    for (int r = 0; r < 3; r++) {
+        float a = minv[3 * r] * x[0];
+        a = a + minv[3 * r + 1] * x[1];
+        a = a + minv[3 * r + 2] * x[2];
+        oo[r] = a;
+        float b = minv[3 * r] * d[0];
+        b = b + minv[3 * r + 1] * d[1];
+        b = b + minv[3 * r + 2] * d[2];
+        od[r] = b;
+    }
+}
+
+typedef struct {
+    int64_t n_inst;
+    const float* xf;       /* 12 per instance */
+    const float* minv;     /* 9 per instance */
+    const int32_t* mesh;   /* mesh of each instance */
+    const uint32_t* custom;
+    const uint32_t* mask;
+    const float* V;        /* all meshes' vertices */
+    const uint32_t* F;     /* all meshes' triangles, global vertex indices */
+    const int64_t* off;    /* triangles of mesh m: [off[m], off[m+1]) */
+    const float* R;
+    int32_t* obj;
+    int32_t* prim;
+    float* t;
+    float* uv;
+    uint8_t* occ;
+} inst_ctx;
+
+static void inst_ray_range(void* vctx, int64_t lo, int64_t hi) {
+    inst_ctx* c = (inst_ctx*)vctx;
+    for (int64_t i = lo; i < hi; i++) {
+        const float* r = c->R + 8 * i;
+        float tmin = r[3], best = r[7], bu = 0, bv = 0;
+        int64_t bi = -1, bk = -1;
+        uint8_t hit = 0;
+        for (int64_t n = 0; n < c->n_inst && !hit; n++) {
+            if ((c->mask[n] & 0xFFu) == 0) continue;
+            float oo[3], od[3];
+            to_object_f32(c->minv + 9 * n, c->xf + 12 * n, r, r + 4, oo, od);
+            const int32_t m = c->mesh[n];
+            for (int64_t k = c->off[m]; k < c->off[m + 1]; k++) {
+                const uint32_t* f = c->F + 3 * k;
+                float t, u, v;
+                if (!orc_mt_f32(oo, od, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], &t, &u, &v)) continue;
+                if (!(t >= tmin)) continue;
+                if (c->occ) {
+                    if (t <= r[7]) { hit = 1; break; }
+                    continue;
+                }
+                if (t < best || (t == best && bi < 0)) { best = t; bi = n; bk = k - c->off[m]; bu = u; bv = v; }
+            }
+        }
+        if (c->occ) { c->occ[i] = hit; continue; }
+        c->obj[i] = bi >= 0 ? (int32_t)c->custom[bi] : -1;
+        c->prim[i] = bi >= 0 ? (int32_t)bk : -1;
+        c->t[i] = bi >= 0 ? best : INFINITY;
+        if (c->uv) { c->uv[2 * i] = bi >= 0 ? bu : 0.0f; c->uv[2 * i + 1] = bi >= 0 ? bv : 0.0f; }
+    }
+}
+
+/* Brute force over every instance and every triangle of its mesh.  Closest hit
+ * (occ == NULL): minimum over (t, instance, triangle), obj = custom index, prim =
+ * triangle index within the instance's mesh, -1 / +inf on a miss; any hit (occ != NULL):
+ * some visible instance has a hit with t in [tmin, tmax].  Instances with mask & 0xFF
+ * == 0 are invisible.  Returns 0 if a transform is singular. */
+ORC_API int orc_inst_ray_brute_f32(int64_t n_inst, const float* xf, const int32_t* mesh, const uint32_t* custom,
+                                   const uint32_t* mask, const float* V, const uint32_t* F, const int64_t* off,
+                                   const float* R, int64_t nr, int32_t* obj, int32_t* prim, float* t, float* uv,
+                                   uint8_t* occ) {
+    float* minv = (float*)malloc(sizeof(float) * 9 * (size_t)(n_inst > 0 ? n_inst : 1));
+    if (!minv) return 0;
+    for (int64_t n = 0; n < n_inst; n++)
+        if (!orc_invert3(xf + 12 * n, minv + 9 * n)) { free(minv); return 0; }
+    inst_ctx c = {n_inst, xf, minv, mesh, custom, mask, V, F, off, R, obj, prim, t, uv, occ};
+    parallel_for(nr, 4, inst_ray_range, &c);
+    free(minv);
+    return 1;
 }
 
 /* ------------------------------------------------------------------------------------ */

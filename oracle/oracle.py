@@ -54,6 +54,11 @@ def load():
     lib.orc_cpq_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P]
     lib.orc_ray_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P, P]
     lib.orc_morton_codes.argtypes = [_f32p, I64, _f32p, _f32p, _u32p]
+    lib.orc_invert3.argtypes = [_f32p, _f32p]
+    lib.orc_invert3.restype = I
+    lib.orc_inst_ray_brute_f32.argtypes = [I64, _f32p, _i32p, _u32p, _u32p, _f32p, _u32p, _i64p, _f32p, I64,
+                                           P, P, P, P, P]
+    lib.orc_inst_ray_brute_f32.restype = I
     lib.orc_lbvh_build.argtypes = [_f32p, I64, _u32p, I64, I]
     lib.orc_lbvh_build.restype = P
     lib.orc_lbvh_free.argtypes = [P]
@@ -147,6 +152,54 @@ def ray_any_brute_f32(V, F, R):
     occ = np.empty(R.shape[0], np.uint8)
     load().orc_ray_any_brute_f32(V, F, F.shape[0], R, R.shape[0], occ)
     return occ
+
+
+def invert3(xf):
+    """FP32 inverse of the 3x3 part of a row-major 3x4 transform (FP64 cofactors)."""
+    xf = _c(np.asarray(xf, np.float32).reshape(12), np.float32)
+    out = np.empty(9, np.float32)
+    if not load().orc_invert3(xf, out):
+        raise ValueError("singular transform")
+    return out.reshape(3, 3)
+
+
+def _flatten_meshes(meshes):
+    Vs, Fs, off, base = [], [], [0], 0
+    for V, F in meshes:
+        V = np.asarray(V, np.float32).reshape(-1, 3)
+        Fs.append(np.asarray(F, np.uint32).reshape(-1, 3) + np.uint32(base))
+        Vs.append(V)
+        base += V.shape[0]
+        off.append(off[-1] + Fs[-1].shape[0])
+    return (_c(np.concatenate(Vs), np.float32), _c(np.concatenate(Fs), np.uint32),
+            _c(np.asarray(off, np.int64), np.int64))
+
+
+def inst_ray_brute_f32(xf, mesh_of, meshes, R, custom=None, mask=None, any_hit=False):
+    """Two-level brute force: instance n places meshes[mesh_of[n]] = (V, F) by the
+    row-major 3x4 transform xf[n].  Closest hit: (obj, prim, t, uv); any hit: occ."""
+    xf = _c(np.asarray(xf, np.float32).reshape(-1, 12), np.float32)
+    n_inst = xf.shape[0]
+    mesh_of = _c(np.asarray(mesh_of, np.int32), np.int32)
+    custom = _c(np.arange(n_inst, dtype=np.uint32) if custom is None else np.asarray(custom, np.uint32), np.uint32)
+    mask = _c(np.full(n_inst, 0xFF, np.uint32) if mask is None else np.asarray(mask, np.uint32), np.uint32)
+    V, F, off = _flatten_meshes(meshes)
+    R = _c(R, np.float32)
+    n = R.shape[0]
+    if any_hit:
+        occ = np.empty(n, np.uint8)
+        ok = load().orc_inst_ray_brute_f32(n_inst, xf, mesh_of, custom, mask, V, F, off, R, n,
+                                           None, None, None, None, _p(occ))
+        if not ok:
+            raise ValueError("singular transform")
+        return occ
+    obj = np.empty(n, np.int32); prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32); uv = np.empty((n, 2), np.float32)
+    ok = load().orc_inst_ray_brute_f32(n_inst, xf, mesh_of, custom, mask, V, F, off, R, n,
+                                       _p(obj), _p(prim), _p(t), _p(uv), None)
+    if not ok:
+        raise ValueError("singular transform")
+    return obj, prim, t, uv
 
 
 def ray_closest_brute_f64(V, F, R):

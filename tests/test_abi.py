@@ -46,7 +46,8 @@ def _c_layout(struct, fields):
                                         (_lib.QueryOpts, "gpurt_query_opts"),
                                         (_lib.BvhInfo, "gpurt_bvh_info"),
                                         (_lib.BatchMesh, "gpurt_batch_mesh"),
-                                        (_lib.BatchGrad, "gpurt_batch_grad")])
+                                        (_lib.BatchGrad, "gpurt_batch_grad"),
+                                        (_lib.Instance, "gpurt_instance")])
 def test_struct_layouts_match_header(cls, struct):
     fields = [f for f, _ in cls._fields_]
     size, offs = _c_layout(struct, fields)

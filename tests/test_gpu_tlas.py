@@ -1,0 +1,96 @@
+"""Two-level instancing (gpurt_tlas_*) against the oracle's instanced brute force
+(orc_inst_ray_brute_f32): the same FP32 object-space rays and the same triangle test, so
+object ids, primitive ids, t, (u,v) and occlusion are compared bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+import gpurt
+from gpurt import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _np(*ts):
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() if t is not None else None for t in ts]
+
+
+def _check(ctx, dev, sc, rays, rays_any, custom=None, mask=None, **bkw):
+    blas = [gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **bkw) for V, F in sc["meshes"]]
+    n = sc["xf"].shape[0]
+    custom = list(range(100, 100 + n)) if custom is None else custom
+    mask = [0xFF] * n if mask is None else mask
+    inst = [(blas[m], sc["xf"][i].reshape(3, 4), custom[i], mask[i]) for i, m in enumerate(sc["mesh_of"])]
+    tlas = gpurt.Tlas(ctx, inst)
+    obj, prim, t, uv = _np(*tlas.trace_closest(_t(rays, dev)))
+    eo, ep, et, euv = O.inst_ray_brute_f32(sc["xf"], sc["mesh_of"], sc["meshes"], rays, custom=custom, mask=mask)
+    assert np.array_equal(prim, ep), f"{(prim != ep).sum()} prim ids differ"
+    assert np.array_equal(obj, eo), f"{(obj != eo).sum()} object ids differ"
+    assert np.array_equal(t, et)
+    assert np.array_equal(uv, euv)
+    occ, = _np(tlas.trace_any(_t(rays_any, dev)))
+    eocc = O.inst_ray_brute_f32(sc["xf"], sc["mesh_of"], sc["meshes"], rays_any, custom=custom, mask=mask,
+                                any_hit=True)
+    assert np.array_equal(occ, eocc)
+    return obj, tlas
+
+
+@pytest.mark.parametrize("fmt", [1, 2, 3])
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_tlas_matches_instanced_brute_force(ctx, dev, width, fmt):
+    sc = W.instanced_scene(n_rays=6000, grid=41, n_spheres=12, sphere_level=2, seed=5)
+    print("tlas width", width, "format", fmt, sc["meta"], flush=True)
+    obj, _ = _check(ctx, dev, sc, sc["rays"], sc["rays_any"], node_width=width, node_format=fmt)
+    assert len(set(obj.tolist())) > 5  # rays hit many different instances
+
+
+def test_tlas_masks_and_custom_indices(ctx, dev):
+    sc = W.instanced_scene(n_rays=4000, grid=33, n_spheres=8, sphere_level=2, seed=6)
+    n = sc["xf"].shape[0]
+    mask = [0xFF] * n
+    mask[0] = 0     # the terrain is invisible
+    mask[3] = 0x100  # no bit of the 0xFF ray mask: invisible
+    custom = [7 * i + 3 for i in range(n)]
+    obj, _ = _check(ctx, dev, sc, sc["rays"], sc["rays_any"], custom=custom, mask=mask)
+    assert 3 not in obj.tolist() and custom[3] not in obj.tolist()
+
+
+def test_tlas_ties_between_coincident_instances(ctx, dev):
+    """Two instances with the same transform of the same mesh: every hit is an exact tie,
+    resolved to the lower instance."""
+    sc = W.instanced_scene(n_rays=3000, grid=25, n_spheres=3, sphere_level=2, seed=7)
+    sc["xf"] = np.concatenate([sc["xf"], sc["xf"][2:3]])
+    sc["mesh_of"] = np.concatenate([sc["mesh_of"], sc["mesh_of"][2:3]])
+    _check(ctx, dev, sc, sc["rays"], sc["rays_any"])
+
+
+def test_identity_instance_equals_plain_bvh(ctx, dev):
+    w = W.rays_scene(n_rays=4096, grid=61, n_spheres=3)
+    V, F = _t(w["verts"], dev), _t(w["tris"], dev)
+    bvh = gpurt.Bvh(ctx, V, F)
+    tlas = gpurt.Tlas(ctx, [(bvh, np.eye(4), 42)])
+    R = _t(w["rays"], dev)
+    o, p, t, uv = tlas.trace_closest(R)
+    pp, pt, puv = bvh.trace_closest(R)
+    torch.cuda.synchronize()
+    assert torch.equal(p, pp) and torch.equal(t, pt) and torch.equal(uv, puv)
+    assert torch.equal(o, torch.where(pp >= 0, 42, -1).to(torch.int32))
+
+
+def test_tlas_rejects_bad_instances(ctx, dev):
+    sc = W.instanced_scene(n_rays=16, grid=9, n_spheres=2, sphere_level=1, seed=8)
+    (V0, F0), (V1, F1) = sc["meshes"]
+    b4 = gpurt.Bvh(ctx, _t(V0, dev), _t(F0, dev), node_width=4)
+    b8 = gpurt.Bvh(ctx, _t(V1, dev), _t(F1, dev), node_width=8)
+    with pytest.raises(gpurt.GpurtError):
+        gpurt.Tlas(ctx, [(b4, np.eye(4)), (b8, np.eye(4))])  # mixed node widths
+    singular = np.eye(4)
+    singular[2, 2] = 0.0
+    with pytest.raises(gpurt.GpurtError):
+        gpurt.Tlas(ctx, [(b4, singular)])

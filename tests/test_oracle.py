@@ -282,3 +282,119 @@ def test_ray_gradients_match_fp64_central_differences():
             fd = (O.ray_closest_brute_f64(Vp, F, R)[1].sum() -
                   O.ray_closest_brute_f64(Vm, F, R)[1].sum()) / (2 * h)
             assert gv[j, k] == pytest.approx(fd, abs=1e-3)
+
+
+# ---- two-level instancing -------------------------------------------------------------
+
+def _xf(M=None, c=(0.0, 0.0, 0.0)):
+    M = np.eye(3) if M is None else np.asarray(M, np.float64)
+    X = np.zeros((3, 4), np.float32)
+    X[:, :3] = M
+    X[:, 3] = c
+    return X.reshape(12)
+
+
+def test_invert3_matches_numpy_and_rejects_singular():
+    rng = np.random.default_rng(21)
+    for _ in range(50):
+        M = rng.normal(size=(3, 3)) + 3 * np.eye(3)
+        inv = O.invert3(_xf(M))
+        np.testing.assert_allclose(inv, np.linalg.inv(M.astype(np.float32).astype(np.float64)), rtol=2e-6,
+                                   atol=2e-7)
+    with pytest.raises(ValueError):
+        O.invert3(_xf(np.array([[1, 2, 3], [2, 4, 6], [0, 0, 1.0]])))
+
+
+def _dyadic_scene(rng, n_tris=40, n_rays=600):
+    # coordinates on a 1/16 grid in [-4, 4]: every transformed vertex and ray origin below
+    # is exactly representable, so only the MT arithmetic rounds
+    V = rng.integers(-64, 65, size=(3 * n_tris, 3)).astype(np.float32) / 16
+    F = np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)
+    o = rng.integers(-128, 129, size=(n_rays, 3)).astype(np.float64) / 16
+    d = rng.integers(-16, 17, size=(n_rays, 3)).astype(np.float64) / 16
+    d[np.all(d == 0, axis=1)] = [0, 0, 1]
+    R = W.pack_rays(o, d, 0.0, 1e7)
+    return V, F, R
+
+
+def test_instance_identity_equals_plain_brute_force():
+    V, F, R = _dyadic_scene(np.random.default_rng(22))
+    obj, prim, t, uv = O.inst_ray_brute_f32([_xf()], [0], [(V, F)], R, custom=[7])
+    ep, et, euv = O.ray_closest_brute_f32(V, F, R)
+    assert np.array_equal(prim, ep) and np.array_equal(t, et) and np.array_equal(uv, euv)
+    assert np.array_equal(obj, np.where(ep >= 0, 7, -1))
+    assert np.array_equal(O.inst_ray_brute_f32([_xf()], [0], [(V, F)], R, any_hit=True),
+                          O.ray_any_brute_f32(V, F, R))
+
+
+@pytest.mark.parametrize("s", [0.5, 1.0, 2.0, 4.0])
+def test_scaled_translated_instance_equals_transformed_mesh(s):
+    """A power-of-two scale and a dyadic translation transform the scene exactly, and IEEE
+    rounding commutes with power-of-two scaling: the instance answers exactly like the
+    plain brute force on the mesh moved into world space."""
+    rng = np.random.default_rng(23)
+    V, F, R = _dyadic_scene(rng)
+    c = np.array([0.75, -1.5, 2.25], np.float32)
+    Vw = (V * np.float32(s) + c).astype(np.float32)
+    # aim two thirds of the rays at dyadic interior points (2a + b + c) / 4 of triangles
+    k = rng.integers(0, F.shape[0], 400)
+    tgt = (2 * Vw[F[k, 0]] + Vw[F[k, 1]] + Vw[F[k, 2]]) / 4
+    R[:400, 4:7] = tgt - R[:400, :3]
+    R = R[np.any(R[:, 4:7] != 0, axis=1)]
+    obj, prim, t, uv = O.inst_ray_brute_f32([_xf(np.eye(3) * s, c)], [0], [(V, F)], R)
+    ep, et, euv = O.ray_closest_brute_f32(Vw, F, R)
+    assert (ep >= 0).mean() > 0.2
+    assert np.array_equal(prim, ep) and np.array_equal(t, et) and np.array_equal(uv, euv)
+
+
+def test_instances_masks_ties_and_custom_indices():
+    V, F, R = _dyadic_scene(np.random.default_rng(24))
+    # three copies of the same placement: exact ties go to the lowest visible instance
+    xfs = [_xf()] * 3
+    obj, prim, t, _ = O.inst_ray_brute_f32(xfs, [0, 0, 0], [(V, F)], R, custom=[10, 11, 12], mask=[0, 0xFF, 0xFF])
+    ep, et, _ = O.ray_closest_brute_f32(V, F, R)
+    assert np.array_equal(obj, np.where(ep >= 0, 11, -1)) and np.array_equal(prim, ep)
+    # a mask with no bit of the ray's 0xFF cull mask hides an instance
+    occ = O.inst_ray_brute_f32(xfs, [0, 0, 0], [(V, F)], R, mask=[0x100, 0, 0x200], any_hit=True)
+    assert not occ.any()
+
+
+def test_instanced_rays_agree_with_fp64_world_space():
+    """Rotated/scaled instances of two meshes against the FP64 brute force over the
+    meshes transformed to world space in FP64: ids agree except at near-ties, t to the
+    rounding of the transform."""
+    rng = np.random.default_rng(25)
+    v1, f1 = W.icosphere(2)
+    v2, f2 = W.terrain(9, rng)
+    v2 = v2 / 10.0
+    meshes = [(v1.astype(np.float32), f1), (v2.astype(np.float32), f2)]
+    xfs, mesh_of, Vw, Fw, base = [], [], [], [], 0
+    for k in range(6):
+        a = rng.normal(size=3)
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        M = Q * rng.uniform(0.5, 2.0)
+        c = rng.uniform(-3, 3, 3)
+        X = _xf(M, c)
+        xfs.append(X)
+        m = k % 2
+        mesh_of.append(m)
+        X64 = X.astype(np.float64).reshape(3, 4)
+        Vw.append(meshes[m][0].astype(np.float64) @ X64[:, :3].T + X64[:, 3])
+        Fw.append(meshes[m][1].astype(np.uint32) + base)
+        base += meshes[m][0].shape[0]
+    o = rng.uniform(-5, 5, (2000, 3))
+    R = W.pack_rays(o, W.random_dirs(rng, 2000), 1e-5, 1e7)
+    obj, prim, t, _ = O.inst_ray_brute_f32(xfs, mesh_of, meshes, R)
+    ep, et, _ = O.ray_closest_brute_f64(np.concatenate(Vw), np.concatenate(Fw), R.astype(np.float64))
+    # world triangle index -> (instance, local prim)
+    counts = [meshes[m][1].shape[0] for m in mesh_of]
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    inst64 = np.searchsorted(starts, ep, side="right") - 1
+    local64 = ep - starts[np.clip(inst64, 0, len(counts) - 1)]
+    hit = ep >= 0
+    same = (obj == np.where(hit, inst64, -1)) & (prim == np.where(hit, local64, -1))
+    assert same.mean() > 0.995
+    both = same & hit
+    # the object-space ray carries the FP32 rounding of (o - c) and M^-1: an absolute
+    # position error of a few 2^-24 of the scene scale (~5), plus FP32 MT rounding
+    np.testing.assert_allclose(t[both], et[both], rtol=1e-5, atol=4e-6)
