@@ -355,6 +355,45 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
                     "gradients of sum(dist) and sum(t) w.r.t. queries, rays and vertices"}
 
 
+def tlas_bench(args, torch, gpurt, W, O, dev, ctx):
+    """rays_scene laid out as the reference builds it: a terrain BLAS and one icosphere
+    BLAS placed by 65 instances (TLAS), the same 1,048,576 + 1,048,576 rays; closest and
+    any hit through two levels, checked bit for bit against the oracle's instanced brute
+    force on a sample of rays."""
+    sc = W.instanced_scene()
+    blas = [gpurt.Bvh(ctx, torch.from_numpy(V).to(dev), torch.from_numpy(F).to(dev), node_width=args.width,
+                      treelet_size=args.treelet, node_format=args.format, leaf_size=args.leaf)
+            for V, F in sc["meshes"]]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tlas = gpurt.Tlas(ctx, [(blas[m], sc["xf"][i].reshape(3, 4)) for i, m in enumerate(sc["mesh_of"])])
+    build_ms = (time.perf_counter() - t0) * 1e3
+    rays = torch.from_numpy(sc["rays"]).to(dev)
+    rays_any = torch.from_numpy(sc["rays_any"]).to(dev)
+    n = rays.shape[0]
+    out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
+           torch.empty(n, device=dev), torch.empty((n, 2), device=dev))
+    occ = torch.empty(n, dtype=torch.uint8, device=dev)
+    q = dict(stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+    ms_c = timed(lambda: tlas.trace_closest(rays, out=out, **q), args.steps, 1)
+    ms_a = timed(lambda: tlas.trace_any(rays_any, out=occ, **q), args.steps, 1)
+    st = torch.zeros(4, dtype=torch.int64, device=dev)
+    tlas.trace_closest(rays, out=out, stats=st, **q)
+    torch.cuda.synchronize()
+    nodes, tris, _, _ = st.cpu().tolist()
+    idx = np.arange(0, n, 4096)
+    eo, ep, et, euv = O.inst_ray_brute_f32(sc["xf"], sc["mesh_of"], sc["meshes"], sc["rays"][idx])
+    go, gp, gt, guv = (x.cpu().numpy()[idx] for x in out)
+    parity = bool(np.array_equal(go, eo) and np.array_equal(gp, ep) and np.array_equal(gt, et)
+                  and np.array_equal(guv, euv))
+    tlas.close()
+    return {"workload": "instanced_scene", **{k: v for k, v in sc["meta"].items() if k != "workload"},
+            "closest_rays_per_s": n / (ms_c * 1e-3), "any_rays_per_s": n / (ms_a * 1e-3),
+            "closest_ms": ms_c, "any_ms": ms_a, "tlas_build_ms": build_ms,
+            "nodes_per_ray": nodes / n, "tris_per_ray": tris / n,
+            "parity_sample": int(idx.size), "parity_vs_instanced_brute_force": parity}
+
+
 def profile_mode(args, torch, gpurt, W, dev, ctx):
     """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
     rays_scene, then the same number of elementwise adds over 256 MiB (a plain streaming
@@ -437,6 +476,8 @@ def main():
     if not args.no_extra:
         log("[bench] cpq_small")
         extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx, O, 1)
+        log("[bench] instanced scene (TLAS)")
+        extra["tlas"] = tlas_bench(args, torch, gpurt, W, O, dev, ctx)
         log("[bench] diff fwd / fwd+bwd")
         extra["diff"] = diff_bench(args, torch, gpurt, W, dev, ctx)
         if not args.no_batched:

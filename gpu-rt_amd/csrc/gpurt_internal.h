@@ -235,8 +235,9 @@ struct TravScratch {
     uint32_t* counters = nullptr;
     uint2* spill = nullptr;
 };
+// waves_per_simd: what the kernel is compiled for (0: the single-level kernels' WavesHint)
 int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
-                   size_t n, TravPlan& p);
+                   size_t n, TravPlan& p, int waves_per_simd = 0);
 int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t);
 void free_traversal_scratch(TravScratch& t, hipStream_t s);
 

@@ -144,7 +144,7 @@ static int default_waves_per_cu(QKind kind) { return kind == QKind::Any ? 24 : 0
 static int block_for_stack(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 
 int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
-                   size_t n, TravPlan& p) {
+                   size_t n, TravPlan& p, int waves_per_simd) {
     if (n > 0xFFFFFFFFull - 1024u) return set_error(GPURT_E_INVALID, "too many queries in one call");
     // default: the smallest LDS stack (most resident waves); 8-wide nodes push more
     // entries per visit, so they default to 16
@@ -155,8 +155,9 @@ int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need
     // resident blocks per CU: limited by LDS (the stacks), by the 32 wave slots of a CU
     // and by registers (WavesHint waves per SIMD), unless the caller asks for waves/CU
     const int waves_per_block = p.BS / 64;
-    const int hint = width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
-                                : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
+    const int hint = waves_per_simd > 0 ? waves_per_simd
+                     : width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
+                                  : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
     int blocks_per_cu = std::min(ctx->lds_per_cu / (p.S * p.BS * 8), std::min(32, 4 * hint) / waves_per_block);
     if (opts && opts->waves_per_cu > 0)
         blocks_per_cu = std::max(1, opts->waves_per_cu / waves_per_block);

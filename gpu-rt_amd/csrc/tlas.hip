@@ -235,6 +235,12 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
     return ok;
 }
 
+// Resident waves per SIMD the two-level kernels are compiled for: the second level's
+// state (instance, stack base, BLAS pointers) does not fit 64 VGPRs without spilling, so
+// 4-wide nodes get up to 72 (7 waves); 8-wide nodes take what they need.
+template <int W>
+constexpr int kTlasWaves = W == 8 ? 4 : 7;
+
 struct TlasArgs {
     const void* top_nodes;
     const float4* top_tris;
@@ -255,7 +261,7 @@ struct TlasArgs {
 };
 
 template <int W, bool Q, int S, int BS, bool ANY, bool STATS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W, Q>::value))) void k_tlas_ray(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W>))) void k_tlas_ray(
     TlasArgs a) {
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
@@ -536,7 +542,8 @@ static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* ob
     // worst-case stack: the top levels' entries plus one BLAS's
     const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
     TravPlan p;
-    int rc = plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p);
+    int rc = plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
+                            W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
     if (rc) return rc;
     TravScratch sc;
     if ((rc = alloc_traversal_scratch(p, s, sc))) return rc;
