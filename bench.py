@@ -394,6 +394,50 @@ def tlas_bench(args, torch, gpurt, W, O, dev, ctx):
             "parity_sample": int(idx.size), "parity_vs_instanced_brute_force": parity}
 
 
+def tracer_bench(args, torch, gpurt, W, O, dev, ctx):
+    """The wavefront path tracer on a closed Cornell-style box (two 81,920-triangle
+    spheres, a block, five walls, one area light; 163,878 triangles over 9 BLASes) at
+    1024 x 1024 with the reference's defaults (8 samples per pixel per frame, max depth 8,
+    Russian roulette, Blinn-Phong), for direct light sampling and MIS.  Paths/s counts
+    camera paths; rays/s every ray traced (bounce rays through the TLAS plus shadow / light
+    rays).  Parity: a 32 x 32 render against the oracle's per-pixel restatement (fraction of
+    pixels within 1e-4 relative).  CPU: the oracle on a 64 x 64, 2-sample sample."""
+    from gpurt import tracer as T
+    sc = W.tracer_box(sphere_level=6)
+    scene = T.Scene(ctx, sc["objects"], node_width=args.width)
+    cam = sc["camera"]
+    w = h = 1024
+    iv, ip = T.camera(cam["eye"], cam["target"], yfov_deg=cam["yfov_deg"], aspect=1.0)
+    res = {"workload": "tracer_box", "triangles": int(sum(len(o["indices"]) for o in sc["objects"])),
+           "width": w, "height": h, "samples": 8, "max_depth": 8}
+    for name, integ in (("direct", 0), ("mis", 2)):
+        kw = dict(samples=8, max_depth=8, integrator=integ, brdf=0, use_rr=True)
+        img = torch.zeros((h, w, 4), dtype=torch.float32, device=dev)
+        counts = np.zeros(2, np.uint64)
+        scene.render(w, h, iv, ip, image=img, seed=1, ray_counts=counts, **kw)
+        steps = max(2, args.steps // 4)
+        ms = timed(lambda: scene.render(w, h, iv, ip, image=img, seed=1, **kw), steps, 1)
+        rays = int(counts.sum())
+        small = dict(width=32, height=32, inv_view=iv, inv_proj=ip, clear_col=(0.3,) * 3, env_light=(0.0,) * 3,
+                     frame=0, samples=2, max_frame=256, qmc=0, max_depth=8, use_metalness=0, integrator=integ, brdf=0,
+                     use_rr=1, seed=5)
+        g = scene.render(32, 32, iv, ip, samples=2, max_depth=8, integrator=integ, seed=5)
+        torch.cuda.synchronize()
+        ref = O.render(sc["objects"], small)
+        g = g.cpu().numpy()[..., :3]
+        agree = float((np.abs(g - ref[..., :3]) <= 1e-4 * (1 + np.abs(ref[..., :3]))).all(-1).mean())
+        cpu = dict(small, width=64, height=64)
+        t0 = time.perf_counter()
+        O.render(sc["objects"], cpu)
+        cpu_s = time.perf_counter() - t0
+        res[name] = {"ms_per_frame": ms, "paths_per_s": w * h * 8 / (ms * 1e-3),
+                     "rays_per_frame": rays, "rays_per_s": rays / (ms * 1e-3),
+                     "parity_pixels_agreeing_32x32": agree,
+                     "cpu_oracle_paths_per_s": 64 * 64 * 2 / cpu_s, "cpu_threads": O.load().orc_num_threads()}
+    scene.close()
+    return res
+
+
 def profile_mode(args, torch, gpurt, W, dev, ctx):
     """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
     rays_scene, then the same number of elementwise adds over 256 MiB (a plain streaming
@@ -448,6 +492,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched-vs-plain leg (for profiling it)")
+    ap.add_argument("--only-tracer", action="store_true", help="print only the path-tracer leg")
     ap.add_argument("--profile", action="store_true",
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -471,6 +516,9 @@ def main():
     if args.only_batched:
         print(json.dumps(batched_bench(args, torch, gpurt, W, dev, ctx)), flush=True)
         return
+    if args.only_tracer:
+        print(json.dumps(tracer_bench(args, torch, gpurt, W, O, dev, ctx)), flush=True)
+        return
     r = ray_bench(args, torch, gpurt, W, O, dev, ctx)
     extra = {"rays": r.pop("extra_rays")}
     if not args.no_extra:
@@ -478,6 +526,8 @@ def main():
         extra["cpq_small"] = cpq_bench("cpq_small", W.cpq_small(), args, torch, gpurt, dev, ctx, O, 1)
         log("[bench] instanced scene (TLAS)")
         extra["tlas"] = tlas_bench(args, torch, gpurt, W, O, dev, ctx)
+        log("[bench] wavefront path tracer")
+        extra["tracer"] = tracer_bench(args, torch, gpurt, W, O, dev, ctx)
         log("[bench] diff fwd / fwd+bwd")
         extra["diff"] = diff_bench(args, torch, gpurt, W, dev, ctx)
         if not args.no_batched:

@@ -257,6 +257,11 @@ int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_
 
 int launch_refit(gpurt_bvh_s* b, const float* verts, size_t vstride_floats, hipStream_t s);
 
+// host-side facts of a TLAS (tlas.hip): its context, and per instance the custom index
+// and the BLAS
+int tlas_host_info(gpurt_tlas t, gpurt_ctx* ctx, const std::vector<uint32_t>** custom,
+                   const std::vector<gpurt_bvh_s*>** blas);
+
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace gpurt_impl

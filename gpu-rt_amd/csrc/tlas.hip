@@ -348,6 +348,8 @@ struct gpurt_tlas_s {
     int width = 4;
     bool q8 = false;
     int max_blas_depth = 0;
+    std::vector<uint32_t> custom;       // custom index of every instance (host copy)
+    std::vector<gpurt_bvh_s*> blas;     // BLAS of every instance
     ~gpurt_tlas_s() {
         if (top) gpurt_bvh_destroy(top);
         if (inst_d) (void)hipFree(inst_d);
@@ -393,6 +395,19 @@ int blas_object_box(gpurt_bvh_s* b, double lo[3], double hi[3]) {
 }
 
 }  // namespace
+
+namespace gpurt_impl {
+
+int tlas_host_info(gpurt_tlas t, gpurt_ctx* ctx, const std::vector<uint32_t>** custom,
+                   const std::vector<gpurt_bvh_s*>** blas) {
+    if (!t) return set_error(GPURT_E_INVALID, "null tlas");
+    *ctx = t->ctx;
+    *custom = &t->custom;
+    *blas = &t->blas;
+    return GPURT_OK;
+}
+
+}  // namespace gpurt_impl
 
 extern "C" {
 
@@ -485,6 +500,12 @@ int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n, c
     T->width = W;
     T->q8 = q8;
     T->max_blas_depth = max_depth;
+    T->custom.resize(n);
+    T->blas.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        T->custom[i] = instances[i].custom_index;
+        T->blas[i] = instances[i].blas;
+    }
     float* vd = nullptr;
     uint32_t* id = nullptr;
     if (hipMalloc((void**)&vd, sizeof(float) * verts.size()) != hipSuccess ||

@@ -31,6 +31,7 @@ EXPORTS = (
     "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd", "gpurt_batch_create",
     "gpurt_batch_run", "gpurt_batch_set_grads", "gpurt_batch_run_backward", "gpurt_batch_destroy",
     "gpurt_tlas_build", "gpurt_tlas_destroy", "gpurt_tlas_trace_closest", "gpurt_tlas_trace_any",
+    "gpurt_scene_create", "gpurt_scene_destroy", "gpurt_render", "gpurt_scene_light_pdf",
     "gpurt_last_error",
 )
 
@@ -84,6 +85,22 @@ class BatchGrad(ctypes.Structure):
                 ("g_verts", ctypes.c_void_p)]
 
 
+class Object(ctypes.Structure):
+    _fields_ = [("verts", ctypes.c_void_p), ("indices", ctypes.c_void_p), ("n_verts", ctypes.c_uint32),
+                ("n_tris", ctypes.c_uint32), ("model", ctypes.c_float * 16), ("albedo", ctypes.c_float * 3),
+                ("emissive", ctypes.c_float * 3), ("metal_rough", ctypes.c_float * 2)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("inv_view", ctypes.c_float * 16), ("inv_proj", ctypes.c_float * 16),
+                ("clear_col", ctypes.c_float * 3), ("env_light", ctypes.c_float * 3),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("frame", ctypes.c_int32),
+                ("samples", ctypes.c_int32), ("max_frame", ctypes.c_int32), ("qmc", ctypes.c_int32),
+                ("max_depth", ctypes.c_int32), ("use_metalness", ctypes.c_int32), ("integrator", ctypes.c_int32),
+                ("brdf", ctypes.c_int32), ("use_rr", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("ray_counts", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -120,6 +137,10 @@ def load():
         "gpurt_tlas_trace_closest": (I, [P, P, S, P, P, P, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_tlas_trace_any": (I, [P, P, S, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_batch_destroy": (V, [P]),
+        "gpurt_scene_create": (I, [P, ctypes.POINTER(Object), S, P, ctypes.POINTER(P)]),
+        "gpurt_scene_destroy": (V, [P]),
+        "gpurt_render": (I, [P, ctypes.POINTER(RenderParams), P, P, P, P, P]),
+        "gpurt_scene_light_pdf": (I, [P, P, P, S, P, P]),
         "gpurt_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -337,3 +337,106 @@ def batched(n_meshes=256, tri_lo=1024, tri_hi=524288, q_lo=256, q_hi=262144):
             "n_tris_total": int(sum(m["tris"].shape[0] for m in meshes)),
             "n_queries_total": int(sum(m["queries"].shape[0] for m in meshes))}
     return {"meshes": meshes, "meta": meta}
+
+
+# ---- path-tracer scenes ----------------------------------------------------------------
+
+def _mesh_record(pos, nrm, tris):
+    """Mesh::Vertex records (n, 12): pos.xyz, u=0, normal.xyz, v=0, tangent 0."""
+    v = np.zeros((pos.shape[0], 12), np.float32)
+    v[:, 0:3] = pos
+    v[:, 4:7] = nrm
+    return v, np.ascontiguousarray(tris, np.uint32)
+
+
+def _quad():
+    """Unit quad in the xz plane (y = 0, x, z in [-1, 1]), normal +y, two triangles."""
+    p = np.array([(-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)], np.float64)
+    return _mesh_record(p, np.tile([0.0, 1.0, 0.0], (4, 1)), [(0, 2, 1), (0, 3, 2)])
+
+
+def _cube():
+    """Unit cube [-1, 1]^3 with flat per-face normals (24 vertices, 12 triangles)."""
+    pos, nrm, tri = [], [], []
+    for axis in range(3):
+        for s in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = s
+            u, w = (axis + 1) % 3, (axis + 2) % 3
+            base = len(pos)
+            for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                p = np.zeros(3)
+                p[axis], p[u], p[w] = s, a, b
+                pos.append(p)
+                nrm.append(n)
+            tri += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
+    return _mesh_record(np.array(pos), np.array(nrm), tri)
+
+
+def _sphere(level):
+    v, f = icosphere(level)
+    return _mesh_record(v, v, f)
+
+
+def _affine(scale=(1, 1, 1), rot_deg=(0, 0, 0), trans=(0, 0, 0)):
+    """Row-major 4x4 T * Rz * Ry * Rx * S."""
+    ax, ay, az = np.radians(rot_deg)
+    rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    rz = np.array([[np.cos(az), -np.sin(az), 0], [np.sin(az), np.cos(az), 0], [0, 0, 1]])
+    m = np.eye(4)
+    m[:3, :3] = rz @ ry @ rx @ np.diag(scale)
+    m[:3, 3] = trans
+    return m
+
+
+def _obj(mesh, model, albedo, emissive=(0, 0, 0), metal_rough=(0.0, 1.0), name=""):
+    v, f = mesh
+    return {"verts": v, "indices": f, "model": model, "albedo": np.asarray(albedo, np.float32),
+            "emissive": np.asarray(emissive, np.float32), "metal_rough": np.asarray(metal_rough, np.float32),
+            "name": name}
+
+
+def tracer_box(sphere_level=3):
+    """A closed Cornell-style box for the path tracer: five walls (quads placed by
+    rotations and non-uniform scales), a two-triangle light under the ceiling (emission
+    10), a glossy sphere (roughness 0.2), a rotated and sheared-scale block (roughness 1),
+    a mirror sphere (roughness 0) and metallic factors for use_metalness.  Camera at
+    (0, 0, 14) looking at the origin, 45 degree vertical field of view."""
+    q, c, s = _quad(), _cube(), _sphere(sphere_level)
+    objs = [
+        _obj(q, _affine((5, 1, 5), (0, 0, 0), (0, -5, 0)), (0.75, 0.75, 0.75), name="floor"),
+        _obj(q, _affine((5, 1, 5), (180, 0, 0), (0, 5, 0)), (0.75, 0.75, 0.75), name="ceiling"),
+        _obj(q, _affine((5, 1, 5), (90, 0, 0), (0, 0, -5)), (0.7, 0.7, 0.7), name="back"),
+        _obj(q, _affine((5, 1, 5), (0, 0, -90), (-5, 0, 0)), (0.75, 0.1, 0.1), name="left"),
+        _obj(q, _affine((5, 1, 5), (0, 0, 90), (5, 0, 0)), (0.1, 0.75, 0.1), name="right"),
+        _obj(q, _affine((1.5, 1, 1.2), (180, 0, 0), (0, 4.95, 0)), (0.0, 0.0, 0.0), emissive=(10, 10, 10),
+             metal_rough=(0.0, 0.0), name="light"),
+        _obj(s, _affine((1.6, 1.6, 1.6), (0, 30, 0), (-2.2, -3.4, -1.0)), (0.9, 0.8, 0.6), metal_rough=(0.5, 0.2),
+             name="glossy sphere"),
+        _obj(c, _affine((1.2, 2.5, 1.2), (0, 25, 0), (2.3, -2.5, -1.5)), (0.6, 0.6, 0.7), metal_rough=(0.0, 1.0),
+             name="block"),
+        _obj(s, _affine((1.0, 1.0, 1.0), (0, 0, 0), (1.5, -4.0, 2.5)), (0.95, 0.95, 0.95), metal_rough=(1.0, 0.0),
+             name="mirror sphere"),
+    ]
+    cam = {"eye": (0.0, 0.0, 14.0), "target": (0.0, 0.0, 0.0), "yfov_deg": 45.0}
+    return {"objects": objs, "camera": cam}
+
+
+def tracer_floor(occluder=True, roughness=0.5):
+    """An open scene whose every light path has at most one non-black bounce: a floor
+    (Blinn-Phong / GGX, `roughness`), an emitting quad above it and, optionally, a black
+    (albedo 0) block that casts a shadow.  Direct light sampling, material sampling and
+    MIS therefore estimate the same image (with env_light = 0)."""
+    q, c = _quad(), _cube()
+    objs = [
+        _obj(q, _affine((6, 1, 6), (0, 0, 0), (0, 0, 0)), (0.8, 0.7, 0.6), metal_rough=(0.0, roughness),
+             name="floor"),
+        _obj(q, _affine((1.5, 1, 1.0), (180, 20, 0), (0.5, 4.0, 0.0)), (0.0, 0.0, 0.0), emissive=(8, 8, 8),
+             metal_rough=(0.0, 0.0), name="light"),
+    ]
+    if occluder:
+        objs.append(_obj(c, _affine((0.6, 0.8, 0.6), (0, 30, 0), (0.3, 1.6, 0.2)), (0.0, 0.0, 0.0),
+                         metal_rough=(0.0, 1.0), name="black block"))
+    cam = {"eye": (0.0, 7.0, 9.0), "target": (0.0, 0.0, 0.0), "yfov_deg": 50.0}
+    return {"objects": objs, "camera": cam}

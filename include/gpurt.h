@@ -280,6 +280,83 @@ GPURT_API int gpurt_tlas_trace_closest(gpurt_tlas tlas, const gpurt_ray* rays, s
 GPURT_API int gpurt_tlas_trace_any(gpurt_tlas tlas, const gpurt_ray* rays, size_t n, uint8_t* occluded,
                                    const gpurt_query_opts* opts, void* stream);
 
+/* ---- wavefront path tracer ----------------------------------------------------------- */
+
+typedef struct gpurt_scene_s* gpurt_scene;
+
+/* One object of a rendered scene: the per-object record of RTPipe::build_desc
+ * (Scene_Desc, src/vk/rt.h:67-78, filled at src/vk/rt.cpp:26-45) with its mesh
+ * (Mesh::Vertex records, src/scene/mesh.h:16-22).  Textures are not supported: the
+ * albedo / emissive / metal_rough factors are used everywhere (rt.rgen:97-133 with every
+ * *_tex index -1).  Every object whose emissive factor is non-zero is a light, in object
+ * order (rt.cpp:47-60). */
+typedef struct gpurt_object {
+    const float* verts;        /* device, n_verts x 12 floats: pos.xyz, u, normal.xyz, v, tangent.xyzw */
+    const uint32_t* indices;   /* device, n_tris x 3 */
+    uint32_t n_verts;
+    uint32_t n_tris;
+    float model[16];           /* row-major object-to-world (scale * pose), affine */
+    float albedo[3];
+    float emissive[3];
+    float metal_rough[2];      /* metalness, roughness */
+} gpurt_object;
+
+/* Create a render scene over a TLAS whose instance custom indices address `objects`
+ * (custom index i = objects[i], as GPURT::build_accel and build_desc number them,
+ * gpurt.cpp:228-240).  Copies the meshes, derives the light list and builds a BVH over
+ * the world-space light triangles that answers light_pdf (rt.rgen:222-255) in
+ * O(log n) instead of looping over every light triangle.  The TLAS must outlive the
+ * scene.  Returns when the scene is complete. */
+GPURT_API int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_objects,
+                                 void* stream, gpurt_scene* out);
+GPURT_API void gpurt_scene_destroy(gpurt_scene scene);
+
+/* Per-frame constants: RTPipe_Constants (src/vk/rt.h:85-102) and the camera's inverse
+ * view / projection (CameraConstants, rt.cpp:121-127).  Matrices are row-major (M v). */
+typedef struct gpurt_render_params {
+    float inv_view[16];
+    float inv_proj[16];
+    float clear_col[3];   /* camera-ray miss colour */
+    float env_light[3];   /* radiance of a miss after the first bounce (env * env_scale) */
+    int32_t width;
+    int32_t height;
+    int32_t frame;        /* 0: overwrite the image; k > 0: image = mix(image, new, 1/(k+1)) */
+    int32_t samples;      /* paths per pixel this frame (>= 1) */
+    int32_t max_frame;    /* Hammersley sequence length per sample, for qmc */
+    int32_t qmc;          /* 1: Hammersley pixel jitter, 0: random (centre on frame 0) */
+    int32_t max_depth;    /* bounces per path (1..64) */
+    int32_t use_metalness;
+    int32_t integrator;   /* 0 direct light sampling, 1 material sampling, 2 MIS */
+    int32_t brdf;         /* 0 Blinn-Phong, 1 GGX */
+    int32_t use_rr;       /* Russian roulette after every bounce */
+    uint32_t seed;        /* per-frame RNG seed: pixel p starts from tea(p, seed) */
+    uint64_t* ray_counts; /* host, nullable: [0] += camera / bounce rays traced (closest hit),
+                             [1] += shadow (any hit) or light (closest hit) rays traced */
+} gpurt_render_params;
+
+/* Render one frame of `samples` paths per pixel into image (height x width x 4 floats,
+ * device, row y = launch row y; .w = 1), the wavefront form of rt.rgen main()
+ * (rt.rgen:611-677) with integrate_direct / integrate_mats / integrate_mis
+ * (rt.rgen:303-414), Blinn-Phong and GGX (rtcommon.glsl:267-369) and the reference's RNG
+ * (tea / lcg, rtcommon.glsl:99-124) consumed in the reference's order per pixel.  Each
+ * bounce is a closest-hit launch over the live paths, a shading launch that writes the
+ * next rays and the shadow (direct: any hit) or light (MIS: closest hit) rays in
+ * compacted order, and the shadow launch.  The G-buffer images (nullable, same layout)
+ * receive the first sample's first hit position, shading normal and albedo (0 on a
+ * miss).  The host waits on `stream` once per bounce to size the next launch.  The
+ * reference's clockARB() seeding is replaced by `seed`.  Not supported: textures,
+ * normal maps, ReSTIR (integrators 3, 4) and debug views. */
+GPURT_API int gpurt_render(gpurt_scene scene, const gpurt_render_params* params, float* image,
+                           float* gbuf_pos, float* gbuf_norm, float* gbuf_albedo, void* stream);
+
+/* light_pdf(p, d) of rt.rgen:222-255 for n (point, unit direction) pairs (device n x 3
+ * each): the sum over light triangles crossed by the ray at t >= 0 (triangle_hit,
+ * rtcommon.glsl:181-202) of triangle_pdf (2 / |cross| * dist^2 / |cos|, lines 204-214),
+ * each divided by its light's triangle count, over the light count; found through the
+ * scene's light BVH. */
+GPURT_API int gpurt_scene_light_pdf(gpurt_scene scene, const float* p, const float* d, size_t n,
+                                    float* pdf, void* stream);
+
 /* ---- errors ----------------------------------------------------------------------- */
 
 /* Message of the last failing call on this thread ("" if none). */

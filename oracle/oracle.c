@@ -102,6 +102,10 @@ static void parallel_for(int64_t n, int64_t chunk, range_fn fn, void* ctx) {
     for (int i = 0; i < started; i++) pthread_join(th[i], NULL);
 }
 
+/* the same, for pathtrace.c */
+typedef void (*orc_range_fn)(void* ctx, int64_t lo, int64_t hi);
+void orc_parallel_for(int64_t n, int64_t chunk, orc_range_fn fn, void* ctx) { parallel_for(n, chunk, fn, ctx); }
+
 /* ------------------------------------------------------------------------------------ */
 /* FP32 restatement: per-primitive arithmetic                                           */
 /* ------------------------------------------------------------------------------------ */

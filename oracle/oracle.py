@@ -67,6 +67,18 @@ def load():
     lib.orc_lbvh_cpq.argtypes = [P, _f32p, I64, _i32p, P, P, P, P]
     lib.orc_lbvh_ray_closest.argtypes = [P, _f32p, I64, _i32p, _f32p, P, P]
     lib.orc_lbvh_ray_any.argtypes = [P, _f32p, I64, _u8p, P]
+    U32 = ctypes.c_uint32
+    lib.orc_tea_rounds.argtypes = [U32, U32, _u32p, I]
+    lib.orc_tea_rounds.restype = U32
+    lib.orc_tea.argtypes = [U32, U32]
+    lib.orc_tea.restype = U32
+    lib.orc_randf_seq.argtypes = [U32, I64, _f32p]
+    lib.orc_mat_samples.argtypes = [I, _f32p, ctypes.c_float, _f32p, _f32p, U32, I64, _f32p]
+    lib.orc_mat_eval.argtypes = [I, _f32p, ctypes.c_float, _f32p, _f32p, _f32p, I64, _f32p]
+    lib.orc_render.argtypes = [I64, _f32p, _i64p, _f32p, _f32p, _i64p, _u32p, _f32p, _f32p, _i32p, _f32p, P, P, P]
+    lib.orc_render.restype = I
+    lib.orc_light_pdf.argtypes = [I64, _f32p, _i64p, _f32p, _i64p, _u32p, _f32p, _f32p, _f32p, I64, _f32p]
+    lib.orc_light_pdf.restype = I
     _lib = lib
     return lib
 
@@ -288,3 +300,87 @@ class CpuBvh:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_lbvh_free(self.h)
             self.h = None
+
+
+# ---- path tracer (pathtrace.c) ---------------------------------------------------------
+
+def tea(v0, v1, key=None, rounds=16):
+    """tea of rtcommon.glsl:99-111; key / rounds default to the reference's."""
+    if key is None:
+        return int(load().orc_tea(v0 & 0xFFFFFFFF, v1 & 0xFFFFFFFF))
+    return int(load().orc_tea_rounds(v0 & 0xFFFFFFFF, v1 & 0xFFFFFFFF, _c(key, np.uint32), rounds))
+
+
+def randf_seq(seed, n):
+    out = np.empty(n, np.float32)
+    load().orc_randf_seq(seed & 0xFFFFFFFF, n, out)
+    return out
+
+
+def mat_samples(brdf, albedo, roughness, wo, N, seed, n):
+    """n draws of MAT_sample at the frame of N: (ok, wi (n,3), pdf(wi), eval(wi) (n,3))."""
+    out = np.empty((n, 8), np.float32)
+    load().orc_mat_samples(brdf, _c(albedo, np.float32), roughness, _c(wo, np.float32), _c(N, np.float32),
+                           seed & 0xFFFFFFFF, n, out)
+    return out[:, 0] > 0, out[:, 1:4], out[:, 4], out[:, 5:8]
+
+
+def mat_eval(brdf, albedo, roughness, wo, N, wi):
+    wi = _c(wi, np.float32).reshape(-1, 3)
+    out = np.empty((wi.shape[0], 4), np.float32)
+    load().orc_mat_eval(brdf, _c(albedo, np.float32), roughness, _c(wo, np.float32), _c(N, np.float32), wi,
+                        wi.shape[0], out)
+    return out[:, 0], out[:, 1:4]
+
+
+def _tracer_scene(objects):
+    """objects: dicts with verts (n, 12) Mesh::Vertex floats, indices (m, 3), model (4, 4)
+    row-major object-to-world, albedo, emissive, metal_rough."""
+    model = _c(np.stack([np.asarray(o["model"], np.float32).reshape(4, 4)[:3].reshape(12) for o in objects]),
+               np.float32)
+    vs = [np.asarray(o["verts"], np.float32).reshape(-1, 12) for o in objects]
+    fs = [np.asarray(o["indices"], np.uint32).reshape(-1, 3) for o in objects]
+    voff = _c(np.concatenate([[0], np.cumsum([v.shape[0] for v in vs])]), np.int64)
+    foff = _c(np.concatenate([[0], np.cumsum([f.shape[0] for f in fs])]), np.int64)
+    V = _c(np.concatenate([v[:, 0:3] for v in vs]), np.float32)
+    N = _c(np.concatenate([v[:, 4:7] for v in vs]), np.float32)
+    F = _c(np.concatenate(fs), np.uint32)
+    mat = _c(np.stack([np.concatenate([np.asarray(o["albedo"], np.float32).reshape(3),
+                                       np.asarray(o["emissive"], np.float32).reshape(3),
+                                       np.asarray(o["metal_rough"], np.float32).reshape(2)]) for o in objects]),
+             np.float32)
+    return len(objects), model, voff, V, N, foff, F, mat
+
+
+_RENDER_INT = ("width", "height", "frame", "samples", "max_frame", "qmc", "max_depth", "use_metalness",
+               "integrator", "brdf", "use_rr", "seed")
+
+
+def render(objects, params, image=None, gbuffer=False):
+    """One frame of the reference's path tracer on the CPU (pathtrace.c).  params: dict with
+    inv_view, inv_proj (4x4 row-major), clear_col, env_light and the integer fields of
+    _RENDER_INT.  Returns image (h, w, 4) [, (pos, norm, albedo)]."""
+    n, model, voff, V, N, foff, F, mat = _tracer_scene(objects)
+    fp = _c(np.concatenate([np.asarray(params["inv_view"], np.float32).reshape(16),
+                            np.asarray(params["inv_proj"], np.float32).reshape(16),
+                            np.asarray(params["clear_col"], np.float32).reshape(3),
+                            np.asarray(params["env_light"], np.float32).reshape(3)]), np.float32)
+    ip = _c(np.array([int(params[k]) & 0xFFFFFFFF if k == "seed" else int(params[k]) for k in _RENDER_INT],
+                     np.int64).astype(np.uint32).view(np.int32), np.int32)
+    h, w = int(params["height"]), int(params["width"])
+    img = np.zeros((h, w, 4), np.float32) if image is None else _c(image, np.float32).copy()
+    g = [np.zeros((h, w, 4), np.float32) for _ in range(3)] if gbuffer else [None] * 3
+    if not load().orc_render(n, model, voff, V, N, foff, F, mat, fp, ip, img, _p(g[0]), _p(g[1]), _p(g[2])):
+        raise ValueError("singular model matrix")
+    return (img, tuple(g)) if gbuffer else img
+
+
+def light_pdf(objects, P, D):
+    """light_pdf (rt.rgen:222-255) for (point, direction) pairs, looping over every light
+    triangle as the reference does."""
+    n, model, voff, V, N, foff, F, mat = _tracer_scene(objects)
+    P, D = _c(P, np.float32).reshape(-1, 3), _c(D, np.float32).reshape(-1, 3)
+    out = np.empty(P.shape[0], np.float32)
+    if not load().orc_light_pdf(n, model, voff, V, foff, F, mat, P, D, P.shape[0], out):
+        raise ValueError("singular model matrix")
+    return out

@@ -64,7 +64,7 @@ def node_world(doc, ni, parent):
     return parent @ M
 
 
-def scene_objects(path):
+def scene_objects(path, shading=False):
     import base64
     doc = json.load(open(path))
     bufs = []
@@ -82,7 +82,17 @@ def scene_objects(path):
                 assert p.get("mode", 4) == 4
                 pos = np.array(read_accessor(doc, bufs, p["attributes"]["POSITION"]), np.float32)
                 idx = np.array([t[0] for t in read_accessor(doc, bufs, p["indices"])], np.int64).reshape(-1, 3)
-                objs.append((pos, idx, W))
+                if shading:
+                    nrm = np.array(read_accessor(doc, bufs, p["attributes"]["NORMAL"]), np.float32)
+                    mats = doc.get("materials", [])
+                    m = mats[p["material"]] if "material" in p else {}
+                    pbr = m.get("pbrMetallicRoughness", {})
+                    mat = (np.array(pbr.get("baseColorFactor", [1, 1, 1, 1])[:3], np.float32),
+                           np.array(m.get("emissiveFactor", [0, 0, 0]), np.float32),
+                           np.array([pbr.get("metallicFactor", 1.0), pbr.get("roughnessFactor", 1.0)], np.float32))
+                    objs.append((pos, idx, W, nrm, mat))
+                else:
+                    objs.append((pos, idx, W))
         for c in n.get("children", []):
             visit(c, W)
 
@@ -135,7 +145,38 @@ def main(media):
         print(name, len(objs), "objects", int(Fw.shape[0]), "triangles; hit rate", float((rp >= 0).mean()))
     with open(os.path.join(HERE, "gltf_facts.json"), "w") as f:
         json.dump(facts, f, indent=1)
+    shading_fixtures(media)
+
+
+# Cameras for rendering: cbox's own camera node (position (0, 0, 19.99), looking at the
+# origin, yfov 0.7175 rad, aspect 4/3); mis_test has none, so a view of its four plates.
+CAMERAS = {"cbox": {"eye": [0.0, 0.0, 19.992091970966317], "target": [0.0, 0.0, 0.0],
+                    "yfov_deg": float(np.degrees(0.7175414562225342)), "aspect": 4.0 / 3.0},
+           "mis_test": {"eye": [-2.5, 1.2, 0.0], "target": [0.5, 0.3, 0.0], "yfov_deg": 45.0, "aspect": 4.0 / 3.0}}
+
+
+def shading_fixtures(media):
+    """gltf_<name>_shading.npz: per object the vertex normals and the material factors
+    (baseColorFactor rgb, emissiveFactor, metallic / roughness factors with the glTF
+    defaults 1, 0, 1, 1), and a camera -- what the path tracer needs beyond gltf_<name>.npz."""
+    for name, rel in [("mis_test", "mis_test/mis_test.gltf"), ("cbox", "cbox/cbox.gltf")]:
+        objs = scene_objects(os.path.join(media, rel), shading=True)
+        arrays = {"albedo": np.stack([o[4][0] for o in objs]), "emissive": np.stack([o[4][1] for o in objs]),
+                  "metal_rough": np.stack([o[4][2] for o in objs])}
+        for k, o in enumerate(objs):
+            arrays[f"obj{k}_nrm"] = o[3]
+        cam = CAMERAS[name]
+        for key in ("eye", "target"):
+            arrays["camera_" + key] = np.array(cam[key], np.float64)
+        arrays["camera_yfov_deg"] = np.float64(cam["yfov_deg"])
+        arrays["camera_aspect"] = np.float64(cam["aspect"])
+        np.savez_compressed(os.path.join(HERE, f"gltf_{name}_shading.npz"), **arrays)
+        print(name, "shading fixture:", len(objs), "objects,", int((arrays["emissive"] > 0).any(1).sum()), "lights")
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/media")
+    media = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/media"
+    if len(sys.argv) > 2 and sys.argv[2] == "--shading-only":
+        shading_fixtures(media)
+    else:
+        main(media)

@@ -47,7 +47,9 @@ def _c_layout(struct, fields):
                                         (_lib.BvhInfo, "gpurt_bvh_info"),
                                         (_lib.BatchMesh, "gpurt_batch_mesh"),
                                         (_lib.BatchGrad, "gpurt_batch_grad"),
-                                        (_lib.Instance, "gpurt_instance")])
+                                        (_lib.Instance, "gpurt_instance"),
+                                        (_lib.Object, "gpurt_object"),
+                                        (_lib.RenderParams, "gpurt_render_params")])
 def test_struct_layouts_match_header(cls, struct):
     fields = [f for f, _ in cls._fields_]
     size, offs = _c_layout(struct, fields)
@@ -68,6 +70,10 @@ def test_null_arguments_return_invalid_without_device_work():
     assert lib.gpurt_trace_bwd(None, None, 1, None, None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_bvh_get_info(None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_ctx_create(0, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_scene_create(None, None, 1, None, ctypes.byref(out)) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_render(None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_scene_light_pdf(None, None, None, 1, None, None) == _lib.GPURT_E_INVALID
+    lib.gpurt_scene_destroy(None)
     lib.gpurt_bvh_destroy(None)
     lib.gpurt_ctx_destroy(None)
 

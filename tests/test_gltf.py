@@ -34,6 +34,14 @@ def test_reference_scenes_match_independent_reader(name, rel):
         assert np.array_equal(o["verts"][:, :3].max(0), np.float32(f["pos_max"]))
         assert np.array_equal(o["verts"][:, :3], fx[f"obj{k}_pos"]) and np.array_equal(o["indices"], fx[f"obj{k}_tris"])
         assert np.array_equal(o["transform"][:3].astype(np.float32).reshape(12), fx["xf"][k])
+    shading = os.path.join(GOLDEN, f"gltf_{name}_shading.npz")
+    if os.path.exists(shading):  # normals and material factors (the path tracer's inputs)
+        sh = np.load(shading)
+        for k, o in enumerate(objs):
+            assert np.array_equal(o["verts"][:, 4:7], sh[f"obj{k}_nrm"])
+            m = o["material"]
+            assert np.array_equal(m["albedo"], sh["albedo"][k]) and np.array_equal(m["emissive"], sh["emissive"][k])
+            assert np.array_equal(m["metal_rough"], sh["metal_rough"][k])
 
 
 def _synthetic(tmp_path, prims, positions, extra_attrs=None):
