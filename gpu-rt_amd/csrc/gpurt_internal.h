@@ -240,6 +240,10 @@ int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need
                    size_t n, TravPlan& p, int waves_per_simd = 0);
 int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t);
 void free_traversal_scratch(TravScratch& t, hipStream_t s);
+// the same scratch in memory the caller holds
+size_t traversal_scratch_bytes(const TravPlan& p);
+void carve_traversal_scratch(const TravPlan& p, void* mem, TravScratch& t);
+void clear_traversal_counters(const TravScratch& t, hipStream_t s);
 
 // traversal launchers (query.hip)
 int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,
@@ -261,6 +265,13 @@ int launch_refit(gpurt_bvh_s* b, const float* verts, size_t vstride_floats, hipS
 // and the BLAS
 int tlas_host_info(gpurt_tlas t, gpurt_ctx* ctx, const std::vector<uint32_t>** custom,
                    const std::vector<gpurt_bvh_s*>** blas);
+// TLAS traversal for wavefront callers (tlas.hip): the ray count may be read from device
+// memory at launch (n_dev, at most n) and the traversal scratch may be the caller's
+// (scratch of >= tlas_scratch_bytes(T, n) bytes; nullptr: allocated per call)
+int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t* n_dev, int32_t* obj,
+                  int32_t* prim, float* t, float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts,
+                  hipStream_t s, void* scratch, size_t scratch_bytes);
+size_t tlas_scratch_bytes(gpurt_tlas T, size_t n);
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 

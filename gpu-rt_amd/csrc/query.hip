@@ -173,19 +173,33 @@ int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need
     return GPURT_OK;
 }
 
+static constexpr size_t kCounterBytes = (size_t)kQueues * kQueueStride * sizeof(uint32_t);
+
+// the queue counters and the stack entries beyond the LDS slots
+size_t traversal_scratch_bytes(const TravPlan& p) {
+    return kCounterBytes + (size_t)p.spill_slots * p.total * sizeof(uint2);
+}
+
+void carve_traversal_scratch(const TravPlan& p, void* mem, TravScratch& t) {
+    t.mem = mem;
+    t.counters = (uint32_t*)mem;
+    t.spill = p.spill_slots ? (uint2*)((char*)mem + kCounterBytes) : nullptr;
+}
+
+void clear_traversal_counters(const TravScratch& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, t.counters);
+}
+
 int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t) {
-    // one stream-ordered allocation owned by the call: the queue counters and the stack
-    // entries beyond the LDS slots
-    constexpr size_t kCounterBytes = (size_t)kQueues * kQueueStride * sizeof(uint32_t);
-    const size_t bytes = kCounterBytes + (size_t)p.spill_slots * p.total * sizeof(uint2);
-    if (hipMallocAsync(&t.mem, bytes, s) != hipSuccess) {
+    // one stream-ordered allocation owned by the call
+    void* mem = nullptr;
+    if (hipMallocAsync(&mem, traversal_scratch_bytes(p), s) != hipSuccess) {
         (void)hipGetLastError();
         t.mem = nullptr;
         return set_error(GPURT_E_OOM, "traversal scratch allocation failed");
     }
-    t.counters = (uint32_t*)t.mem;
-    t.spill = p.spill_slots ? (uint2*)((char*)t.mem + kCounterBytes) : nullptr;
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, t.counters);
+    carve_traversal_scratch(p, mem, t);
+    clear_traversal_counters(t, s);
     return GPURT_OK;
 }
 

@@ -246,7 +246,8 @@ struct TlasArgs {
     const float4* top_tris;
     const InstDev* inst;
     const float4* rays;
-    uint32_t n;
+    uint32_t n;             // rays (or, with n_dev, the capacity of the ray array)
+    const uint32_t* n_dev;  // nullable: the ray count, read from device memory at launch
     int32_t* obj;
     int32_t* prim;
     float* t;
@@ -267,7 +268,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
-    QueueSource src{a.rays, nullptr, a.n, gid, total, a.counter};
+    const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
+    QueueSource src{a.rays, nullptr, n, gid, total, a.counter};
     auto out = [&](uint32_t ri, uint32_t obj, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
             a.occ[ri] = any ? 1 : 0;
@@ -548,8 +550,31 @@ void gpurt_tlas_destroy(gpurt_tlas t) {
     delete t;
 }
 
-static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t,
-                      float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, void* stream) {
+}  // extern "C"
+
+namespace gpurt_impl {
+
+static int tlas_plan(gpurt_tlas T, size_t n, bool any, const gpurt_query_opts* opts, TravPlan& p) {
+    const int W = T->width;
+    // worst-case stack: the top levels' entries plus one BLAS's
+    const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
+    return plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
+                          W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
+}
+
+size_t tlas_scratch_bytes(gpurt_tlas T, size_t n) {
+    size_t bytes = 0;
+    for (bool any : {false, true}) {
+        TravPlan p;
+        if (tlas_plan(T, n, any, nullptr, p)) return 0;
+        bytes = std::max(bytes, traversal_scratch_bytes(p));
+    }
+    return bytes;
+}
+
+int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t* n_dev, int32_t* obj,
+                  int32_t* prim, float* t, float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts,
+                  hipStream_t stream, void* scratch, size_t scratch_bytes) {
     if (!T) return set_error(GPURT_E_INVALID, "null tlas");
     if (n == 0) return GPURT_OK;
     if (!rays || (any ? !occ : (!obj || !prim || !t))) return set_error(GPURT_E_INVALID, "null ray/output pointer");
@@ -558,22 +583,27 @@ static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* ob
         return set_error(GPURT_E_UNSUPPORTED, "query options out of range");
     TlasDevGuard dg(T->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = stream;
     const int W = T->width;
-    // worst-case stack: the top levels' entries plus one BLAS's
-    const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
     TravPlan p;
-    int rc = plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
-                            W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
+    int rc = tlas_plan(T, n, any, opts, p);
     if (rc) return rc;
     TravScratch sc;
-    if ((rc = alloc_traversal_scratch(p, s, sc))) return rc;
+    if (scratch) {  // the caller's buffer: clear the queue counters, keep the memory
+        if (scratch_bytes < traversal_scratch_bytes(p))
+            return set_error(GPURT_E_INVALID, "traversal scratch buffer too small");
+        carve_traversal_scratch(p, scratch, sc);
+        clear_traversal_counters(sc, s);
+    } else if ((rc = alloc_traversal_scratch(p, s, sc))) {
+        return rc;
+    }
     TlasArgs a{};
     a.top_nodes = T->q8 ? T->top->qnodes : T->top->nodes;
     a.top_tris = T->top->tris;
     a.inst = T->inst_d;
     a.rays = reinterpret_cast<const float4*>(rays);
     a.n = (uint32_t)n;
+    a.n_dev = n_dev;
     a.obj = obj;
     a.prim = prim;
     a.t = t;
@@ -603,10 +633,19 @@ static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* ob
         default: break;
     }
     hipError_t le = hipGetLastError();
-    free_traversal_scratch(sc, s);
+    if (!scratch) free_traversal_scratch(sc, s);
     if (!launched) return set_error(GPURT_E_UNSUPPORTED, "node width / stack depth");
     if (le != hipSuccess) return hip_error(le, "TLAS traversal launch");
     return GPURT_OK;
+}
+
+}  // namespace gpurt_impl
+
+extern "C" {
+
+static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t,
+                      float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, void* stream) {
+    return tlas_trace_ex(T, rays, n, nullptr, obj, prim, t, uv, occ, any, opts, (hipStream_t)stream, nullptr, 0);
 }
 
 int gpurt_tlas_trace_closest(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t,

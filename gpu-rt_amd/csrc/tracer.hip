@@ -21,7 +21,9 @@
 // random numbers for the same seed.  light_pdf (rt.rgen:222-255) is answered by a BVH over
 // the world-space light triangles instead of a loop over all of them.  Path state lives
 // in HBM as SoA float4 records indexed by pixel; rays are compacted, so each launch
-// covers only live paths.
+// covers only live paths.  The list lengths stay on the device: every launch reads its
+// count there (the traversal's persistent blocks and the shading blocks past the count
+// exit at once), so a frame is enqueued without any host synchronisation.
 #include <cmath>
 #include <string>
 #include <vector>
@@ -36,6 +38,7 @@ using pt::kEps;
 using pt::kLargeDist;
 
 constexpr int kShadeBlock = 256;
+constexpr int kCountSlots = 2 * 64 + 2;  // two list lengths per bounce, up to 64 bounces
 constexpr int kLightStack = 64;  // light BVH traversal stack (checked against its depth)
 constexpr float kExitSlack = 0x1p-20f;  // relative slack on box exits (conservative culls)
 
@@ -238,7 +241,7 @@ struct WaveArgs {
     SceneDev sc;
     float clear[3], env[3];
     int depth, max_depth, sample, use_rr, use_metalness;
-    uint32_t n;
+    const uint32_t* n_in;  // live paths (rays_in / ids_in entries)
     const uint32_t* ids_in;
     const float4* rays_in;
     const int32_t* hobj;
@@ -252,7 +255,8 @@ struct WaveArgs {
     float4* sh_rays;
     uint32_t* sh_ids;
     float4* sh_contrib;
-    uint32_t* counters;  // [0] next rays, [1] shadow / light rays
+    uint32_t* n_next;    // appended next rays
+    uint32_t* n_shadow;  // appended shadow / light rays
     float4* gpos;
     float4* gnorm;
     float4* galb;
@@ -262,12 +266,14 @@ struct WaveArgs {
 // (integrate_mats), 2 MIS (integrate_mis); BRDF 0 Blinn-Phong, 1 GGX.
 template <int INTEG, int BRDF>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a) {
+    const uint32_t n = *a.n_in;
+    if (blockIdx.x * kShadeBlock >= n) return;  // whole block past the live paths
     const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
     bool next = false, shadow = false;
     V3 npos = v3(0.f, 0.f, 0.f), ndir = npos, sdir = npos, contrib = npos;
     float stmax = 0.0f;
     uint32_t pid = 0;
-    if (i < a.n) {
+    if (i < n) {
         pid = a.ids_in[i];
         const float4 r1 = a.rays_in[2 * (size_t)i + 1];
         const V3 d = v3(r1.x, r1.y, r1.z);
@@ -426,8 +432,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a) {
         a.accs[pid] = make_float4(acc.x, acc.y, acc.z, 0.f);
     }
     // shadow rays start at the hit point too
-    const uint32_t ns = wave_append(next, a.counters);
-    const uint32_t ss = wave_append(shadow, a.counters + 1);
+    const uint32_t ns = wave_append(next, a.n_next);
+    const uint32_t ss = wave_append(shadow, a.n_shadow);
     if (next) {
         a.rays_out[2 * (size_t)ns] = make_float4(npos.x, npos.y, npos.z, kEps);
         a.rays_out[2 * (size_t)ns + 1] = make_float4(ndir.x, ndir.y, ndir.z, kLargeDist);
@@ -442,11 +448,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WaveArgs a) {
 }
 
 // direct lighting: acc += contrib * (visible ? 1 : 0)
-__global__ __launch_bounds__(kBlock) void k_resolve_shadow(uint32_t n, const uint32_t* __restrict__ ids,
+__global__ __launch_bounds__(kBlock) void k_resolve_shadow(const uint32_t* __restrict__ n_dev,
+                                                           const uint32_t* __restrict__ ids,
                                                            const float4* __restrict__ contrib,
                                                            const uint8_t* __restrict__ occ, float4* accs) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *n_dev) return;
     const uint32_t pid = ids[i];
     const float sh = occ[i] ? 0.0f : 1.0f;
     const float4 c = contrib[i];
@@ -458,13 +465,14 @@ __global__ __launch_bounds__(kBlock) void k_resolve_shadow(uint32_t n, const uin
 }
 
 // MIS light rays: acc += contrib * (emissive of the closest hit, or the environment)
-__global__ __launch_bounds__(kBlock) void k_resolve_light(uint32_t n, const uint32_t* __restrict__ ids,
+__global__ __launch_bounds__(kBlock) void k_resolve_light(const uint32_t* __restrict__ n_dev,
+                                                          const uint32_t* __restrict__ ids,
                                                           const float4* __restrict__ contrib,
                                                           const int32_t* __restrict__ obj,
                                                           const ObjDev* __restrict__ objs, float ex, float ey,
                                                           float ez, float4* accs) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *n_dev) return;
     const uint32_t pid = ids[i];
     const int32_t o = obj[i];
     float lx = ex, ly = ey, lz = ez;
@@ -550,8 +558,22 @@ __global__ __launch_bounds__(kBlock) void k_check_indices(size_t n, const uint32
     if (idx[i] >= n_verts) gatomic_or(bad, 1u);
 }
 
-__global__ void k_zero2(uint32_t* c) {
-    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+// per-sample list lengths: counts[2d] = rays of bounce d (bounce 0: every pixel),
+// counts[2d + 1] = shadow / light rays of bounce d
+__global__ void k_sample_counts(uint32_t* counts, uint32_t n_entries, uint32_t P) {
+    for (uint32_t k = threadIdx.x; k < n_entries; k += blockDim.x) counts[k] = k == 0 ? P : 0u;
+}
+
+// ray statistics: tally[0] += bounce rays, tally[1] += shadow / light rays of one sample
+__global__ void k_tally(const uint32_t* counts, uint32_t depth, unsigned long long* tally) {
+    if (threadIdx.x != 0) return;
+    unsigned long long b = 0, sh = 0;
+    for (uint32_t d = 0; d < depth; d++) {
+        b += counts[2 * d];
+        sh += counts[2 * d + 1];
+    }
+    tally[0] += b;
+    tally[1] += sh;
 }
 
 struct DevScope {
@@ -601,17 +623,18 @@ struct gpurt_scene_s {
     float4* lworld = nullptr;
     float* ltri_n = nullptr;
     gpurt_bvh lbvh = nullptr;
-    // wavefront state, grown on demand
+    // wavefront state and traversal scratch, grown on demand
     size_t cap = 0;
     void* wf = nullptr;
-    uint32_t* counters = nullptr;     // device [2]
-    uint32_t* host_counts = nullptr;  // pinned [2]
+    void* trav = nullptr;
+    size_t trav_bytes = 0;
+    uint32_t* counts = nullptr;            // device: list lengths of one sample (kCountSlots)
+    unsigned long long* tally = nullptr;   // device [2]: rays traced (ray_counts)
     ~gpurt_scene_s() {
         if (lbvh) gpurt_bvh_destroy(lbvh);
-        for (void* p : {(void*)objs, (void*)verts, (void*)idx, (void*)lights, (void*)lworld, (void*)ltri_n, wf,
-                        (void*)counters})
+        for (void* p : {(void*)objs, (void*)verts, (void*)idx, (void*)lights, (void*)lworld, (void*)ltri_n, wf, trav,
+                        (void*)counts, (void*)tally})
             if (p) (void)hipFree(p);
-        if (host_counts) (void)hipHostFree(host_counts);
     }
     SceneDev dev() const {
         SceneDev s{};
@@ -674,8 +697,8 @@ size_t carve(void* base, size_t P, Wave* w) {
 }
 
 template <int I, int B>
-void launch_shade(const WaveArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_shade<I, B>), dim3(cdiv(a.n, kShadeBlock)), dim3(kShadeBlock), 0, s, a);
+void launch_shade(const WaveArgs& a, uint32_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL((k_shade<I, B>), dim3(cdiv(capacity, kShadeBlock)), dim3(kShadeBlock), 0, s, a);
 }
 
 }  // namespace
@@ -772,9 +795,9 @@ int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_ob
     if (hipMalloc((void**)&S->objs, sizeof(ObjDev) * n_objects) != hipSuccess ||
         hipMalloc((void**)&S->verts, 48 * std::max<uint64_t>(nv, 1)) != hipSuccess ||
         hipMalloc((void**)&S->idx, 4 * std::max<uint64_t>(ni, 1)) != hipSuccess ||
-        hipMalloc((void**)&S->counters, 2 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&S->counts, kCountSlots * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&S->tally, 2 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&bad, sizeof(uint32_t)) != hipSuccess ||
-        hipHostMalloc((void**)&S->host_counts, 2 * sizeof(uint32_t)) != hipSuccess ||
         (nl && (hipMalloc((void**)&S->lights, sizeof(LightDev) * lights.size()) != hipSuccess ||
                 hipMalloc((void**)&S->lworld, 48 * nl) != hipSuccess ||
                 hipMalloc((void**)&S->ltri_n, 4 * nl) != hipSuccess ||
@@ -880,12 +903,18 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     const uint32_t P = (uint32_t)r.width * (uint32_t)r.height;
     if (P > S->cap) {
         GPURT_HIP(hipStreamSynchronize(s));
-        if (S->wf) (void)hipFree(S->wf);
-        S->wf = nullptr;
+        for (void** q : {&S->wf, &S->trav})
+            if (*q) (void)hipFree(*q);
+        S->wf = S->trav = nullptr;
         S->cap = 0;
-        if (hipMalloc(&S->wf, carve(nullptr, P, nullptr)) != hipSuccess) {
+        S->trav_bytes = tlas_scratch_bytes(S->tlas, P);
+        if (S->trav_bytes == 0) return set_error(GPURT_E_INVALID, "cannot plan the traversal");
+        if (hipMalloc(&S->wf, carve(nullptr, P, nullptr)) != hipSuccess ||
+            hipMalloc(&S->trav, S->trav_bytes) != hipSuccess) {
             (void)hipGetLastError();
-            S->wf = nullptr;
+            for (void** q : {&S->wf, &S->trav})
+                if (*q) (void)hipFree(*q);
+            S->wf = S->trav = nullptr;
             return set_error(GPURT_E_OOM, "wavefront state allocation failed");
         }
         S->cap = P;
@@ -895,6 +924,7 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     float4* img = (float4*)image;
     hipLaunchKernelGGL(k_frame_init, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, r.seed, w.seeds, w.pix,
                        (float4*)gbuf_pos, (float4*)gbuf_norm, (float4*)gbuf_albedo);
+    if (r.ray_counts) GPURT_HIP(hipMemsetAsync(S->tally, 0, 2 * sizeof(unsigned long long), s));
     CamArgs c{};
     for (int k = 0; k < 16; k++) {
         c.iv[k] = r.inv_view[k];
@@ -924,67 +954,64 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     a.sh_rays = w.sh_rays;
     a.sh_ids = w.sh_ids;
     a.sh_contrib = w.sh_contrib;
-    a.counters = S->counters;
     a.gpos = (float4*)gbuf_pos;
     a.gnorm = (float4*)gbuf_norm;
     a.galb = (float4*)gbuf_albedo;
+    // integrate_direct ends every path at its first hit: one bounce launch suffices
+    const int bounces = r.integrator == 0 ? std::min(r.max_depth, 1) : r.max_depth;
+    const uint32_t n_counts = 2u * (uint32_t)r.max_depth + 2u;
     int rc = GPURT_OK;
     for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
         c.sample = (uint32_t)smp;
+        hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, S->counts, n_counts, P);
         hipLaunchKernelGGL(k_camera, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, c, w.seeds, w.rays[0], w.ids[0],
                            w.thrmis, w.accs);
-        uint32_t n = P;
         int cur = 0;
-        for (int depth = 0; depth < r.max_depth && n > 0; depth++) {
-            rc = gpurt_tlas_trace_closest(S->tlas, (const gpurt_ray*)w.rays[cur], n, w.hobj, w.hprim, w.ht, w.huv,
-                                          nullptr, stream);
+        for (int depth = 0; depth < bounces; depth++) {
+            const uint32_t* n_in = S->counts + 2 * depth;
+            uint32_t* n_sh = S->counts + 2 * depth + 1;
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.rays[cur], P, n_in, w.hobj, w.hprim, w.ht, w.huv,
+                               nullptr, false, nullptr, s, S->trav, S->trav_bytes);
             if (rc) break;
-            hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, s, S->counters);
             a.depth = depth;
             a.sample = smp;
-            a.n = n;
+            a.n_in = n_in;
+            a.n_next = S->counts + 2 * (depth + 1);
+            a.n_shadow = n_sh;
             a.ids_in = w.ids[cur];
             a.rays_in = w.rays[cur];
             a.rays_out = w.rays[cur ^ 1];
             a.ids_out = w.ids[cur ^ 1];
             switch (r.integrator * 2 + r.brdf) {
-                case 0: launch_shade<0, 0>(a, s); break;
-                case 1: launch_shade<0, 1>(a, s); break;
-                case 2: launch_shade<1, 0>(a, s); break;
-                case 3: launch_shade<1, 1>(a, s); break;
-                case 4: launch_shade<2, 0>(a, s); break;
-                default: launch_shade<2, 1>(a, s); break;
+                case 0: launch_shade<0, 0>(a, P, s); break;
+                case 1: launch_shade<0, 1>(a, P, s); break;
+                case 2: launch_shade<1, 0>(a, P, s); break;
+                case 3: launch_shade<1, 1>(a, P, s); break;
+                case 4: launch_shade<2, 0>(a, P, s); break;
+                default: launch_shade<2, 1>(a, P, s); break;
             }
-            if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(S->host_counts, S->counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s) !=
-                    hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess) {
+            if (hipGetLastError() != hipSuccess) {
                 rc = hip_error(hipGetLastError(), "shade launch");
                 break;
             }
-            const uint32_t n_next = S->host_counts[0], n_sh = S->host_counts[1];
-            if (r.ray_counts) {
-                r.ray_counts[0] += n;
-                r.ray_counts[1] += n_sh;
+            if (r.integrator == 1) {  // material sampling casts no shadow or light rays
+                cur ^= 1;
+                continue;
             }
-            if (n_sh) {
-                if (r.integrator == 0) {
-                    rc = gpurt_tlas_trace_any(S->tlas, (const gpurt_ray*)w.sh_rays, n_sh, w.occ, nullptr, stream);
-                    if (rc) break;
-                    hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(n_sh, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
-                                       w.sh_contrib, w.occ, w.accs);
-                } else {
-                    rc = gpurt_tlas_trace_closest(S->tlas, (const gpurt_ray*)w.sh_rays, n_sh, w.hobj, w.hprim, w.ht,
-                                                  w.huv, nullptr, stream);
-                    if (rc) break;
-                    hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(n_sh, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
-                                       w.sh_contrib, w.hobj, S->objs, r.env_light[0], r.env_light[1], r.env_light[2],
-                                       w.accs);
-                }
-            }
-            n = n_next;
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, w.hobj, w.hprim, w.ht, w.huv, w.occ,
+                               r.integrator == 0, nullptr, s, S->trav, S->trav_bytes);
+            if (rc) break;
+            if (r.integrator == 0)
+                hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
+                                   w.sh_contrib, w.occ, w.accs);
+            else
+                hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
+                                   w.sh_contrib, w.hobj, S->objs, r.env_light[0], r.env_light[1], r.env_light[2],
+                                   w.accs);
             cur ^= 1;
         }
+        if (rc == GPURT_OK && r.ray_counts)
+            hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, S->counts, (uint32_t)bounces, S->tally);
         if (rc) break;
         hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, w.accs, w.pix);
     }
@@ -992,6 +1019,13 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, (uint32_t)r.samples, (uint32_t)r.frame,
                        w.pix, img);
     GPURT_HIP(hipGetLastError());
+    if (r.ray_counts) {  // the only wait of a frame, and only when statistics are asked for
+        unsigned long long t[2] = {0, 0};
+        GPURT_HIP(hipMemcpyAsync(t, S->tally, sizeof(t), hipMemcpyDeviceToHost, s));
+        GPURT_HIP(hipStreamSynchronize(s));
+        r.ray_counts[0] += t[0];
+        r.ray_counts[1] += t[1];
+    }
     return GPURT_OK;
 }
 
