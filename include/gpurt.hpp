@@ -17,6 +17,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cstdint>
 #include <stdexcept>
@@ -231,6 +232,80 @@ public:
     }
 
     gpurt_bvh accel = nullptr;
+};
+
+// Mirrors the top-level VK::Accel(const std::vector<Drop<Accel>>&, const std::vector<Mat4>&)
+// (vulkan.h:259, vulkan.cpp:777-856): instance i places blas[i] by the row-major 3x4
+// object-to-world transform T[i] with custom index i and mask 0xFF.  The BLASes must
+// outlive it.
+class Tlas {
+public:
+    Tlas() = default;
+    Tlas(const Context& ctx, const std::vector<const Accel*>& blas, const std::vector<std::array<float, 12>>& T,
+         hipStream_t stream = nullptr) {
+        recreate(ctx, blas, T, stream);
+    }
+    ~Tlas() { destroy(); }
+    Tlas(const Tlas&) = delete;
+    Tlas& operator=(const Tlas&) = delete;
+    Tlas(Tlas&& o) noexcept : accel(std::exchange(o.accel, nullptr)) {}
+    Tlas& operator=(Tlas&& o) noexcept {
+        std::swap(accel, o.accel);
+        return *this;
+    }
+    void recreate(const Context& ctx, const std::vector<const Accel*>& blas,
+                  const std::vector<std::array<float, 12>>& T, hipStream_t stream = nullptr) {
+        destroy();
+        if (blas.size() != T.size()) throw Error(GPURT_E_INVALID, "one transform per BLAS");
+        std::vector<gpurt_instance> inst(blas.size());
+        for (size_t i = 0; i < blas.size(); i++) {
+            std::copy(T[i].begin(), T[i].end(), inst[i].transform);
+            inst[i].custom_index = (uint32_t)i;
+            inst[i].mask = 0xFF;
+            inst[i].blas = blas[i]->accel;
+        }
+        check(gpurt_tlas_build(ctx.handle(), inst.data(), inst.size(), nullptr, stream, &accel));
+    }
+    void destroy() {
+        gpurt_tlas_destroy(accel);
+        accel = nullptr;
+    }
+    // trace_ray through both levels: obj = custom index of the hit instance (-1: miss)
+    void trace_closest(const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t, float* uv,
+                       const gpurt_query_opts* opts = nullptr, hipStream_t stream = nullptr) const {
+        check(gpurt_tlas_trace_closest(accel, rays, n, obj, prim, t, uv, opts, stream));
+    }
+    void trace_any(const gpurt_ray* rays, size_t n, uint8_t* occluded, const gpurt_query_opts* opts = nullptr,
+                   hipStream_t stream = nullptr) const {
+        check(gpurt_tlas_trace_any(accel, rays, n, occluded, opts, stream));
+    }
+
+    gpurt_tlas accel = nullptr;
+};
+
+// Mirrors VK::RTPipe (src/vk/rt.h): build_desc + use_accel at construction, trace() per
+// frame.  object i of `objects` is the TLAS instance with custom index i.
+class RTPipe {
+public:
+    RTPipe() = default;
+    RTPipe(const Tlas& tlas, const std::vector<gpurt_object>& objects, hipStream_t stream = nullptr) {
+        check(gpurt_scene_create(tlas.accel, objects.data(), objects.size(), stream, &scene));
+    }
+    ~RTPipe() { gpurt_scene_destroy(scene); }
+    RTPipe(const RTPipe&) = delete;
+    RTPipe& operator=(const RTPipe&) = delete;
+    RTPipe(RTPipe&& o) noexcept : scene(std::exchange(o.scene, nullptr)) {}
+    RTPipe& operator=(RTPipe&& o) noexcept {
+        std::swap(scene, o.scene);
+        return *this;
+    }
+    // one frame into `image` (height x width float4, device); G-buffers optional
+    void trace(const gpurt_render_params& p, float* image, float* gbuf_pos = nullptr, float* gbuf_norm = nullptr,
+               float* gbuf_albedo = nullptr, hipStream_t stream = nullptr) const {
+        check(gpurt_render(scene, &p, image, gbuf_pos, gbuf_norm, gbuf_albedo, stream));
+    }
+
+    gpurt_scene scene = nullptr;
 };
 
 }  // namespace gpurt
