@@ -290,13 +290,12 @@ int gpurt_closest_point_bwd(gpurt_bvh b, const float* q, size_t n, const int32_t
                             const float* g_dist, float* g_q, float* g_verts, void* stream) {
     if (!b) return set_error(GPURT_E_INVALID, "null bvh");
     if (n == 0) return GPURT_OK;
-    if (!q || !prim || !pt || !dist || !g_dist)
-        return set_error(GPURT_E_INVALID, "null input pointer");
-    if (g_verts && !bary) return set_error(GPURT_E_INVALID, "g_verts needs the saved barycentrics");
+    (void)bary, (void)pt, (void)dist;  // recomputed in FP64 from the saved triangle
+    if (!q || !prim || !g_dist) return set_error(GPURT_E_INVALID, "null input pointer");
     if (n > 0xFFFFFFFFull) return set_error(GPURT_E_INVALID, "too many queries in one call");
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_cpq_bwd(b, q, n, prim, bary, pt, dist, g_dist, g_q, g_verts, (hipStream_t)stream);
+    return launch_cpq_bwd(b, q, n, prim, g_dist, g_q, g_verts, (hipStream_t)stream);
 }
 
 int gpurt_trace_bwd(gpurt_bvh b, const gpurt_ray* rays, size_t n, const int32_t* prim,

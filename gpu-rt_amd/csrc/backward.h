@@ -4,6 +4,7 @@
 //   distance  d = |q - p|, p = sum_j b_j v_j the closest point (envelope theorem, the
 //             minimiser over the convex triangle is unique):
 //             dd/dq = (q - p)/d,  dd/dv_j = -b_j (q - p)/d
+//             (p, b and q - p in FP64 on the saved triangle)
 //   hit t     t = n.(v0 - o)/(n.d), n = (v1-v0) x (v2-v0), b = (1-u-v, u, v) fixed:
 //             dt/do = -n/(n.d),  dt/dd = -t n/(n.d),  dt/dv_j = b_j n/(n.d)
 //
@@ -20,55 +21,50 @@ __device__ __forceinline__ V3 vert_at(const float* __restrict__ verts, uint64_t 
     return v3(verts[vs * v], verts[vs * v + 1], verts[vs * v + 2]);
 }
 
-// Query i of a closest-point backward.  The offset q - p is re-formed relative to the
-// triangle's first vertex, (q - v0) - (b1 (v1 - v0) + b2 (v2 - v0)), like the forward
-// pass, instead of from the rounded closest point: its error then scales with the
-// triangle, not the coordinates, which keeps the direction accurate for queries very
-// close to the surface.  Without barycentrics it falls back to q - pt (and no vertex
-// gradient can be formed).
+// Query i of a closest-point backward.  The saved id selects the triangle (no
+// re-traversal); its closest point, offset q - p and barycentrics are recomputed in FP64
+// (closest_pt_tri_f64_core, the computation the forward pass reports from), so the
+// direction (q - p)/|q - p| stays accurate however close the query is to the surface --
+// a point rounded to FP32 would be off by ~1e-7 of the coordinates, a large angle when
+// |q - p| is tiny.  The saved pt / dist / bary are not needed.  Distance 0 contributes
+// nothing (the gradient is undefined on the surface).
 __device__ __forceinline__ void cpq_bwd_one(size_t i, const float* __restrict__ q, const int32_t* __restrict__ prim,
-                                            const float* __restrict__ bary, const float* __restrict__ pt,
-                                            const float* __restrict__ dist, const float* __restrict__ g,
-                                            const uint32_t* __restrict__ tri_idx, uint32_t n_tris,
-                                            const float* __restrict__ verts, uint64_t vs, float* __restrict__ g_q,
-                                            float* g_verts) {
+                                            const float* __restrict__ g, const uint32_t* __restrict__ tri_idx,
+                                            uint32_t n_tris, const float* __restrict__ verts, uint64_t vs,
+                                            float* __restrict__ g_q, float* g_verts) {
     const int32_t p = prim[i];
-    const float d = dist[i];
     float gx = 0.f, gy = 0.f, gz = 0.f;
-    // ids outside the mesh (not produced by the forward pass) contribute nothing
-    const bool live = p >= 0 && (uint32_t)p < n_tris && d > 0.0f && d < INFINITY;
+    double b[3] = {0.0, 0.0, 0.0};
+    bool live = p >= 0 && (uint32_t)p < n_tris;
     if (live) {
-        const float s = g[i] / d;
         const V3 qq = v3(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
-        V3 df;
-        if (bary) {
-            const V3 a = vert_at(verts, vs, tri_idx[3 * (size_t)p]);
-            const V3 e1 = sub(vert_at(verts, vs, tri_idx[3 * (size_t)p + 1]), a);
-            const V3 e2 = sub(vert_at(verts, vs, tri_idx[3 * (size_t)p + 2]), a);
-            const float b1 = bary[3 * i + 1], b2 = bary[3 * i + 2];
-            const V3 ap = sub(qq, a);
-            df = v3(ap.x - (e1.x * b1 + e2.x * b2), ap.y - (e1.y * b1 + e2.y * b2), ap.z - (e1.z * b1 + e2.z * b2));
-        } else {
-            df = sub(qq, v3(pt[3 * i], pt[3 * i + 1], pt[3 * i + 2]));
+        const CpF64 r = closest_pt_tri_f64_core(qq, vert_at(verts, vs, tri_idx[3 * (size_t)p]),
+                                                vert_at(verts, vs, tri_idx[3 * (size_t)p + 1]),
+                                                vert_at(verts, vs, tri_idx[3 * (size_t)p + 2]));
+        const double d = sqrt(r.d2);
+        live = d > 0.0;
+        if (live) {
+            const double s = (double)g[i] / d;
+            gx = (float)(r.df.x * s);
+            gy = (float)(r.df.y * s);
+            gz = (float)(r.df.z * s);
+            b[0] = r.b0, b[1] = r.b1, b[2] = r.b2;
         }
-        gx = df.x * s;
-        gy = df.y * s;
-        gz = df.z * s;
     }
     if (g_q) {
         g_q[3 * i] = gx;
         g_q[3 * i + 1] = gy;
         g_q[3 * i + 2] = gz;
     }
-    if (g_verts && bary && live) {
+    if (g_verts && live) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const float b = bary[3 * i + j];
-            if (b == 0.0f) continue;
+            const float bj = (float)b[j];
+            if (bj == 0.0f) continue;
             const size_t v = tri_idx[3 * (size_t)p + j];
-            unsafeAtomicAdd(&g_verts[3 * v], -b * gx);
-            unsafeAtomicAdd(&g_verts[3 * v + 1], -b * gy);
-            unsafeAtomicAdd(&g_verts[3 * v + 2], -b * gz);
+            unsafeAtomicAdd(&g_verts[3 * v], -bj * gx);
+            unsafeAtomicAdd(&g_verts[3 * v + 1], -bj * gy);
+            unsafeAtomicAdd(&g_verts[3 * v + 2], -bj * gz);
         }
     }
 }

@@ -10,16 +10,13 @@ namespace gpurt_impl {
 
 __global__ __launch_bounds__(kBlock) void k_cpq_bwd(uint32_t n, const float* __restrict__ q,
                                                     const int32_t* __restrict__ prim,
-                                                    const float* __restrict__ bary,
-                                                    const float* __restrict__ pt,
-                                                    const float* __restrict__ dist,
                                                     const float* __restrict__ g,
                                                     const uint32_t* __restrict__ tri_idx,
                                                     uint32_t n_tris, const float* __restrict__ verts,
                                                     float* __restrict__ g_q, float* g_verts) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    cpq_bwd_one(i, q, prim, bary, pt, dist, g, tri_idx, n_tris, verts, 3, g_q, g_verts);
+    cpq_bwd_one(i, q, prim, g, tri_idx, n_tris, verts, 3, g_q, g_verts);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ray_bwd(uint32_t n, const float4* __restrict__ rays,
@@ -37,12 +34,11 @@ __global__ __launch_bounds__(kBlock) void k_ray_bwd(uint32_t n, const float4* __
     ray_bwd_one(i, rays, prim, t, uv, g, tri_idx, n_tris, verts, 3, g_o, g_d, g_verts);
 }
 
-int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim,
-                   const float* bary, const float* pt, const float* dist, const float* g,
+int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim, const float* g,
                    float* g_q, float* g_verts, hipStream_t s) {
     if (n == 0) return GPURT_OK;
-    hipLaunchKernelGGL(k_cpq_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim,
-                       bary, pt, dist, g, b->tri_idx, (uint32_t)b->n_tris, b->verts, g_q, g_verts);
+    hipLaunchKernelGGL(k_cpq_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim, g,
+                       b->tri_idx, (uint32_t)b->n_tris, b->verts, g_q, g_verts);
     GPURT_HIP(hipGetLastError());
     return GPURT_OK;
 }

@@ -20,6 +20,10 @@
 namespace gpurt_impl {
 
 struct BatchMeshDev {
+    const float* verts;  // the mesh (for the closest-point finishing pass)
+    uint64_t vstride;    // floats
+    const uint32_t* tri_idx;
+    uint32_t n_tris, n_verts;
     const float* q;
     int32_t* q_prim;
     float* q_pt;
@@ -122,19 +126,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, blockIdx.x * BS + threadIdx.x, gridDim.x * BS);
     ChunkSource src{a.meshes, a.chunks, a.n_chunks, a.counter, false};
-    auto out = [&](const ChunkSource::Handle& h, uint32_t id, float d2, V3 cp, V3 br) {
-        const BatchMeshDev& md = a.meshes[h.m];
-        const size_t i = h.i;
-        md.q_prim[i] = (int32_t)id;
-        md.q_pt[3 * i] = cp.x;
-        md.q_pt[3 * i + 1] = cp.y;
-        md.q_pt[3 * i + 2] = cp.z;
-        md.q_dist[i] = (id != 0xFFFFFFFFu) ? sqrtf(d2) : INFINITY;
-        if (md.q_bary) {
-            md.q_bary[3 * i] = br.x;
-            md.q_bary[3 * i + 1] = br.y;
-            md.q_bary[3 * i + 2] = br.z;
-        }
+    auto out = [&](const ChunkSource::Handle& h, uint32_t id) {
+        a.meshes[h.m].q_prim[h.i] = id == 0xFFFFFFFFu ? -1 : (int32_t)id;
     };
     unsigned long long dn = 0, dt = 0;
     if (!cpq_stream<NodeOf<W, Q>, S, BS, false>(nodes, a.tris, src, a.leaf_trigger, st, out, dn, dt)) gatomic_or(a.err, 1u);
@@ -225,6 +218,20 @@ __global__ __launch_bounds__(256) void k_batch_zero_vgrads(const BatchGradDev* _
     for (uint32_t i = sg.y + threadIdx.x; i < end; i += 256) gd.g_verts[i] = 0.0f;
 }
 
+// Closest-point finishing pass over the forward chunk list (one wave per 64-query
+// chunk): the winners' point, distance and barycentrics in FP64.
+__global__ __launch_bounds__(256) void k_batch_cpq_finish(const BatchMeshDev* __restrict__ meshes,
+                                                          const uint2* __restrict__ chunks, uint32_t n_chunks) {
+    const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (c >= n_chunks) return;
+    const uint2 ch = chunks[c];
+    const BatchMeshDev& md = meshes[ch.x];
+    const uint32_t i = ch.y + (threadIdx.x & 63u);
+    if (i >= md.n_q) return;
+    cpq_finish_one(i, md.q, md.q_prim, md.tri_idx, md.n_tris, md.verts, md.vstride, md.n_verts, md.q_pt, md.q_dist,
+                   md.q_bary);
+}
+
 // One wave per 64-query chunk of the forward chunk list.
 template <bool RAYS>
 __global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restrict__ meshes,
@@ -242,8 +249,8 @@ __global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restric
                     gd.g_origin, gd.g_dir, gd.g_verts);
     } else {
         if (i >= md.n_q || !gd.g_dist) return;
-        cpq_bwd_one(i, md.q, md.q_prim, md.q_bary, md.q_pt, md.q_dist, gd.g_dist, gd.tri_idx, gd.n_tris,
-                    gd.verts, gd.vstride, gd.g_q, gd.g_verts);
+        cpq_bwd_one(i, md.q, md.q_prim, gd.g_dist, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride, gd.g_q,
+                    gd.g_verts);
     }
 }
 
@@ -338,6 +345,8 @@ int enqueue_batch(gpurt_batch_s* b, int treelet_size, int treelet_rounds, hipStr
         const unsigned grid = std::max<size_t>(1, std::min<size_t>(b->grid, (size_t)n * 64 / b->block + 1));
         if (!launch_batch(b->width, q8, b->stack, rays != 0, grid, s, a))
             return set_error(GPURT_E_UNSUPPORTED, "batch stack/width");
+        if (!rays)
+            hipLaunchKernelGGL(k_batch_cpq_finish, dim3(cdiv(n, 4)), dim3(256), 0, s, b->meshes_d, b->chunks_d, n);
     }
     GPURT_HIP(hipGetLastError());
     return GPURT_OK;
@@ -389,7 +398,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
         segs[m].idx = g.tri_idx;
         segs[m].tri_offset = (uint32_t)std::min<uint64_t>(off, 0xFFFFFFFFull);
         off += g.n_tris;
-        md[m] = BatchMeshDev{g.q, g.q_prim, g.q_pt, g.q_dist, g.q_bary,
+        md[m] = BatchMeshDev{g.verts, g.vstride_bytes / 4, g.tri_idx, (uint32_t)g.n_tris, (uint32_t)g.n_verts,
+                             g.q, g.q_prim, g.q_pt, g.q_dist, g.q_bary,
                              reinterpret_cast<const float4*>(g.rays), g.r_prim, g.r_t, g.r_uv,
                              (uint32_t)g.n_q, (uint32_t)g.n_r};
     }
@@ -512,8 +522,6 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         const gpurt_batch_mesh& hm = b->host_meshes[m];
         const gpurt_batch_grad& g = grads[m];
         const std::string at = " (mesh " + std::to_string(m) + ")";
-        if (g.g_dist && hm.n_q && g.g_verts && !hm.q_bary)
-            return set_error(GPURT_E_INVALID, "vertex gradients of distances need the saved barycentrics" + at);
         if (g.g_t && hm.n_r && g.g_verts && !hm.r_uv)
             return set_error(GPURT_E_INVALID, "vertex gradients of hit t need the saved (u,v)" + at);
         if (g.g_verts && 3 * (uint64_t)hm.n_verts >= 0xFFFFFFFFull)

@@ -252,8 +252,7 @@ int launch_ray(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, f
                float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, hipStream_t s);
 
 // backward (backward.hip)
-int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim,
-                   const float* bary, const float* pt, const float* dist, const float* g,
+int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim, const float* g,
                    float* g_q, float* g_verts, hipStream_t s);
 int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_t* prim,
                    const float* t, const float* uv, const float* g, float* g_o, float* g_d,

@@ -15,11 +15,8 @@ struct TravArgs {
     const float4* tris;
     uint32_t root;
     uint32_t n;
-    // closest point
+    // closest point (the traversal writes the winner's id; k_cpq_finish the rest)
     const float* q;
-    float* pt;
-    float* dist;
-    float* bary;
     // rays
     const float4* rays;
     float* t;
@@ -45,18 +42,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
     QueueSource src{nullptr, a.q, a.n, gid, total, a.counter};
-    auto out = [&](uint32_t qi, uint32_t id, float d2, V3 cp, V3 br) {
-        a.prim[qi] = (int32_t)id;
-        a.pt[3 * (size_t)qi] = cp.x;
-        a.pt[3 * (size_t)qi + 1] = cp.y;
-        a.pt[3 * (size_t)qi + 2] = cp.z;
-        a.dist[qi] = (id != 0xFFFFFFFFu) ? sqrtf(d2) : INFINITY;
-        if (a.bary) {
-            a.bary[3 * (size_t)qi] = br.x;
-            a.bary[3 * (size_t)qi + 1] = br.y;
-            a.bary[3 * (size_t)qi + 2] = br.z;
-        }
-    };
+    auto out = [&](uint32_t qi, uint32_t id) { a.prim[qi] = id == 0xFFFFFFFFu ? -1 : (int32_t)id; };
     if (!cpq_stream<NodeOf<W, Q>, S, BS, STATS>(nodes, a.tris, src, a.leaf_trigger, st, out, n_nodes, n_tris))
         gatomic_or(a.err, 1u);
     if (STATS) {
@@ -252,15 +238,29 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
     return GPURT_OK;
 }
 
+__global__ __launch_bounds__(kBlock) void k_cpq_finish(uint32_t n, const float* __restrict__ q,
+                                                       const int32_t* __restrict__ prim,
+                                                       const uint32_t* __restrict__ tri_idx, uint32_t n_tris,
+                                                       const float* __restrict__ verts, uint32_t n_verts,
+                                                       float* __restrict__ pt, float* __restrict__ dist,
+                                                       float* __restrict__ bary) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    cpq_finish_one(i, q, prim, tri_idx, n_tris, verts, 3, n_verts, pt, dist, bary);
+}
+
 int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* pt, float* dist,
                float* bary, const gpurt_query_opts* opts, hipStream_t s) {
     TravArgs a{};
     a.q = q;
     a.prim = prim;
-    a.pt = pt;
-    a.dist = dist;
-    a.bary = bary;
-    return run_query(b, QKind::Cpq, a, n, opts, s);
+    int rc = run_query(b, QKind::Cpq, a, n, opts, s);
+    if (rc || n == 0) return rc;
+    // the winners' point, distance and barycentrics in FP64 (one gather per query)
+    hipLaunchKernelGGL(k_cpq_finish, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim, b->tri_idx,
+                       (uint32_t)b->n_tris, b->verts, (uint32_t)b->n_verts, pt, dist, bary);
+    GPURT_HIP(hipGetLastError());
+    return GPURT_OK;
 }
 
 int launch_ray(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t, float* uv,

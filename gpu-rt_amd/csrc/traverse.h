@@ -476,20 +476,13 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
     bool has = false;
     V3 q = v3(0.f, 0.f, 0.f);
     float best = INFINITY, thr = INFINITY;
-    uint32_t bid = 0xFFFFFFFFu, bpos = 0, cur = 0, pleaf = kEmpty;
+    uint32_t bid = 0xFFFFFFFFu, cur = 0, pleaf = kEmpty;
     bool have = false;
     for (;;) {
         if (has && !have && pleaf == kEmpty && st.sp == 0) {
-            // the closest point and barycentrics of the winner, recomputed once with the
-            // same arithmetic (cheaper than carrying them through the traversal)
-            V3 cp = v3(0.f, 0.f, 0.f), br = cp;
-            if (bid != 0xFFFFFFFFu) {
-                const float4 A = tris[3 * (size_t)bpos];
-                const float4 B = tris[3 * (size_t)bpos + 1];
-                const float4 C = tris[3 * (size_t)bpos + 2];
-                (void)closest_pt_tri(q, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), cp, br);
-            }
-            out(h, bid, best, cp, br);
+            // only the winner's id leaves the traversal: its point, distance and
+            // barycentrics are computed by a separate FP64 pass (cpq_finish_one)
+            out(h, bid);
             has = false;
         }
         if (src.refill(has, h)) {
@@ -530,7 +523,6 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
                 if (d2 < best || (d2 == best && id < bid)) {
                     best = d2;
                     bid = id;
-                    bpos = f + k;
                     thr = best * kCullSlack;
                 }
             }

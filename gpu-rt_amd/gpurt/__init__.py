@@ -301,8 +301,6 @@ class Batch:
         self.grads = []
         for i, (r, m) in enumerate(zip(self.results, self._meshes)):
             nq, nr = r["q_prim"].shape[0], r["r_prim"].shape[0]
-            if vertex_grads and distance and nq and r["q_bary"] is None:
-                raise ValueError("vertex gradients of distances need with_bary=True")
             if vertex_grads and hit_t and nr and r["r_uv"] is None:
                 raise ValueError("vertex gradients of hit t need with_uv=True")
             g = {

@@ -142,7 +142,9 @@ GPURT_API int gpurt_bvh_get_info(gpurt_bvh bvh, gpurt_bvh_info* out);
 
 /* Closest point on the mesh for n query points q (n x 3 floats).  Outputs: prim (int32,
  * -1 only for an empty mesh), pt (n x 3 closest point), dist (unsigned distance), bary
- * (n x 3 barycentrics of pt, nullable).  No counterpart in the mounted reference (its
+ * (n x 3 barycentrics of pt, nullable).  prim is the FP32 argmin (lowest id on ties);
+ * pt, dist and bary are that triangle's closest point computed in FP64 and rounded once
+ * to FP32 (a second, gather-only pass).  No counterpart in the mounted reference (its
  * closest-point work is on a branch not included, README.md:6-8); it uses the same
  * handle as the ray queries. */
 GPURT_API int gpurt_closest_point(gpurt_bvh bvh, const float* q, size_t n, int32_t* prim, float* pt,
@@ -164,10 +166,14 @@ GPURT_API int gpurt_trace_any(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, ui
 
 /* ---- backward --------------------------------------------------------------------- */
 
-/* Gradient of sum_i g_dist[i] * dist_i.  Uses the forward pass's saved prim and bary
- * (no re-traversal): g_q[i] = g_dist[i] (q_i - pt_i)/dist_i; for the three vertices v_j
- * of prim_i, g_verts[v_j] -= b_j g_q[i]  (FP32 atomic scatter-add: g_verts is
- * ACCUMULATED, n_verts x 3, zero it first).  g_q and g_verts are each nullable. */
+/* Gradient of sum_i g_dist[i] * dist_i.  Uses the forward pass's saved prim (no
+ * re-traversal): the closest point p_i of that triangle, its barycentrics b and q_i - p_i
+ * are recomputed in FP64 (the computation the forward pass reports from), then
+ * g_q[i] = g_dist[i] (q_i - p_i)/|q_i - p_i|; for the three vertices v_j of prim_i,
+ * g_verts[v_j] -= b_j g_q[i]  (FP32 atomic scatter-add: g_verts is ACCUMULATED,
+ * n_verts x 3, zero it first).  g_q and g_verts are each nullable; so are the saved bary,
+ * pt and dist (accepted for the forward's output layout, not read).  A query on the
+ * surface (distance 0) contributes zero. */
 GPURT_API int gpurt_closest_point_bwd(gpurt_bvh bvh, const float* q, size_t n, const int32_t* prim,
                             const float* bary, const float* pt, const float* dist,
                             const float* g_dist, float* g_q, float* g_verts, void* stream);
@@ -226,8 +232,9 @@ typedef struct gpurt_batch_grad {
 /* Record the batched backward pass (one gpurt_batch_grad per mesh, in the order of
  * gpurt_batch_create) into a second HIP graph: it clears the vertex gradients, then
  * forms every mesh's gradients from the outputs of the last gpurt_batch_run (saved ids,
- * barycentrics, points, distances, hit t, (u,v)) without re-traversal, scatter-adding
- * vertex terms with FP32 atomics.  Vertex gradients need the forward's q_bary / r_uv.
+ * closest-point triangles recomputed in FP64, hit t, (u,v)) without re-traversal,
+ * scatter-adding vertex terms with FP32 atomics.  Vertex gradients of hit t need the
+ * forward's r_uv.
  * No counterpart in the reference (it has no gradients); it is the batched form of
  * gpurt_closest_point_bwd + gpurt_trace_bwd.  Calling it again replaces the buffers. */
 GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* grads);

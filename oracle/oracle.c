@@ -21,6 +21,9 @@
  *                         cull only exists in its light-pdf helper; the hardware
  *                         traversal that rt.rgen:258,278 calls has no such cull), and
  *                         the accepted interval is [tmin, tmax] (rt.rgen:257-291).
+ *     orc_cp_tri_refine   the reported point / distance / barycentrics of the winning
+ *                         triangle: the same region tests in FP64, rounded once to FP32
+ *                         (the FP32 test chooses the triangle, this sharpens the report).
  *     *_brute_f32         global (value, lowest index) minimum over all triangles;
  *                         any-hit = "some triangle has t in [tmin, tmax]"
  *                         (rt.rgen:272-291, rt.rmiss:9-11).
@@ -200,6 +203,94 @@ ORC_API float orc_cp_tri_f32(const float* p, const float* a, const float* b, con
     return dot3f(df, df);
 }
 
+/* The reported closest point of a winning triangle: the same Ericson region tests in
+ * FP64 on the FP32 inputs (exactly converted), every product and sum rounded to double
+ * in the written order, the offset taken from the region's base vertex, then the point,
+ * the barycentrics and sqrt(d2) each rounded once to FP32.  The FP32 search
+ * (orc_cp_tri_f32) chooses the triangle; this only sharpens what is reported about it.
+ * Returns the distance. */
+static inline double dot3dd(const double* a, const double* b) {
+    double s = a[0] * b[0];
+    s = s + a[1] * b[1];
+    s = s + a[2] * b[2];
+    return s;
+}
+ORC_API float orc_cp_tri_refine(const float* pf, const float* af, const float* bf, const float* cf, float* cp,
+                                float* bary) {
+    double p[3], a[3], b[3], c[3], ab[3], ac[3], ap[3], df[3], q[3];
+    for (int k = 0; k < 3; k++) {
+        p[k] = pf[k]; a[k] = af[k]; b[k] = bf[k]; c[k] = cf[k];
+        ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k]; ap[k] = p[k] - a[k];
+    }
+    double d1 = dot3dd(ab, ap), d2 = dot3dd(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        for (int k = 0; k < 3; k++) cp[k] = af[k];
+        bary[0] = 1.0f; bary[1] = 0.0f; bary[2] = 0.0f;
+        return (float)sqrt(dot3dd(ap, ap));
+    }
+    double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    double d3 = dot3dd(ab, bp), d4 = dot3dd(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+        for (int k = 0; k < 3; k++) cp[k] = bf[k];
+        bary[0] = 0.0f; bary[1] = 1.0f; bary[2] = 0.0f;
+        return (float)sqrt(dot3dd(bp, bp));
+    }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        for (int k = 0; k < 3; k++) {
+            double off = v * ab[k];
+            df[k] = ap[k] - off;
+            q[k] = a[k] + off;
+            cp[k] = (float)q[k];
+        }
+        bary[0] = (float)(1.0 - v); bary[1] = (float)v; bary[2] = 0.0f;
+        return (float)sqrt(dot3dd(df, df));
+    }
+    double cpv[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    double d5 = dot3dd(ab, cpv), d6 = dot3dd(ac, cpv);
+    if (d6 >= 0.0 && d5 <= d6) {
+        for (int k = 0; k < 3; k++) cp[k] = cf[k];
+        bary[0] = 0.0f; bary[1] = 0.0f; bary[2] = 1.0f;
+        return (float)sqrt(dot3dd(cpv, cpv));
+    }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        for (int k = 0; k < 3; k++) {
+            double off = w * ac[k];
+            df[k] = ap[k] - off;
+            q[k] = a[k] + off;
+            cp[k] = (float)q[k];
+        }
+        bary[0] = (float)(1.0 - w); bary[1] = 0.0f; bary[2] = (float)w;
+        return (float)sqrt(dot3dd(df, df));
+    }
+    double va = d3 * d6 - d5 * d4;
+    double e43 = d4 - d3, e56 = d5 - d6;
+    if (va <= 0.0 && e43 >= 0.0 && e56 >= 0.0) {
+        double w = e43 / (e43 + e56);
+        for (int k = 0; k < 3; k++) {
+            double off = w * (c[k] - b[k]);
+            df[k] = bp[k] - off;
+            q[k] = b[k] + off;
+            cp[k] = (float)q[k];
+        }
+        bary[0] = 0.0f; bary[1] = (float)(1.0 - w); bary[2] = (float)w;
+        return (float)sqrt(dot3dd(df, df));
+    }
+    double denom = 1.0 / ((va + vb) + vc);
+    double v = vb * denom, w = vc * denom;
+    for (int k = 0; k < 3; k++) {
+        double off = ab[k] * v + ac[k] * w;
+        df[k] = ap[k] - off;
+        q[k] = a[k] + off;
+        cp[k] = (float)q[k];
+    }
+    bary[0] = (float)((1.0 - v) - w); bary[1] = (float)v; bary[2] = (float)w;
+    return (float)sqrt(dot3dd(df, df));
+}
+
 /* cross(a,b) in the GLSL component order. */
 static inline void cross3f(const float* a, const float* b, float* r) {
     r[0] = a[1] * b[2] - a[2] * b[1];
@@ -251,7 +342,7 @@ static void cpq_bf32_range(void* vctx, int64_t lo, int64_t hi) {
     cpq_bf32_ctx* c = (cpq_bf32_ctx*)vctx;
     for (int64_t i = lo; i < hi; i++) {
         const float* q = c->Q + 3 * i;
-        float best = INFINITY, bcp[3] = {0, 0, 0}, bb[3] = {0, 0, 0};
+        float best = INFINITY, bcp[3] = {0, 0, 0}, bb[3] = {0, 0, 0}, bd = INFINITY;
         int64_t bi = -1;
         for (int64_t t = 0; t < c->nt; t++) {
             const uint32_t* f = c->F + 3 * t;
@@ -259,12 +350,15 @@ static void cpq_bf32_range(void* vctx, int64_t lo, int64_t hi) {
             float d2 = orc_cp_tri_f32(q, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], cp, br);
             if (d2 < best) { /* strict: ties keep the lower index (scan is ascending) */
                 best = d2; bi = t;
-                memcpy(bcp, cp, sizeof cp); memcpy(bb, br, sizeof br);
             }
+        }
+        if (bi >= 0) {
+            const uint32_t* f = c->F + 3 * bi;
+            bd = orc_cp_tri_refine(q, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], bcp, bb);
         }
         c->prim[i] = (int32_t)bi;
         if (c->pt) memcpy(c->pt + 3 * i, bcp, sizeof bcp);
-        if (c->dist) c->dist[i] = bi >= 0 ? sqrtf(best) : INFINITY;
+        if (c->dist) c->dist[i] = bd;
         if (c->bary) memcpy(c->bary + 3 * i, bb, sizeof bb);
     }
 }
@@ -635,7 +729,9 @@ ORC_API void orc_ray_closest_brute_f64(const double* V, const uint32_t* F, int64
 /* Unsigned distance d(q) = |q - p(q)| with p the closest point of triangle `prim`.
  * By the envelope theorem (the minimiser over the convex triangle is unique):
  *   dd/dq = (q - p)/d,   dd/dv_i = -b_i (q - p)/d.
- * g_q[i] = g[i] * dd/dq ; g_V[vertex] += g[i] * dd/dv_i.  d == 0 gives zero gradient. */
+ * g_q[i] = g[i] * dd/dq ; g_V[vertex] += g[i] * dd/dv_i.  d == 0 gives zero gradient.
+ * With pt == NULL the closest point and barycentrics of `prim` are computed here by the
+ * independent FP64 formulation (cp_tri_f64) instead of being taken from the caller. */
 ORC_API void orc_cpq_grad_f64(const double* V, const uint32_t* F, const double* Q, int64_t nq,
                               const int32_t* prim, const double* pt, const double* bary,
                               const double* g, double* g_q, double* g_V) {
@@ -643,7 +739,13 @@ ORC_API void orc_cpq_grad_f64(const double* V, const uint32_t* F, const double* 
         double gq[3] = {0, 0, 0};
         if (prim[i] >= 0) {
             const double* q = Q + 3 * i;
-            const double* p = pt + 3 * i;
+            double own_p[3], own_b[3];
+            const double* p = pt ? pt + 3 * i : own_p;
+            const double* bi = pt ? bary + 3 * i : own_b;
+            if (!pt) {
+                const uint32_t* f = F + 3 * (int64_t)prim[i];
+                (void)cp_tri_f64(q, V + 3 * f[0], V + 3 * f[1], V + 3 * f[2], own_p, own_b);
+            }
             double df[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
             double d = sqrt(dot3d(df, df));
             if (d > 0) {
@@ -652,13 +754,12 @@ ORC_API void orc_cpq_grad_f64(const double* V, const uint32_t* F, const double* 
                 if (g_V) {
                     const uint32_t* f = F + 3 * (int64_t)prim[i];
                     for (int j = 0; j < 3; j++)
-                        for (int k = 0; k < 3; k++) g_V[3 * (int64_t)f[j] + k] -= bary[3 * i + j] * gq[k];
+                        for (int k = 0; k < 3; k++) g_V[3 * (int64_t)f[j] + k] -= bi[j] * gq[k];
                 }
             }
         }
         if (g_q) memcpy(g_q + 3 * i, gq, sizeof gq);
     }
-    (void)V;
 }
 
 /* Ray hit distance t = n.(v0 - o)/(n.d), n = (v1-v0) x (v2-v0).  Implicit differentiation
@@ -971,7 +1072,8 @@ static void lbvh_cpq_range(void* vctx, int64_t lo, int64_t hi) {
     float st_d[ORC_STACK];
     for (int64_t i = lo; i < hi; i++) {
         const float* q = c->Q + 3 * i;
-        float best = INFINITY, bcp[3] = {0, 0, 0}, bb[3] = {0, 0, 0};
+        float best = INFINITY, bcp[3] = {0, 0, 0}, bb[3] = {0, 0, 0}, bd = INFINITY;
+        const float* btr = NULL;
         int32_t bi = -1;
         int sp = 0;
         st_n[sp] = 0; st_d[sp] = 0.0f; sp++;
@@ -990,8 +1092,7 @@ static void lbvh_cpq_range(void* vctx, int64_t lo, int64_t hi) {
                     int32_t id = b->ids[p];
                     tris_tested++;
                     if (d2 < best || (d2 == best && id < bi)) {
-                        best = d2; bi = id;
-                        memcpy(bcp, cp, sizeof cp); memcpy(bb, br, sizeof br);
+                        best = d2; bi = id; btr = tr;
                     }
                 }
                 continue;
@@ -1004,9 +1105,10 @@ static void lbvh_cpq_range(void* vctx, int64_t lo, int64_t hi) {
             if (dr <= thr && sp < ORC_STACK) { st_n[sp] = nr; st_d[sp] = dr; sp++; }
             if (dl <= thr && sp < ORC_STACK) { st_n[sp] = nl; st_d[sp] = dl; sp++; }
         }
+        if (btr) bd = orc_cp_tri_refine(q, btr, btr + 3, btr + 6, bcp, bb);
         c->prim[i] = bi;
         if (c->pt) memcpy(c->pt + 3 * i, bcp, sizeof bcp);
-        if (c->dist) c->dist[i] = bi >= 0 ? sqrtf(best) : INFINITY;
+        if (c->dist) c->dist[i] = bd;
         if (c->bary) memcpy(c->bary + 3 * i, bb, sizeof bb);
     }
     if (c->counters) {

@@ -40,6 +40,8 @@ def load():
     lib.orc_num_threads.restype = I
     lib.orc_cp_tri_f32.argtypes = [_f32p] * 4 + [_f32p, _f32p]
     lib.orc_cp_tri_f32.restype = ctypes.c_float
+    lib.orc_cp_tri_refine.argtypes = [_f32p] * 4 + [_f32p, _f32p]
+    lib.orc_cp_tri_refine.restype = ctypes.c_float
     lib.orc_mt_f32.argtypes = [_f32p] * 5 + [ctypes.POINTER(ctypes.c_float)] * 3
     lib.orc_mt_f32.restype = I
     lib.orc_cp_tri_f64.argtypes = [_f64p] * 4 + [_f64p, _f64p]
@@ -51,7 +53,7 @@ def load():
     lib.orc_ray_closest_brute_f32.argtypes = [_f32p, _u32p, I64, _f32p, I64, _i32p, _f32p, P]
     lib.orc_ray_any_brute_f32.argtypes = [_f32p, _u32p, I64, _f32p, I64, _u8p]
     lib.orc_ray_closest_brute_f64.argtypes = [_f64p, _u32p, I64, _f64p, I64, _i32p, _f64p, P]
-    lib.orc_cpq_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P]
+    lib.orc_cpq_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, P, P, P, P, P]
     lib.orc_ray_grad_f64.argtypes = [_f64p, _u32p, _f64p, I64, _i32p, _f64p, _f64p, P, P, P, P]
     lib.orc_morton_codes.argtypes = [_f32p, I64, _f32p, _f32p, _u32p]
     lib.orc_invert3.argtypes = [_f32p, _f32p]
@@ -224,13 +226,17 @@ def ray_closest_brute_f64(V, F, R):
 
 # ---- gradients (FP64 closed forms) ---------------------------------------------------
 
-def cpq_grad_f64(V, F, Q, prim, pt, bary, g=None):
+def cpq_grad_f64(V, F, Q, prim, pt=None, bary=None, g=None):
+    """FP64 closed-form distance gradients at the given argmin; pt / bary None: the
+    closest point of each prim is computed in FP64 here (independent formulation)."""
     V, F, Q = _c(V, np.float64), _c(F, np.uint32), _c(Q, np.float64)
-    prim = _c(prim, np.int32); pt = _c(pt, np.float64); bary = _c(bary, np.float64)
+    prim = _c(prim, np.int32)
+    pt = _c(pt, np.float64) if pt is not None else None
+    bary = _c(bary, np.float64) if bary is not None else None
     g = _c(g, np.float64) if g is not None else None
     n = Q.shape[0]
     gq = np.zeros((n, 3)); gv = np.zeros((V.shape[0], 3))
-    load().orc_cpq_grad_f64(V, F, Q, n, prim, pt, bary, _p(g), _p(gq), _p(gv))
+    load().orc_cpq_grad_f64(V, F, Q, n, prim, _p(pt), _p(bary), _p(g), _p(gq), _p(gv))
     return gq, gv
 
 

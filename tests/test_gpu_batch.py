@@ -169,13 +169,26 @@ def test_batch_backward_matches_plain_backward(ctx, dev, fmt):
             np.testing.assert_allclose(g["g_q"].cpu().numpy()[ok], eq[ok], atol=1e-3, rtol=1e-3)
 
 
-def test_batch_backward_needs_saved_barycentrics(ctx, dev):
+def test_batch_backward_saved_outputs(ctx, dev):
+    """Distance gradients recompute the saved triangle's closest point, so they need no
+    saved barycentrics; vertex gradients of hit t need the saved (u,v)."""
     host = _small_meshes(seed=2)[:3]
-    batch = gpurt.Batch(ctx, _dev(host, dev), with_bary=False)
+    with_b = gpurt.Batch(ctx, _dev(host, dev))
+    no_b = gpurt.Batch(ctx, _dev(host, dev), with_bary=False)
+    ga = with_b.enable_backward(hit_t=False)
+    gb = no_b.enable_backward(hit_t=False)
+    for b in (with_b, no_b):
+        b.run()
+        b.backward()
+    torch.cuda.synchronize()
+    for a, b in zip(ga, gb):
+        assert torch.equal(a["g_q"], b["g_q"])
+        assert torch.allclose(a["g_verts"], b["g_verts"], atol=1e-5)  # atomic order
+    no_uv = gpurt.Batch(ctx, _dev(host, dev), with_uv=False)
     with pytest.raises(ValueError):
-        batch.enable_backward()
-    g = batch.enable_backward(vertex_grads=False)
-    batch.run()
-    batch.backward()
+        no_uv.enable_backward()
+    g = no_uv.enable_backward(vertex_grads=False)
+    no_uv.run()
+    no_uv.backward()
     torch.cuda.synchronize()
     assert g[0]["g_verts"] is None and g[0]["g_q"].shape[1] == 3
