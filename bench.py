@@ -166,8 +166,14 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         cpu_s += time.perf_counter() - t0
         passes += 1
     cpu_rate = 2 * n * passes / cpu_s
+    # the achievable HBM rate on this box: a device copy of 1 GiB (read + write)
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    copy_ms = timed(lambda: dst.copy_(src), 10, 2)
+    copy_gbs = 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del src, dst
     roof = {
-        "kernel": "k_ray<W=%d,%s,closest>" % (args.width, "q8" if info["node_format"] == 2 else "f32"),
+        "kernel": "k_ray<W=%d,%s,closest>" % (args.width, "q8" if info["node_format"] in (2, 3) else "f32"),
         "bound": "hbm",
         "achieved": bytes_closest / (ms_closest * 1e-3) / 1e9,
         "peak": HBM_PEAK_GBS,
@@ -180,6 +186,8 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "kernel_ms": ms_closest,
     }
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["copy_achievable_GBps"] = copy_gbs
+    roof["frac_of_copy"] = roof["achieved"] / copy_gbs
     roof["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
     res = {
         "value": 2 * n / (ms * 1e-3),

@@ -198,22 +198,10 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
         }
         if (!node_ready) continue;
         if (STATS) n_nodes++;
-        ChildBoxes<W> nd;
-        fetch_node(in_inst ? bnodes : top_nodes, cur, nd);
         const float slack = in_inst ? kTSlack : kTSlackTop;
         float tn[W];
         uint32_t c[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            const float t0x = fmaf(nd.lox[j], r.inv.x, r.oix), t1x = fmaf(nd.hix[j], r.inv.x, r.oix);
-            const float t0y = fmaf(nd.loy[j], r.inv.y, r.oiy), t1y = fmaf(nd.hiy[j], r.inv.y, r.oiy);
-            const float t0z = fmaf(nd.loz[j], r.inv.z, r.oiz), t1z = fmaf(nd.hiz[j], r.inv.z, r.oiz);
-            const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-            const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-            const bool hb = (nd.child[j] != kEmpty) & (lo <= hi + fabsf(hi) * slack);
-            tn[j] = hb ? lo : INFINITY;
-            c[j] = hb ? nd.child[j] : kEmpty;
-        }
+        ray_children<W>(in_inst ? bnodes : top_nodes, cur, r.inv, r.oix, r.oiy, r.oiz, tmin, best, slack, tn, c);
         if (!ANY) sort_children<W>(tn, c);
         have = false;
         uint32_t pend = kEmpty;

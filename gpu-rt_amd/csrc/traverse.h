@@ -185,6 +185,75 @@ __device__ __forceinline__ void fetch_node(const QNode<W>* __restrict__ nodes, u
 template <int W, bool Q>
 using NodeOf = typename std::conditional<Q, QNode<W>, WNode<W>>::type;
 
+// Ray / child-box slab tests of one wide node: tn[j] = entry distance of child j and c[j]
+// its child word, or INFINITY / kEmpty when the ray's [tmin, best] misses the box.  The
+// plane distances are t = (plane - o) / d formed as one FMA with inv = 1/d and
+// oi = -o * inv precomputed; `slack` is the relative slack on box exits.
+template <int W>
+__device__ __forceinline__ void ray_children(const WNode<W>* __restrict__ nodes, uint32_t i, V3 inv, float oix,
+                                             float oiy, float oiz, float tmin, float best, float slack, float* tn,
+                                             uint32_t* c) {
+    ChildBoxes<W> nd;
+    fetch_node(nodes, i, nd);
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
+        const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
+        const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
+        const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+        const bool hb = (nd.child[j] != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+        tn[j] = hb ? lo : INFINITY;
+        c[j] = hb ? nd.child[j] : kEmpty;
+    }
+}
+
+// 8-bit node: the plane distances come straight from the byte planes.  With
+// a = 2^(e-127) * inv and b = origin * inv + oi per axis (once per node), a plane of
+// byte q is at t = q * a + b: the distance of the decoded plane q * 2^(e-127) + origin,
+// with two FMAs per child and axis fewer than decoding first.  Its rounding differs from
+// the decode-then-slab form by a few FP32 ulps of the coordinates, far inside the
+// builder's box inflation (2^-18 of the scene's extent), so the cull stays conservative.
+template <int W>
+__device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes, uint32_t i, V3 inv, float oix,
+                                             float oiy, float oiz, float tmin, float best, float slack, float* tn,
+                                             uint32_t* c) {
+    constexpr int NW = (int)sizeof(QNode<W>) / 4;
+    constexpr int Q0 = 4 + W;  // first word of the 8-bit planes
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    uint32_t w[NW];
+#pragma unroll
+    for (int k = 0; k < NW / 4; k++) {
+        const uint4 v = p[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    const float ax = __uint_as_float((w[3] & 0xFFu) << 23) * inv.x;
+    const float ay = __uint_as_float(((w[3] >> 8) & 0xFFu) << 23) * inv.y;
+    const float az = __uint_as_float(((w[3] >> 16) & 0xFFu) << 23) * inv.z;
+    const float bx = fmaf(__uint_as_float(w[0]), inv.x, oix);
+    const float by = fmaf(__uint_as_float(w[1]), inv.y, oiy);
+    const float bz = fmaf(__uint_as_float(w[2]), inv.z, oiz);
+    auto qb = [&](int row, int j) -> float {
+        const int byte = row * W + j;
+        return (float)((w[Q0 + byte / 4] >> (8 * (byte % 4))) & 0xFFu);
+    };
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const uint32_t ch = w[4 + j];
+        const float t0x = fmaf(qb(0, j), ax, bx), t1x = fmaf(qb(3, j), ax, bx);
+        const float t0y = fmaf(qb(1, j), ay, by), t1y = fmaf(qb(4, j), ay, by);
+        const float t0z = fmaf(qb(2, j), az, bz), t1z = fmaf(qb(5, j), az, bz);
+        const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+        const bool hb = (ch != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+        tn[j] = hb ? lo : INFINITY;
+        c[j] = hb ? ch : kEmpty;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // closest point
 // ------------------------------------------------------------------------------------
@@ -426,23 +495,11 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
         }
         if (!node_ready) continue;
         if (STATS) n_nodes++;
-        ChildBoxes<W> nd;
-        fetch_node(nodes, cur, nd);
+        // t = (b - o) / d as one FMA with o/d precomputed; its rounding is far below the
+        // build-time box inflation, so the test stays conservative
         float tn[W];
         uint32_t c[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            // t = (b - o) / d as one FMA with o/d precomputed; its rounding is far below
-            // the build-time box inflation, so the test stays conservative
-            const float t0x = fmaf(nd.lox[j], inv.x, oix), t1x = fmaf(nd.hix[j], inv.x, oix);
-            const float t0y = fmaf(nd.loy[j], inv.y, oiy), t1y = fmaf(nd.hiy[j], inv.y, oiy);
-            const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
-            const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-            const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-            const bool hb = (nd.child[j] != kEmpty) & (lo <= hi + fabsf(hi) * kTSlack);
-            tn[j] = hb ? lo : INFINITY;
-            c[j] = hb ? nd.child[j] : kEmpty;
-        }
+        ray_children<W>(nodes, cur, inv, oix, oiy, oiz, tmin, best, kTSlack, tn, c);
         if (!ANY) sort_children<W>(tn, c);
         have = false;
         // push all but the nearest hit child (far to near), then continue with it; the
