@@ -41,6 +41,7 @@ constexpr int kShadeBlock = 256;
 constexpr int kCountSlots = 2 * 64 + 2;  // two list lengths per bounce, up to 64 bounces
 constexpr int kLightStack = 64;  // light BVH traversal stack (checked against its depth)
 constexpr float kExitSlack = 0x1p-20f;  // relative slack on box exits (conservative culls)
+constexpr uint32_t kLoopLightTris = 32;  // up to this many light triangles light_pdf loops
 
 // per object: Scene_Desc (src/vk/rt.h:67-78) without textures
 struct ObjDev {
@@ -77,14 +78,29 @@ struct SceneDev {
 
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
-// light_pdf (rt.rgen:222-255) through the light BVH: every light triangle that
-// triangle_hit accepts for the ray (p, d, t >= 0) contributes triangle_pdf / n_tris(light);
-// the sum is divided by the light count.  The box culls are conservative (boxes padded
-// by the builder, slack on the exit distance), so the set of contributing triangles is
-// the reference's.  With one light the raw sum is divided once by its triangle count, as
-// the reference does.
+// light_pdf (rt.rgen:222-255): every light triangle that triangle_hit accepts for the ray
+// (p, d, t >= 0) contributes triangle_pdf / n_tris(light); the sum is divided by the
+// light count.  Beyond kLoopLightTris light triangles the terms are found through the
+// light BVH: its box culls are conservative (boxes padded by the builder, slack on the
+// exit distance), so the set of contributing triangles is the reference's; with one light
+// the raw sum is divided once by its triangle count, as the reference does.
 __device__ float light_pdf(const SceneDev& sc, V3 p, V3 d) {
     if (sc.n_lights == 0) return 0.0f;
+    if (sc.n_ltris <= kLoopLightTris) {
+        // few light triangles: the reference's own loop (per light, per triangle, in
+        // order), cheaper than any traversal and the same sums bit for bit
+        float oacc = 0.0f;
+        for (uint32_t l = 0; l < sc.n_lights; l++) {
+            const LightDev L = sc.lights[l];
+            float tacc = 0.0f;
+            for (uint32_t t = 0; t < L.n_tris; t++) {
+                const size_t k = 3 * (size_t)(L.first + t);
+                tacc = tacc + pt::triangle_pdf(p, d, xyz(sc.lworld[k]), xyz(sc.lworld[k + 1]), xyz(sc.lworld[k + 2]));
+            }
+            oacc = oacc + tacc / (float)L.n_tris;
+        }
+        return oacc / (float)sc.n_lights;
+    }
     const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
     const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
     const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;

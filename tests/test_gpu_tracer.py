@@ -139,8 +139,24 @@ def test_light_pdf_matches_reference_loop(box, dev):
     assert np.array_equal(got, want)
 
 
+def test_light_pdf_two_small_lights_exact(ctx, dev):
+    """Two emitting objects with 14 triangles in all: light_pdf runs the reference's loop
+    in its order, so the sums equal the oracle's bit for bit."""
+    sc = W.tracer_box(sphere_level=1)
+    objs = [dict(o) for o in sc["objects"]]
+    objs[7]["emissive"] = np.array([0.5, 0.5, 0.5], np.float32)   # block (12 tris)
+    scene = T.Scene(ctx, objs)
+    P, D = _light_queries({"objects": objs}, 20000, 3)
+    got = scene.light_pdf(torch.from_numpy(P).to(dev), torch.from_numpy(D).to(dev))
+    torch.cuda.synchronize()
+    want = O.light_pdf(objs, P, D)
+    assert (want > 0).sum() > 5000
+    assert np.array_equal(got.cpu().numpy(), want)
+
+
 def test_light_pdf_several_lights(ctx, dev):
-    """Three emitting objects with different triangle counts (per-light normalisation)."""
+    """Three emitting objects with different triangle counts (per-light normalisation);
+    94 light triangles, so the light BVH answers."""
     sc = W.tracer_box(sphere_level=1)
     objs = [dict(o) for o in sc["objects"]]
     objs[6]["emissive"] = np.array([1.0, 2.0, 3.0], np.float32)   # glossy sphere (80 tris)
