@@ -643,14 +643,20 @@ struct gpurt_scene_s {
     size_t cap = 0;
     void* wf = nullptr;
     void* trav = nullptr;
+    void* trav2 = nullptr;  // scratch of the light-ray traversal on `aux`
     size_t trav_bytes = 0;
-    uint32_t* counts = nullptr;            // device: list lengths of one sample (kCountSlots)
+    hipStream_t aux = nullptr;  // shadow / light rays, traced beside the main chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done[2] = {nullptr, nullptr};
+    uint32_t* counts = nullptr;            // device: list lengths of a sample (2 x kCountSlots)
     unsigned long long* tally = nullptr;   // device [2]: rays traced (ray_counts)
     ~gpurt_scene_s() {
         if (lbvh) gpurt_bvh_destroy(lbvh);
         for (void* p : {(void*)objs, (void*)verts, (void*)idx, (void*)lights, (void*)lworld, (void*)ltri_n, wf, trav,
-                        (void*)counts, (void*)tally})
+                        trav2, (void*)counts, (void*)tally})
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : {ev_fork, ev_join, ev_done[0], ev_done[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (aux) (void)hipStreamDestroy(aux);
     }
     SceneDev dev() const {
         SceneDev s{};
@@ -682,6 +688,13 @@ struct Wave {
     float4 *sh_rays, *sh_contrib;
     uint32_t* sh_ids;
     uint8_t* occ;
+    int32_t *sh_obj, *sh_prim;  // closest hits of the MIS light rays
+    float* sh_t;
+    // direct lighting keeps two samples in flight: the second set of the per-sample
+    // buffers its shadow-ray stage reads
+    float4 *accs2, *sh_rays2, *sh_contrib2;
+    uint32_t* sh_ids2;
+    uint8_t* occ2;
 };
 
 size_t carve(void* base, size_t P, Wave* w) {
@@ -708,6 +721,14 @@ size_t carve(void* base, size_t P, Wave* w) {
     t.sh_contrib = (float4*)take(16 * P);
     t.sh_ids = (uint32_t*)take(4 * P);
     t.occ = (uint8_t*)take(P);
+    t.sh_obj = (int32_t*)take(4 * P);
+    t.sh_prim = (int32_t*)take(4 * P);
+    t.accs2 = (float4*)take(16 * P);
+    t.sh_rays2 = (float4*)take(32 * P);
+    t.sh_contrib2 = (float4*)take(16 * P);
+    t.sh_ids2 = (uint32_t*)take(4 * P);
+    t.occ2 = (uint8_t*)take(P);
+    t.sh_t = (float*)take(4 * P);
     if (w) *w = t;
     return off;
 }
@@ -811,7 +832,7 @@ int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_ob
     if (hipMalloc((void**)&S->objs, sizeof(ObjDev) * n_objects) != hipSuccess ||
         hipMalloc((void**)&S->verts, 48 * std::max<uint64_t>(nv, 1)) != hipSuccess ||
         hipMalloc((void**)&S->idx, 4 * std::max<uint64_t>(ni, 1)) != hipSuccess ||
-        hipMalloc((void**)&S->counts, kCountSlots * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&S->counts, 2 * kCountSlots * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&S->tally, 2 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&bad, sizeof(uint32_t)) != hipSuccess ||
         (nl && (hipMalloc((void**)&S->lights, sizeof(LightDev) * lights.size()) != hipSuccess ||
@@ -917,20 +938,31 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t P = (uint32_t)r.width * (uint32_t)r.height;
+    if (!S->aux) {
+        if (hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&S->ev_done[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&S->ev_done[1], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(GPURT_E_DEVICE, "creating the light-ray stream failed");
+        }
+    }
     if (P > S->cap) {
         GPURT_HIP(hipStreamSynchronize(s));
-        for (void** q : {&S->wf, &S->trav})
+        GPURT_HIP(hipStreamSynchronize(S->aux));
+        for (void** q : {&S->wf, &S->trav, &S->trav2})
             if (*q) (void)hipFree(*q);
-        S->wf = S->trav = nullptr;
+        S->wf = S->trav = S->trav2 = nullptr;
         S->cap = 0;
         S->trav_bytes = tlas_scratch_bytes(S->tlas, P);
         if (S->trav_bytes == 0) return set_error(GPURT_E_INVALID, "cannot plan the traversal");
         if (hipMalloc(&S->wf, carve(nullptr, P, nullptr)) != hipSuccess ||
-            hipMalloc(&S->trav, S->trav_bytes) != hipSuccess) {
+            hipMalloc(&S->trav, S->trav_bytes) != hipSuccess || hipMalloc(&S->trav2, S->trav_bytes) != hipSuccess) {
             (void)hipGetLastError();
-            for (void** q : {&S->wf, &S->trav})
+            for (void** q : {&S->wf, &S->trav, &S->trav2})
                 if (*q) (void)hipFree(*q);
-            S->wf = S->trav = nullptr;
+            S->wf = S->trav = S->trav2 = nullptr;
             return set_error(GPURT_E_OOM, "wavefront state allocation failed");
         }
         S->cap = P;
@@ -977,18 +1009,92 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     const int bounces = r.integrator == 0 ? std::min(r.max_depth, 1) : r.max_depth;
     const uint32_t n_counts = 2u * (uint32_t)r.max_depth + 2u;
     int rc = GPURT_OK;
-    for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
+    if (r.integrator == 0 && r.max_depth >= 1) {
+        // Direct lighting ends every path at its first hit, so a sample is camera rays ->
+        // closest hit -> shade -> shadow rays.  Sample s's shadow rays, their
+        // contributions and its pixel sum run on `aux` while sample s+1's camera rays are
+        // traced and shaded on `s` (the next sample only needs the RNG states the shading
+        // left); the per-sample buffers the shadow stage reads alternate between two sets.
+        for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
+            const int par = smp & 1;
+            float4* accs = par ? w.accs2 : w.accs;
+            float4* shr = par ? w.sh_rays2 : w.sh_rays;
+            float4* shc = par ? w.sh_contrib2 : w.sh_contrib;
+            uint32_t* shi = par ? w.sh_ids2 : w.sh_ids;
+            uint8_t* occ = par ? w.occ2 : w.occ;
+            uint32_t* cnt = S->counts + par * kCountSlots;
+            if (smp >= 2) GPURT_HIP(hipStreamWaitEvent(s, S->ev_done[par], 0));  // sample smp-2 is done with them
+            c.sample = (uint32_t)smp;
+            hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, cnt, 4u, P);
+            hipLaunchKernelGGL(k_camera, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, c, w.seeds, w.rays[0], w.ids[0],
+                               w.thrmis, accs);
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.rays[0], P, cnt, w.hobj, w.hprim, w.ht, w.huv, nullptr,
+                               false, nullptr, s, S->trav, S->trav_bytes);
+            if (rc) break;
+            a.depth = 0;
+            a.sample = smp;
+            a.n_in = cnt;
+            a.n_next = cnt + 2;
+            a.n_shadow = cnt + 1;
+            a.ids_in = w.ids[0];
+            a.rays_in = w.rays[0];
+            a.rays_out = w.rays[1];
+            a.ids_out = w.ids[1];
+            a.accs = accs;
+            a.sh_rays = shr;
+            a.sh_ids = shi;
+            a.sh_contrib = shc;
+            if (r.brdf == 0)
+                launch_shade<0, 0>(a, P, s);
+            else
+                launch_shade<0, 1>(a, P, s);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(S->ev_fork, s) != hipSuccess ||
+                hipStreamWaitEvent(S->aux, S->ev_fork, 0) != hipSuccess) {
+                rc = hip_error(hipGetLastError(), "shade launch");
+                break;
+            }
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)shr, P, cnt + 1, nullptr, nullptr, nullptr, nullptr, occ,
+                               true, nullptr, S->aux, S->trav2, S->trav_bytes);
+            if (rc) break;
+            hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, S->aux, cnt + 1, shi, shc,
+                               occ, accs);
+            if (r.ray_counts) hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, S->aux, cnt, 1u, S->tally);
+            hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, S->aux, P, accs, w.pix);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(S->ev_done[par], S->aux) != hipSuccess) {
+                rc = hip_error(hipGetLastError(), "shadow stage");
+                break;
+            }
+        }
+        (void)hipStreamWaitEvent(s, S->ev_done[0], 0);
+        (void)hipStreamWaitEvent(s, S->ev_done[1], 0);
+    }
+    for (int smp = 0; smp < r.samples && rc == GPURT_OK && !(r.integrator == 0 && r.max_depth >= 1); smp++) {
         c.sample = (uint32_t)smp;
         hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, S->counts, n_counts, P);
         hipLaunchKernelGGL(k_camera, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, c, w.seeds, w.rays[0], w.ids[0],
                            w.thrmis, w.accs);
         int cur = 0;
+        // MIS: the light rays of bounce d are traced on `aux` while bounce d+1's rays are
+        // traced on `s`; their contributions are added before bounce d+1 is shaded (the
+        // shader's order of additions)
+        auto resolve_light = [&](int d) {
+            GPURT_HIP(hipStreamWaitEvent(s, S->ev_join, 0));
+            hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, S->counts + 2 * d + 1,
+                               w.sh_ids, w.sh_contrib, w.sh_obj, S->objs, r.env_light[0], r.env_light[1],
+                               r.env_light[2], w.accs);
+            return GPURT_OK;
+        };
+        bool pending_light = false;
         for (int depth = 0; depth < bounces; depth++) {
             const uint32_t* n_in = S->counts + 2 * depth;
             uint32_t* n_sh = S->counts + 2 * depth + 1;
             rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.rays[cur], P, n_in, w.hobj, w.hprim, w.ht, w.huv,
                                nullptr, false, nullptr, s, S->trav, S->trav_bytes);
             if (rc) break;
+            if (pending_light) {
+                if ((rc = resolve_light(depth - 1))) break;
+                pending_light = false;
+            }
             a.depth = depth;
             a.sample = smp;
             a.n_in = n_in;
@@ -1014,24 +1120,38 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
                 cur ^= 1;
                 continue;
             }
-            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, w.hobj, w.hprim, w.ht, w.huv, w.occ,
-                               r.integrator == 0, nullptr, s, S->trav, S->trav_bytes);
-            if (rc) break;
-            if (r.integrator == 0)
+            if (r.integrator == 0) {
+                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, nullptr, nullptr, nullptr, nullptr,
+                                   w.occ, true, nullptr, s, S->trav, S->trav_bytes);
+                if (rc) break;
                 hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
                                    w.sh_contrib, w.occ, w.accs);
-            else
-                hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
-                                   w.sh_contrib, w.hobj, S->objs, r.env_light[0], r.env_light[1], r.env_light[2],
-                                   w.accs);
+            } else {
+                if (hipEventRecord(S->ev_fork, s) != hipSuccess || hipStreamWaitEvent(S->aux, S->ev_fork, 0) != hipSuccess) {
+                    rc = hip_error(hipGetLastError(), "light-ray stream fork");
+                    break;
+                }
+                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, w.sh_obj, w.sh_prim, w.sh_t, nullptr,
+                                   nullptr, false, nullptr, S->aux, S->trav2, S->trav_bytes);
+                if (rc) break;
+                if (hipEventRecord(S->ev_join, S->aux) != hipSuccess) {
+                    rc = hip_error(hipGetLastError(), "light-ray stream join");
+                    break;
+                }
+                pending_light = true;
+            }
             cur ^= 1;
         }
+        if (rc == GPURT_OK && pending_light) rc = resolve_light(bounces - 1);
         if (rc == GPURT_OK && r.ray_counts)
             hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, S->counts, (uint32_t)bounces, S->tally);
         if (rc) break;
         hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, w.accs, w.pix);
     }
-    if (rc) return rc;
+    if (rc) {
+        (void)hipStreamWaitEvent(s, S->ev_join, 0);  // no light-ray work outlives the call's stream order
+        return rc;
+    }
     hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, (uint32_t)r.samples, (uint32_t)r.frame,
                        w.pix, img);
     GPURT_HIP(hipGetLastError());
