@@ -39,6 +39,12 @@ using pt::kLargeDist;
 
 constexpr int kShadeBlock = 256;
 constexpr int kCountSlots = 2 * 64 + 2;  // two list lengths per bounce, up to 64 bounces
+// Pixel groups rendered concurrently, by integrator (measured at 1024x1024 on MI355X with
+// HIP's 4 hardware queues per process: direct 4.56 / 4.22 / 4.57 ms for 1 / 2 / 3 groups,
+// material sampling 14.8 / 14.4 / 14.2 / 13.8 ms for 1..4, MIS 21.4 / 26.9 ms for 1 / 2 --
+// MIS already runs two streams per group, and more streams than hardware queues
+// serialise)
+constexpr int kDefaultGroups[3] = {2, 4, 1};
 constexpr int kLightStack = 64;  // light BVH traversal stack (checked against its depth)
 constexpr float kExitSlack = 0x1p-20f;  // relative slack on box exits (conservative culls)
 constexpr uint32_t kLoopLightTris = 32;  // up to this many light triangles light_pdf loops
@@ -216,13 +222,14 @@ __global__ __launch_bounds__(kBlock) void k_frame_init(uint32_t P, uint32_t seed
 struct CamArgs {
     float iv[16], ip[16];
     uint32_t w, h, frame, qmc, sample, samples, max_frame;
+    uint32_t p0, np;  // the pixel group [p0, p0 + np)
 };
 
 __global__ __launch_bounds__(kBlock) void k_camera(CamArgs c, uint32_t* __restrict__ seeds, float4* __restrict__ rays,
                                                    uint32_t* __restrict__ ids, float4* __restrict__ thrmis,
                                                    float4* __restrict__ accs) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= c.w * c.h) return;
+    const uint32_t p = c.p0 + blockIdx.x * kBlock + threadIdx.x;
+    if (p >= c.p0 + c.np) return;
     const uint32_t x = p % c.w, y = p / c.w;
     uint32_t seed = seeds[p];
     float jx, jy;
@@ -245,9 +252,10 @@ __global__ __launch_bounds__(kBlock) void k_camera(CamArgs c, uint32_t* __restri
     const float4 dir = pt::xform4(c.iv, target.x, target.y, target.z, 0.0f);
     const float4 org = pt::xform4(c.iv, 0.0f, 0.0f, 0.0f, 1.0f);
     const V3 d = pt::normalize(v3(dir.x, dir.y, dir.z));
-    rays[2 * (size_t)p] = make_float4(org.x, org.y, org.z, kEps);
-    rays[2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, kLargeDist);
-    ids[p] = p;
+    const uint32_t slot = p - c.p0;  // rays / ids point at the group's list
+    rays[2 * (size_t)slot] = make_float4(org.x, org.y, org.z, kEps);
+    rays[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, kLargeDist);
+    ids[slot] = p;
     thrmis[p] = make_float4(1.f, 1.f, 1.f, 1.f);
     accs[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     seeds[p] = seed;
@@ -588,8 +596,8 @@ __global__ void k_tally(const uint32_t* counts, uint32_t depth, unsigned long lo
         b += counts[2 * d];
         sh += counts[2 * d + 1];
     }
-    tally[0] += b;
-    tally[1] += sh;
+    atomicAdd(&tally[0], b);  // several pixel groups add concurrently
+    atomicAdd(&tally[1], sh);
 }
 
 struct DevScope {
@@ -626,6 +634,48 @@ bool inverse_transpose3(const float* m16, float out[9]) {
 
 using namespace gpurt_impl;
 
+namespace {
+
+// One pixel group's streams and events (the main chain on `st`; shadow / light rays on
+// `aux`), traversal scratch for each, and list lengths (two sets of kCountSlots).
+struct Lane {
+    hipStream_t st = nullptr, aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, done[2] = {nullptr, nullptr}, end = nullptr;
+    void* trav = nullptr;
+    void* trav2 = nullptr;
+    uint32_t* counts = nullptr;
+    bool create() {
+        return hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+               hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) == hipSuccess &&
+               hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&join, hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&done[0], hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&done[1], hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&end, hipEventDisableTiming) == hipSuccess &&
+               hipMalloc((void**)&counts, 2 * kCountSlots * sizeof(uint32_t)) == hipSuccess;
+    }
+    void free_scratch() {
+        for (void** q : {&trav, &trav2})
+            if (*q) (void)hipFree(*q);
+        trav = trav2 = nullptr;
+    }
+    void release() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (aux) (void)hipStreamSynchronize(aux);
+        free_scratch();
+        if (counts) (void)hipFree(counts);
+        counts = nullptr;
+        for (hipEvent_t e : {fork, join, done[0], done[1], end})
+            if (e) (void)hipEventDestroy(e);
+        fork = join = done[0] = done[1] = end = nullptr;
+        for (hipStream_t q : {st, aux})
+            if (q) (void)hipStreamDestroy(q);
+        st = aux = nullptr;
+    }
+};
+
+}  // namespace
+
 struct gpurt_scene_s {
     gpurt_ctx ctx = nullptr;
     gpurt_tlas tlas = nullptr;
@@ -639,24 +689,23 @@ struct gpurt_scene_s {
     float4* lworld = nullptr;
     float* ltri_n = nullptr;
     gpurt_bvh lbvh = nullptr;
-    // wavefront state and traversal scratch, grown on demand
+    // wavefront state (per pixel and per list slot), grown on demand
     size_t cap = 0;
     void* wf = nullptr;
-    void* trav = nullptr;
-    void* trav2 = nullptr;  // scratch of the light-ray traversal on `aux`
+    // pixel groups rendered concurrently, each on its own streams with its own traversal
+    // scratch and list lengths
+    std::vector<Lane> lanes;
+    size_t lane_cap = 0;  // pixels per group the traversal scratch is sized for
     size_t trav_bytes = 0;
-    hipStream_t aux = nullptr;  // shadow / light rays, traced beside the main chain
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done[2] = {nullptr, nullptr};
-    uint32_t* counts = nullptr;            // device: list lengths of a sample (2 x kCountSlots)
-    unsigned long long* tally = nullptr;   // device [2]: rays traced (ray_counts)
+    hipEvent_t ev_start = nullptr;
+    unsigned long long* tally = nullptr;  // device [2]: rays traced (ray_counts)
     ~gpurt_scene_s() {
         if (lbvh) gpurt_bvh_destroy(lbvh);
-        for (void* p : {(void*)objs, (void*)verts, (void*)idx, (void*)lights, (void*)lworld, (void*)ltri_n, wf, trav,
-                        trav2, (void*)counts, (void*)tally})
+        for (void* p : {(void*)objs, (void*)verts, (void*)idx, (void*)lights, (void*)lworld, (void*)ltri_n, wf,
+                        (void*)tally})
             if (p) (void)hipFree(p);
-        for (hipEvent_t e : {ev_fork, ev_join, ev_done[0], ev_done[1]})
-            if (e) (void)hipEventDestroy(e);
-        if (aux) (void)hipStreamDestroy(aux);
+        for (Lane& l : lanes) l.release();
+        if (ev_start) (void)hipEventDestroy(ev_start);
     }
     SceneDev dev() const {
         SceneDev s{};
@@ -739,6 +788,170 @@ void launch_shade(const WaveArgs& a, uint32_t capacity, hipStream_t s) {
 }
 
 }  // namespace
+
+// One pixel group [p0, p0 + np) of a frame, every sample of it, on the group's streams.
+// The group's ray lists are the slots [p0, p0 + np) of the list arrays; per-pixel state is
+// indexed by pixel.  Samples of a pixel run in order (the shader's RNG order).
+static int render_group(gpurt_scene S, Lane& L, const gpurt_render_params& r, const Wave& w, WaveArgs a, CamArgs c,
+                        uint32_t p0, uint32_t np) {
+    hipStream_t s = L.st;
+    c.p0 = p0;
+    c.np = np;
+    const size_t o = p0;  // list offset of this group
+    float4* rays[2] = {w.rays[0] + 2 * o, w.rays[1] + 2 * o};
+    uint32_t* ids[2] = {w.ids[0] + o, w.ids[1] + o};
+    int32_t* hobj = w.hobj + o;
+    int32_t* hprim = w.hprim + o;
+    float* huv = w.huv + 2 * o;
+    a.hobj = hobj;
+    a.hprim = hprim;
+    a.huv = huv;
+    int rc = GPURT_OK;
+    if (r.integrator == 0 && r.max_depth >= 1) {
+        // Direct lighting ends every path at its first hit, so a sample is camera rays ->
+        // closest hit -> shade -> shadow rays.  Sample s's shadow rays, their
+        // contributions and its pixel sum run on `aux` while sample s+1's camera rays are
+        // traced and shaded (the next sample only needs the RNG states the shading left);
+        // the per-sample buffers the shadow stage reads alternate between two sets.
+        for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
+            const int par = smp & 1;
+            float4* accs = par ? w.accs2 : w.accs;
+            float4* shr = (par ? w.sh_rays2 : w.sh_rays) + 2 * o;
+            float4* shc = (par ? w.sh_contrib2 : w.sh_contrib) + o;
+            uint32_t* shi = (par ? w.sh_ids2 : w.sh_ids) + o;
+            uint8_t* occ = (par ? w.occ2 : w.occ) + o;
+            uint32_t* cnt = L.counts + par * kCountSlots;
+            if (smp >= 2) GPURT_HIP(hipStreamWaitEvent(s, L.done[par], 0));  // sample smp-2 is done with them
+            c.sample = (uint32_t)smp;
+            hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, cnt, 4u, np);
+            hipLaunchKernelGGL(k_camera, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, s, c, w.seeds, rays[0], ids[0],
+                               w.thrmis, accs);
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)rays[0], np, cnt, hobj, hprim, w.ht + o, huv, nullptr,
+                               false, nullptr, s, L.trav, S->trav_bytes);
+            if (rc) break;
+            a.depth = 0;
+            a.sample = smp;
+            a.n_in = cnt;
+            a.n_next = cnt + 2;
+            a.n_shadow = cnt + 1;
+            a.ids_in = ids[0];
+            a.rays_in = rays[0];
+            a.rays_out = rays[1];
+            a.ids_out = ids[1];
+            a.accs = accs;
+            a.sh_rays = shr;
+            a.sh_ids = shi;
+            a.sh_contrib = shc;
+            if (r.brdf == 0)
+                launch_shade<0, 0>(a, np, s);
+            else
+                launch_shade<0, 1>(a, np, s);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(L.fork, s) != hipSuccess ||
+                hipStreamWaitEvent(L.aux, L.fork, 0) != hipSuccess) {
+                rc = hip_error(hipGetLastError(), "shade launch");
+                break;
+            }
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)shr, np, cnt + 1, nullptr, nullptr, nullptr, nullptr, occ,
+                               true, nullptr, L.aux, L.trav2, S->trav_bytes);
+            if (rc) break;
+            hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, L.aux, cnt + 1, shi, shc, occ,
+                               accs);
+            if (r.ray_counts) hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, L.aux, cnt, 1u, S->tally);
+            hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, L.aux, np, accs + o, w.pix + o);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(L.done[par], L.aux) != hipSuccess) {
+                rc = hip_error(hipGetLastError(), "shadow stage");
+                break;
+            }
+        }
+        (void)hipStreamWaitEvent(s, L.done[0], 0);
+        (void)hipStreamWaitEvent(s, L.done[1], 0);
+        return rc;
+    }
+    a.accs = w.accs;
+    a.sh_rays = w.sh_rays + 2 * o;
+    a.sh_ids = w.sh_ids + o;
+    a.sh_contrib = w.sh_contrib + o;
+    const int bounces = r.max_depth;
+    const uint32_t n_counts = 2u * (uint32_t)r.max_depth + 2u;
+    // MIS: the light rays of bounce d are traced on `aux` while bounce d+1's rays are
+    // traced on `s`; their contributions are added before bounce d+1 is shaded (the
+    // shader's order of additions)
+    auto resolve_light = [&](int d) {
+        GPURT_HIP(hipStreamWaitEvent(s, L.join, 0));
+        hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, s, L.counts + 2 * d + 1,
+                           w.sh_ids + o, w.sh_contrib + o, w.sh_obj + o, S->objs, r.env_light[0], r.env_light[1],
+                           r.env_light[2], w.accs);
+        return GPURT_OK;
+    };
+    for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
+        c.sample = (uint32_t)smp;
+        hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, L.counts, n_counts, np);
+        hipLaunchKernelGGL(k_camera, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, s, c, w.seeds, rays[0], ids[0], w.thrmis,
+                           w.accs);
+        int cur = 0;
+        bool pending_light = false;
+        for (int depth = 0; depth < bounces; depth++) {
+            const uint32_t* n_in = L.counts + 2 * depth;
+            uint32_t* n_sh = L.counts + 2 * depth + 1;
+            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)rays[cur], np, n_in, hobj, hprim, w.ht + o, huv,
+                               nullptr, false, nullptr, s, L.trav, S->trav_bytes);
+            if (rc) break;
+            if (pending_light) {
+                if ((rc = resolve_light(depth - 1))) break;
+                pending_light = false;
+            }
+            a.depth = depth;
+            a.sample = smp;
+            a.n_in = n_in;
+            a.n_next = L.counts + 2 * (depth + 1);
+            a.n_shadow = n_sh;
+            a.ids_in = ids[cur];
+            a.rays_in = rays[cur];
+            a.rays_out = rays[cur ^ 1];
+            a.ids_out = ids[cur ^ 1];
+            switch (r.integrator * 2 + r.brdf) {
+                case 0: launch_shade<0, 0>(a, np, s); break;
+                case 1: launch_shade<0, 1>(a, np, s); break;
+                case 2: launch_shade<1, 0>(a, np, s); break;
+                case 3: launch_shade<1, 1>(a, np, s); break;
+                case 4: launch_shade<2, 0>(a, np, s); break;
+                default: launch_shade<2, 1>(a, np, s); break;
+            }
+            if (hipGetLastError() != hipSuccess) {
+                rc = hip_error(hipGetLastError(), "shade launch");
+                break;
+            }
+            if (r.integrator == 2) {
+                if (hipEventRecord(L.fork, s) != hipSuccess || hipStreamWaitEvent(L.aux, L.fork, 0) != hipSuccess) {
+                    rc = hip_error(hipGetLastError(), "light-ray stream fork");
+                    break;
+                }
+                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)(w.sh_rays + 2 * o), np, n_sh, w.sh_obj + o,
+                                   w.sh_prim + o, w.sh_t + o, nullptr, nullptr, false, nullptr, L.aux, L.trav2,
+                                   S->trav_bytes);
+                if (rc) break;
+                if (hipEventRecord(L.join, L.aux) != hipSuccess) {
+                    rc = hip_error(hipGetLastError(), "light-ray stream join");
+                    break;
+                }
+                pending_light = true;
+            } else if (r.integrator == 0) {  // direct lighting with max_depth 0 never gets here
+                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)(w.sh_rays + 2 * o), np, n_sh, nullptr, nullptr,
+                                   nullptr, nullptr, w.occ + o, true, nullptr, s, L.trav, S->trav_bytes);
+                if (rc) break;
+                hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids + o,
+                                   w.sh_contrib + o, w.occ + o, w.accs);
+            }
+            cur ^= 1;
+        }
+        if (rc == GPURT_OK && pending_light) rc = resolve_light(bounces - 1);
+        if (rc) break;
+        if (r.ray_counts) hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, L.counts, (uint32_t)bounces, S->tally);
+        hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, s, np, w.accs + o, w.pix + o);
+    }
+    (void)hipStreamWaitEvent(s, L.join, 0);  // no light-ray work outlives the group's stream order
+    return rc;
+}
 
 extern "C" {
 
@@ -832,7 +1045,6 @@ int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_ob
     if (hipMalloc((void**)&S->objs, sizeof(ObjDev) * n_objects) != hipSuccess ||
         hipMalloc((void**)&S->verts, 48 * std::max<uint64_t>(nv, 1)) != hipSuccess ||
         hipMalloc((void**)&S->idx, 4 * std::max<uint64_t>(ni, 1)) != hipSuccess ||
-        hipMalloc((void**)&S->counts, 2 * kCountSlots * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&S->tally, 2 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&bad, sizeof(uint32_t)) != hipSuccess ||
         (nl && (hipMalloc((void**)&S->lights, sizeof(LightDev) * lights.size()) != hipSuccess ||
@@ -927,6 +1139,7 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     if (r.integrator < 0 || r.integrator > 2) return set_error(GPURT_E_INVALID, "integrator must be 0, 1 or 2");
     if (r.brdf < 0 || r.brdf > 1) return set_error(GPURT_E_INVALID, "brdf must be 0 or 1");
     if (r.qmc && r.max_frame < 1) return set_error(GPURT_E_INVALID, "qmc needs max_frame >= 1");
+    if (r.concurrency < 0 || r.concurrency > 8) return set_error(GPURT_E_INVALID, "concurrency must be 0..8");
     if ((gbuf_pos != nullptr) != (gbuf_norm != nullptr) || (gbuf_pos != nullptr) != (gbuf_albedo != nullptr))
         return set_error(GPURT_E_INVALID, "give all three G-buffer images or none");
     for (const float* q : {image, gbuf_pos, gbuf_norm, gbuf_albedo})
@@ -938,41 +1151,59 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t P = (uint32_t)r.width * (uint32_t)r.height;
-    if (!S->aux) {
-        if (hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&S->ev_done[0], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&S->ev_done[1], hipEventDisableTiming) != hipSuccess) {
+    const int G = std::max(1, std::min<int>(r.concurrency > 0 ? r.concurrency : kDefaultGroups[r.integrator],
+                                            (int)std::min<uint32_t>(P, 8u)));
+    if (!S->ev_start && hipEventCreateWithFlags(&S->ev_start, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GPURT_E_DEVICE, "creating a HIP event failed");
+    }
+    while ((int)S->lanes.size() < G) {
+        S->lanes.emplace_back();
+        if (!S->lanes.back().create()) {
             (void)hipGetLastError();
-            return set_error(GPURT_E_DEVICE, "creating the light-ray stream failed");
+            S->lanes.back().release();
+            S->lanes.pop_back();
+            return set_error(GPURT_E_DEVICE, "creating the streams of a pixel group failed");
         }
     }
-    if (P > S->cap) {
+    const uint32_t per = (uint32_t)cdiv(P, G);  // pixels per group (the last may have fewer)
+    if (P > S->cap || per > S->lane_cap) {
         GPURT_HIP(hipStreamSynchronize(s));
-        GPURT_HIP(hipStreamSynchronize(S->aux));
-        for (void** q : {&S->wf, &S->trav, &S->trav2})
-            if (*q) (void)hipFree(*q);
-        S->wf = S->trav = S->trav2 = nullptr;
-        S->cap = 0;
-        S->trav_bytes = tlas_scratch_bytes(S->tlas, P);
+        for (Lane& l : S->lanes) {
+            GPURT_HIP(hipStreamSynchronize(l.st));
+            GPURT_HIP(hipStreamSynchronize(l.aux));
+            l.free_scratch();
+        }
+        if (S->wf) (void)hipFree(S->wf);
+        S->wf = nullptr;
+        S->cap = S->lane_cap = 0;
+        S->trav_bytes = tlas_scratch_bytes(S->tlas, per);
         if (S->trav_bytes == 0) return set_error(GPURT_E_INVALID, "cannot plan the traversal");
-        if (hipMalloc(&S->wf, carve(nullptr, P, nullptr)) != hipSuccess ||
-            hipMalloc(&S->trav, S->trav_bytes) != hipSuccess || hipMalloc(&S->trav2, S->trav_bytes) != hipSuccess) {
+        bool ok = hipMalloc(&S->wf, carve(nullptr, P, nullptr)) == hipSuccess;
+        for (Lane& l : S->lanes)
+            ok = ok && hipMalloc(&l.trav, S->trav_bytes) == hipSuccess && hipMalloc(&l.trav2, S->trav_bytes) == hipSuccess;
+        if (!ok) {
             (void)hipGetLastError();
-            for (void** q : {&S->wf, &S->trav, &S->trav2})
-                if (*q) (void)hipFree(*q);
-            S->wf = S->trav = S->trav2 = nullptr;
+            for (Lane& l : S->lanes) l.free_scratch();
+            if (S->wf) (void)hipFree(S->wf);
+            S->wf = nullptr;
             return set_error(GPURT_E_OOM, "wavefront state allocation failed");
         }
         S->cap = P;
+        S->lane_cap = per;
     }
+    for (int g = 0; g < G; g++)  // groups created since the last sizing
+        if (!S->lanes[g].trav && (hipMalloc(&S->lanes[g].trav, S->trav_bytes) != hipSuccess ||
+                                  hipMalloc(&S->lanes[g].trav2, S->trav_bytes) != hipSuccess)) {
+            (void)hipGetLastError();
+            return set_error(GPURT_E_OOM, "traversal scratch allocation failed");
+        }
     Wave w;
-    carve(S->wf, P, &w);
-    float4* img = (float4*)image;
+    carve(S->wf, S->cap, &w);
     hipLaunchKernelGGL(k_frame_init, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, r.seed, w.seeds, w.pix,
                        (float4*)gbuf_pos, (float4*)gbuf_norm, (float4*)gbuf_albedo);
     if (r.ray_counts) GPURT_HIP(hipMemsetAsync(S->tally, 0, 2 * sizeof(unsigned long long), s));
+    GPURT_HIP(hipEventRecord(S->ev_start, s));
     CamArgs c{};
     for (int k = 0; k < 16; k++) {
         c.iv[k] = r.inv_view[k];
@@ -995,165 +1226,25 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
     a.use_metalness = r.use_metalness ? 1 : 0;
     a.seeds = w.seeds;
     a.thrmis = w.thrmis;
-    a.accs = w.accs;
-    a.hobj = w.hobj;
-    a.hprim = w.hprim;
-    a.huv = w.huv;
-    a.sh_rays = w.sh_rays;
-    a.sh_ids = w.sh_ids;
-    a.sh_contrib = w.sh_contrib;
     a.gpos = (float4*)gbuf_pos;
     a.gnorm = (float4*)gbuf_norm;
     a.galb = (float4*)gbuf_albedo;
-    // integrate_direct ends every path at its first hit: one bounce launch suffices
-    const int bounces = r.integrator == 0 ? std::min(r.max_depth, 1) : r.max_depth;
-    const uint32_t n_counts = 2u * (uint32_t)r.max_depth + 2u;
     int rc = GPURT_OK;
-    if (r.integrator == 0 && r.max_depth >= 1) {
-        // Direct lighting ends every path at its first hit, so a sample is camera rays ->
-        // closest hit -> shade -> shadow rays.  Sample s's shadow rays, their
-        // contributions and its pixel sum run on `aux` while sample s+1's camera rays are
-        // traced and shaded on `s` (the next sample only needs the RNG states the shading
-        // left); the per-sample buffers the shadow stage reads alternate between two sets.
-        for (int smp = 0; smp < r.samples && rc == GPURT_OK; smp++) {
-            const int par = smp & 1;
-            float4* accs = par ? w.accs2 : w.accs;
-            float4* shr = par ? w.sh_rays2 : w.sh_rays;
-            float4* shc = par ? w.sh_contrib2 : w.sh_contrib;
-            uint32_t* shi = par ? w.sh_ids2 : w.sh_ids;
-            uint8_t* occ = par ? w.occ2 : w.occ;
-            uint32_t* cnt = S->counts + par * kCountSlots;
-            if (smp >= 2) GPURT_HIP(hipStreamWaitEvent(s, S->ev_done[par], 0));  // sample smp-2 is done with them
-            c.sample = (uint32_t)smp;
-            hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, cnt, 4u, P);
-            hipLaunchKernelGGL(k_camera, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, c, w.seeds, w.rays[0], w.ids[0],
-                               w.thrmis, accs);
-            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.rays[0], P, cnt, w.hobj, w.hprim, w.ht, w.huv, nullptr,
-                               false, nullptr, s, S->trav, S->trav_bytes);
-            if (rc) break;
-            a.depth = 0;
-            a.sample = smp;
-            a.n_in = cnt;
-            a.n_next = cnt + 2;
-            a.n_shadow = cnt + 1;
-            a.ids_in = w.ids[0];
-            a.rays_in = w.rays[0];
-            a.rays_out = w.rays[1];
-            a.ids_out = w.ids[1];
-            a.accs = accs;
-            a.sh_rays = shr;
-            a.sh_ids = shi;
-            a.sh_contrib = shc;
-            if (r.brdf == 0)
-                launch_shade<0, 0>(a, P, s);
-            else
-                launch_shade<0, 1>(a, P, s);
-            if (hipGetLastError() != hipSuccess || hipEventRecord(S->ev_fork, s) != hipSuccess ||
-                hipStreamWaitEvent(S->aux, S->ev_fork, 0) != hipSuccess) {
-                rc = hip_error(hipGetLastError(), "shade launch");
-                break;
-            }
-            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)shr, P, cnt + 1, nullptr, nullptr, nullptr, nullptr, occ,
-                               true, nullptr, S->aux, S->trav2, S->trav_bytes);
-            if (rc) break;
-            hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, S->aux, cnt + 1, shi, shc,
-                               occ, accs);
-            if (r.ray_counts) hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, S->aux, cnt, 1u, S->tally);
-            hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, S->aux, P, accs, w.pix);
-            if (hipGetLastError() != hipSuccess || hipEventRecord(S->ev_done[par], S->aux) != hipSuccess) {
-                rc = hip_error(hipGetLastError(), "shadow stage");
-                break;
-            }
+    for (int g = 0; g < G && rc == GPURT_OK; g++) {
+        Lane& L = S->lanes[g];
+        const uint32_t p0 = (uint32_t)g * per, np = std::min(P, p0 + per) - std::min(P, p0);
+        if (np == 0) continue;
+        if (hipStreamWaitEvent(L.st, S->ev_start, 0) != hipSuccess) {
+            rc = hip_error(hipGetLastError(), "pixel group fork");
+            break;
         }
-        (void)hipStreamWaitEvent(s, S->ev_done[0], 0);
-        (void)hipStreamWaitEvent(s, S->ev_done[1], 0);
+        rc = render_group(S, L, r, w, a, c, p0, np);
+        if (hipEventRecord(L.end, L.st) != hipSuccess && rc == GPURT_OK) rc = hip_error(hipGetLastError(), "join");
     }
-    for (int smp = 0; smp < r.samples && rc == GPURT_OK && !(r.integrator == 0 && r.max_depth >= 1); smp++) {
-        c.sample = (uint32_t)smp;
-        hipLaunchKernelGGL(k_sample_counts, dim3(1), dim3(64), 0, s, S->counts, n_counts, P);
-        hipLaunchKernelGGL(k_camera, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, c, w.seeds, w.rays[0], w.ids[0],
-                           w.thrmis, w.accs);
-        int cur = 0;
-        // MIS: the light rays of bounce d are traced on `aux` while bounce d+1's rays are
-        // traced on `s`; their contributions are added before bounce d+1 is shaded (the
-        // shader's order of additions)
-        auto resolve_light = [&](int d) {
-            GPURT_HIP(hipStreamWaitEvent(s, S->ev_join, 0));
-            hipLaunchKernelGGL(k_resolve_light, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, S->counts + 2 * d + 1,
-                               w.sh_ids, w.sh_contrib, w.sh_obj, S->objs, r.env_light[0], r.env_light[1],
-                               r.env_light[2], w.accs);
-            return GPURT_OK;
-        };
-        bool pending_light = false;
-        for (int depth = 0; depth < bounces; depth++) {
-            const uint32_t* n_in = S->counts + 2 * depth;
-            uint32_t* n_sh = S->counts + 2 * depth + 1;
-            rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.rays[cur], P, n_in, w.hobj, w.hprim, w.ht, w.huv,
-                               nullptr, false, nullptr, s, S->trav, S->trav_bytes);
-            if (rc) break;
-            if (pending_light) {
-                if ((rc = resolve_light(depth - 1))) break;
-                pending_light = false;
-            }
-            a.depth = depth;
-            a.sample = smp;
-            a.n_in = n_in;
-            a.n_next = S->counts + 2 * (depth + 1);
-            a.n_shadow = n_sh;
-            a.ids_in = w.ids[cur];
-            a.rays_in = w.rays[cur];
-            a.rays_out = w.rays[cur ^ 1];
-            a.ids_out = w.ids[cur ^ 1];
-            switch (r.integrator * 2 + r.brdf) {
-                case 0: launch_shade<0, 0>(a, P, s); break;
-                case 1: launch_shade<0, 1>(a, P, s); break;
-                case 2: launch_shade<1, 0>(a, P, s); break;
-                case 3: launch_shade<1, 1>(a, P, s); break;
-                case 4: launch_shade<2, 0>(a, P, s); break;
-                default: launch_shade<2, 1>(a, P, s); break;
-            }
-            if (hipGetLastError() != hipSuccess) {
-                rc = hip_error(hipGetLastError(), "shade launch");
-                break;
-            }
-            if (r.integrator == 1) {  // material sampling casts no shadow or light rays
-                cur ^= 1;
-                continue;
-            }
-            if (r.integrator == 0) {
-                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, nullptr, nullptr, nullptr, nullptr,
-                                   w.occ, true, nullptr, s, S->trav, S->trav_bytes);
-                if (rc) break;
-                hipLaunchKernelGGL(k_resolve_shadow, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, n_sh, w.sh_ids,
-                                   w.sh_contrib, w.occ, w.accs);
-            } else {
-                if (hipEventRecord(S->ev_fork, s) != hipSuccess || hipStreamWaitEvent(S->aux, S->ev_fork, 0) != hipSuccess) {
-                    rc = hip_error(hipGetLastError(), "light-ray stream fork");
-                    break;
-                }
-                rc = tlas_trace_ex(S->tlas, (const gpurt_ray*)w.sh_rays, P, n_sh, w.sh_obj, w.sh_prim, w.sh_t, nullptr,
-                                   nullptr, false, nullptr, S->aux, S->trav2, S->trav_bytes);
-                if (rc) break;
-                if (hipEventRecord(S->ev_join, S->aux) != hipSuccess) {
-                    rc = hip_error(hipGetLastError(), "light-ray stream join");
-                    break;
-                }
-                pending_light = true;
-            }
-            cur ^= 1;
-        }
-        if (rc == GPURT_OK && pending_light) rc = resolve_light(bounces - 1);
-        if (rc == GPURT_OK && r.ray_counts)
-            hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, S->counts, (uint32_t)bounces, S->tally);
-        if (rc) break;
-        hipLaunchKernelGGL(k_sum_sample, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, w.accs, w.pix);
-    }
-    if (rc) {
-        (void)hipStreamWaitEvent(s, S->ev_join, 0);  // no light-ray work outlives the call's stream order
-        return rc;
-    }
+    for (int g = 0; g < G; g++) (void)hipStreamWaitEvent(s, S->lanes[g].end, 0);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, (uint32_t)r.samples, (uint32_t)r.frame,
-                       w.pix, img);
+                       w.pix, (float4*)image);
     GPURT_HIP(hipGetLastError());
     if (r.ray_counts) {  // the only wait of a frame, and only when statistics are asked for
         unsigned long long t[2] = {0, 0};

@@ -98,7 +98,7 @@ class RenderParams(ctypes.Structure):
                 ("samples", ctypes.c_int32), ("max_frame", ctypes.c_int32), ("qmc", ctypes.c_int32),
                 ("max_depth", ctypes.c_int32), ("use_metalness", ctypes.c_int32), ("integrator", ctypes.c_int32),
                 ("brdf", ctypes.c_int32), ("use_rr", ctypes.c_int32), ("seed", ctypes.c_uint32),
-                ("ray_counts", ctypes.c_void_p)]
+                ("ray_counts", ctypes.c_void_p), ("concurrency", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _lib = None

@@ -115,7 +115,7 @@ class Scene:
 
     def render(self, width, height, inv_view, inv_proj, samples=8, max_depth=8, integrator=0, brdf=0,
                use_rr=True, clear_col=0.3, env_light=0.0, frame=0, seed=0, qmc=False, max_frame=256,
-               use_metalness=False, image=None, gbuffer=False, ray_counts=None):
+               use_metalness=False, image=None, gbuffer=False, ray_counts=None, concurrency=0):
         """One frame (gpurt_render).  Defaults are RTPipe's (src/vk/rt.h:36-51); env_light is
         env * env_scale.  With frame > 0, `image` (the previous frame) is blended in place.
         ray_counts: optional numpy uint64 array of 2 that accumulates the rays traced
@@ -140,6 +140,7 @@ class Scene:
         p.max_frame, p.qmc, p.max_depth = int(max_frame), int(bool(qmc)), int(max_depth)
         p.use_metalness, p.integrator, p.brdf = int(bool(use_metalness)), int(integrator), int(brdf)
         p.use_rr, p.seed = int(bool(use_rr)), int(seed) & 0xFFFFFFFF
+        p.concurrency = int(concurrency)
         if ray_counts is not None:
             if not (isinstance(ray_counts, np.ndarray) and ray_counts.dtype == np.uint64 and ray_counts.size >= 2
                     and ray_counts.flags.c_contiguous):

@@ -339,6 +339,10 @@ typedef struct gpurt_render_params {
     uint32_t seed;        /* per-frame RNG seed: pixel p starts from tea(p, seed) */
     uint64_t* ray_counts; /* host, nullable: [0] += camera / bounce rays traced (closest hit),
                              [1] += shadow (any hit) or light (closest hit) rays traced */
+    int32_t concurrency;  /* pixel groups rendered concurrently on their own streams, 1..8
+                             (0 = library default: 2 / 4 / 1 for integrators 0 / 1 / 2);
+                             results do not depend on it */
+    int32_t reserved;
 } gpurt_render_params;
 
 /* Render one frame of `samples` paths per pixel into image (height x width x 4 floats,
