@@ -126,20 +126,20 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     occ = torch.empty(n, dtype=torch.uint8, device=dev)
 
     def step():
-        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
-        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+        bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
 
     ms = timed(step, args.steps, args.warmup)
-    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger),
+    ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger),
                        args.steps, 1)
-    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger), args.steps, 1)
+    ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger), args.steps, 1)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
-    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
+    bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger, stats=st)
     torch.cuda.synchronize()
     nodes, tris, spills, _ = st.cpu().tolist()
     bytes_closest = n * (32 + 16) + nodes * info["node_bytes"] + tris * TRI_BYTES
     st.zero_()
-    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
+    bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger, stats=st)
     torch.cuda.synchronize()
     nodes_a, tris_a, spills_a, _ = st.cpu().tolist()
     hit_rate = float((prim >= 0).float().mean().item())
@@ -196,7 +196,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                    "passes": ["closest_hit", "any_hit"], "node_width": args.width,
                    "leaf_size": args.leaf, "stack_depth": args.stack, "treelet_size": args.treelet,
                    "node_format": info["node_format"], "waves_per_cu": args.waves,
-                   "leaf_trigger": args.leaf_trigger},
+                   "leaf_trigger": args.leaf_trigger, "refill_trigger": args.refill_trigger},
         "roofline": roof,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
                          "sample": f"{passes} full passes of the {n} closest-hit + {n} any-hit "
@@ -290,7 +290,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = gpurt.Batch(ctx, meshes, node_width=args.width, stack_depth=args.stack,
-                        waves_per_cu=args.waves, leaf_size=args.leaf, leaf_trigger=args.leaf_trigger,
+                        waves_per_cu=args.waves, leaf_size=args.leaf, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger,
                         treelet_size=args.batch_treelet, node_format=args.format)
     create_s = time.perf_counter() - t0
     ms_b = timed(batch.run, max(3, args.steps // 4), 1)
@@ -303,7 +303,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
                             treelet_size=args.batch_treelet, node_format=args.format)
             p, _, d, _ = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
-            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+            rp, rt, _ = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
             plain_out[i] = (p, d, rp, rt)
             bvh.close()
 
@@ -329,7 +329,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
                             treelet_size=args.batch_treelet, node_format=args.format)
             p, pt, d, b = bvh.closest_point(m["queries"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
-            rp, rt, ruv = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+            rp, rt, ruv = bvh.trace_closest(m["rays"], stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
             g = grads[i]
             gq = torch.empty_like(m["queries"])
             go = torch.empty((rp.shape[0], 3), device=dev)
@@ -382,7 +382,7 @@ def tlas_bench(args, torch, gpurt, W, O, dev, ctx):
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
            torch.empty(n, device=dev), torch.empty((n, 2), device=dev))
     occ = torch.empty(n, dtype=torch.uint8, device=dev)
-    q = dict(stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+    q = dict(stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
     ms_c = timed(lambda: tlas.trace_closest(rays, out=out, **q), args.steps, 1)
     ms_a = timed(lambda: tlas.trace_any(rays_any, out=occ, **q), args.steps, 1)
     st = torch.zeros(4, dtype=torch.int64, device=dev)
@@ -461,7 +461,7 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
            torch.empty((n, 2), device=dev))
     for _ in range(args.profile_reps):
-        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger)
+        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(args.profile_reps):
@@ -485,6 +485,9 @@ def main():
     ap.add_argument("--leaf-trigger", type=int, default=0,
                     help="lanes (of 64) with a postponed leaf that switch a wave to leaf tests "
                          "(0 = library default)")
+    ap.add_argument("--refill-trigger", type=int, default=0,
+                    help="idle lanes (of 64) at which a ray wave writes its finished rays and "
+                         "fetches new ones (0 = library default)")
     ap.add_argument("--treelet", type=int, default=7, choices=[0, 5, 6, 7],
                     help="SAH treelet restructuring of the single-mesh BVHs (leaves per "
                          "treelet), 0 = off; the batched leg rebuilds every step and uses "

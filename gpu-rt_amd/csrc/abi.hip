@@ -246,6 +246,8 @@ static int check_opts(const gpurt_query_opts* o) {
         return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
     if (o->leaf_trigger < 0 || o->leaf_trigger > 64)
         return set_error(GPURT_E_UNSUPPORTED, "leaf_trigger must be 0..64");
+    if (o->refill_trigger < 0 || o->refill_trigger > 64)
+        return set_error(GPURT_E_UNSUPPORTED, "refill_trigger must be 0..64");
     return GPURT_OK;
 }
 

@@ -373,6 +373,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
         return set_error(GPURT_E_UNSUPPORTED, "waves_per_cu must be 0..64");
     if (qopts && (qopts->leaf_trigger < 0 || qopts->leaf_trigger > 64))
         return set_error(GPURT_E_UNSUPPORTED, "leaf_trigger must be 0..64");
+    if (qopts && (qopts->refill_trigger < 0 || qopts->refill_trigger > 64))
+        return set_error(GPURT_E_UNSUPPORTED, "refill_trigger must be 0..64");
 
     std::vector<SegDesc> segs(n_meshes);
     std::vector<BatchMeshDev> md(n_meshes);
@@ -414,6 +416,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->stack = S;
     b->format = bp.format;
     if (qopts && qopts->leaf_trigger) b->leaf_trigger = (uint32_t)qopts->leaf_trigger;
+    // bits 8-15: the ray kernel's refill trigger (half a wave by default, as for single meshes)
+    b->leaf_trigger |= (qopts && qopts->refill_trigger ? (uint32_t)qopts->refill_trigger : 32u) << 8;
     b->host_meshes.assign(meshes, meshes + n_meshes);
     int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {

@@ -87,6 +87,9 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
     int sp_base = 0;
     const NodeT* bnodes = top_nodes;
     const float4* btris = top_tris;
+    // the trigger word: bits 0-7 the leaf trigger, bits 8-15 the refill trigger
+    const uint32_t leaf_trig = trig & 0xFFu;
+    const uint32_t refill_trig = max((trig >> 8) & 0xFFu, 1u);
     for (;;) {
         // leave an instance whose BLAS part of the stack is done (its parked leaf, if
         // any, must be tested first: it needs the object-space ray)
@@ -96,42 +99,49 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
             src.ray(h, r0, r1);
             r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
         }
-        if (has && !in_inst && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any))) {
-            float bu = 0.f, bv = 0.f;
-            uint32_t obj = 0xFFFFFFFFu;
-            if (!ANY && bid != 0xFFFFFFFFu) {
-                // (u,v) of the winner, recomputed once with the same arithmetic
-                const InstDev& I = inst[binst];
-                V3 oo, od;
-                to_object(I, r.o, r.d, oo, od);
-                const float4 A = I.tris[3 * (size_t)bpos];
-                const float4 B = I.tris[3 * (size_t)bpos + 1];
-                const float4 C = I.tris[3 * (size_t)bpos + 2];
-                float tt;
-                (void)ray_tri(oo, od, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
-                obj = I.custom;
+        // a finished lane keeps its slot until refill_trig lanes of the wave are idle (or
+        // all are); results are then written and new rays fetched for all of them at once
+        bool done = has && !in_inst && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        const uint32_t n_idle = (uint32_t)__popcll(__ballot(!has || done));
+        if (n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
+            if (done) {
+                float bu = 0.f, bv = 0.f;
+                uint32_t obj = 0xFFFFFFFFu;
+                if (!ANY && bid != 0xFFFFFFFFu) {
+                    // (u,v) of the winner, recomputed once with the same arithmetic
+                    const InstDev& I = inst[binst];
+                    V3 oo, od;
+                    to_object(I, r.o, r.d, oo, od);
+                    const float4 A = I.tris[3 * (size_t)bpos];
+                    const float4 B = I.tris[3 * (size_t)bpos + 1];
+                    const float4 C = I.tris[3 * (size_t)bpos + 2];
+                    float tt;
+                    (void)ray_tri(oo, od, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+                    obj = I.custom;
+                }
+                out(h, obj, bid, best, bu, bv, hit_any);
+                has = false;
+                done = false;
             }
-            out(h, obj, bid, best, bu, bv, hit_any);
-            has = false;
+            if (src.refill(has, h)) {
+                float4 r0, r1;
+                src.ray(h, r0, r1);
+                r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
+                tmin = r0.w;
+                best = r1.w;
+                bid = binst = 0xFFFFFFFFu;
+                hit_any = false;
+                st.sp = 0;
+                sp_base = 0;
+                in_inst = false;
+                cur = 0;
+                pleaf = kEmpty;
+                have = true;
+                has = true;
+            }
+            if (!__any(has)) break;
         }
-        if (src.refill(has, h)) {
-            float4 r0, r1;
-            src.ray(h, r0, r1);
-            r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
-            tmin = r0.w;
-            best = r1.w;
-            bid = binst = 0xFFFFFFFFu;
-            hit_any = false;
-            st.sp = 0;
-            sp_base = 0;
-            in_inst = false;
-            cur = 0;
-            pleaf = kEmpty;
-            have = true;
-            has = true;
-        }
-        if (!__any(has)) break;
-        if (!has) continue;
+        if (!has || done) continue;
         if (!have && st.sp > (in_inst ? sp_base : 0)) {
             const uint2 e = st.pop();
             if (__uint_as_float(e.y) <= best) {
@@ -162,7 +172,7 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
             have = false;
         }
         const bool node_ready = have && !child_is_leaf(cur);
-        if (leaf_step_now(pleaf != kEmpty, node_ready, trig)) {
+        if (leaf_step_now(pleaf != kEmpty, node_ready, leaf_trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {
@@ -543,20 +553,23 @@ void gpurt_tlas_destroy(gpurt_tlas t) {
 namespace gpurt_impl {
 
 // Defaults of the two-level kernels (opts fields left 0): every wave slot the register
-// budget allows (7 waves/SIMD) and a leaf trigger of 32 lanes, for closest and any hit
-// (measured on the 65-instance rays workload: any hit 3.38 -> 3.81 G rays/s, closest
-// 2.25 -> 2.37 against the single-level kernels' defaults).
+// budget allows (7 waves/SIMD), a leaf trigger of 32 lanes, and a refill when 16 lanes of
+// a closest-hit wave are idle, or at once for any hit, whose rays in the two-level scene
+// are mostly short (measured on the 65-instance rays workload: any hit 3.38 -> 3.81 G
+// rays/s, closest 2.25 -> 2.37 against the single-level kernels' earlier defaults, and
+// closest 2.00 -> 2.18 from the refill trigger).
 constexpr int kTlasDefaultTrigger = 32;
 
-static gpurt_query_opts tlas_opts(const gpurt_query_opts* opts, int W) {
+static gpurt_query_opts tlas_opts(const gpurt_query_opts* opts, int W, bool any) {
     gpurt_query_opts o = opts ? *opts : gpurt_query_opts{};
     if (o.waves_per_cu == 0) o.waves_per_cu = 4 * (W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
     if (o.leaf_trigger == 0) o.leaf_trigger = kTlasDefaultTrigger;
+    if (o.refill_trigger == 0) o.refill_trigger = any ? 1 : 16;
     return o;
 }
 
 static int tlas_plan(gpurt_tlas T, size_t n, bool any, const gpurt_query_opts* opts_in, TravPlan& p) {
-    const gpurt_query_opts o = tlas_opts(opts_in, T->width);
+    const gpurt_query_opts o = tlas_opts(opts_in, T->width, any);
     const gpurt_query_opts* opts = &o;
     const int W = T->width;
     // worst-case stack: the top levels' entries plus one BLAS's
@@ -582,7 +595,8 @@ int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t*
     if (n == 0) return GPURT_OK;
     if (!rays || (any ? !occ : (!obj || !prim || !t))) return set_error(GPURT_E_INVALID, "null ray/output pointer");
     if (opts && (opts->stack_depth < 0 || opts->waves_per_cu < 0 || opts->waves_per_cu > 64 ||
-                 opts->leaf_trigger < 0 || opts->leaf_trigger > 64))
+                 opts->leaf_trigger < 0 || opts->leaf_trigger > 64 || opts->refill_trigger < 0 ||
+                 opts->refill_trigger > 64))
         return set_error(GPURT_E_UNSUPPORTED, "query options out of range");
     TlasDevGuard dg(T->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");

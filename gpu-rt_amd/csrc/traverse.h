@@ -412,44 +412,56 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
     float tmin = 0.f, best = 0.f;
     uint32_t bid = 0xFFFFFFFFu, bpos = 0, cur = 0, pleaf = kEmpty;
     bool hit_any = false, have = false;
+    // the trigger word: bits 0-7 the leaf trigger, bits 8-15 the refill trigger
+    const uint32_t leaf_trig = trig & 0xFFu;
+    const uint32_t refill_trig = max((trig >> 8) & 0xFFu, 1u);
     for (;;) {
-        if (has && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any))) {
-            // barycentrics of the winner, recomputed once with the same arithmetic
-            float bu = 0.f, bv = 0.f;
-            if (!ANY && bid != 0xFFFFFFFFu) {
-                const float4 A = tris[3 * (size_t)bpos];
-                const float4 B = tris[3 * (size_t)bpos + 1];
-                const float4 C = tris[3 * (size_t)bpos + 2];
-                float tt;
-                (void)ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+        // A lane whose ray is finished keeps its slot until refill_trig lanes of the wave are
+        // idle (or all are): the results are then written and new rays fetched for all of
+        // them at once, so the wave pays for the refill code once per batch rather than
+        // every time some lane finishes.
+        bool done = has && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        const uint32_t n_idle = (uint32_t)__popcll(__ballot(!has || done));
+        if (n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
+            if (done) {
+                // barycentrics of the winner, recomputed once with the same arithmetic
+                float bu = 0.f, bv = 0.f;
+                if (!ANY && bid != 0xFFFFFFFFu) {
+                    const float4 A = tris[3 * (size_t)bpos];
+                    const float4 B = tris[3 * (size_t)bpos + 1];
+                    const float4 C = tris[3 * (size_t)bpos + 2];
+                    float tt;
+                    (void)ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+                }
+                out(h, bid, best, bu, bv, hit_any);
+                has = false;
+                done = false;
             }
-            out(h, bid, best, bu, bv, hit_any);
-            has = false;
+            if (src.refill(has, h)) {
+                float4 r0, r1;
+                src.ray(h, r0, r1);
+                o = v3(r0.x, r0.y, r0.z);
+                d = v3(r1.x, r1.y, r1.z);
+                tmin = r0.w;
+                best = r1.w;
+                const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+                const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+                const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+                inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+                oix = -o.x * inv.x;
+                oiy = -o.y * inv.y;
+                oiz = -o.z * inv.z;
+                bid = 0xFFFFFFFFu;
+                hit_any = false;
+                st.sp = 0;
+                cur = src.root(h);
+                pleaf = kEmpty;
+                have = true;
+                has = true;
+            }
+            if (!__any(has)) break;
         }
-        if (src.refill(has, h)) {
-            float4 r0, r1;
-            src.ray(h, r0, r1);
-            o = v3(r0.x, r0.y, r0.z);
-            d = v3(r1.x, r1.y, r1.z);
-            tmin = r0.w;
-            best = r1.w;
-            const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
-            const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
-            const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
-            inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
-            oix = -o.x * inv.x;
-            oiy = -o.y * inv.y;
-            oiz = -o.z * inv.z;
-            bid = 0xFFFFFFFFu;
-            hit_any = false;
-            st.sp = 0;
-            cur = src.root(h);
-            pleaf = kEmpty;
-            have = true;
-            has = true;
-        }
-        if (!__any(has)) break;
-        if (!has) continue;
+        if (!has || done) continue;
         if (!have && st.sp > 0) {
             const uint2 e = st.pop();
             if (__uint_as_float(e.y) <= best) {
@@ -462,7 +474,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
             have = false;
         }
         const bool node_ready = have && !child_is_leaf(cur);
-        if (leaf_step_now(pleaf != kEmpty, node_ready, trig)) {
+        if (leaf_step_now(pleaf != kEmpty, node_ready, leaf_trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {
@@ -566,7 +578,7 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
             have = false;
         }
         const bool node_ready = have && !child_is_leaf(cur);
-        if (leaf_step_now(pleaf != kEmpty, node_ready, trig)) {
+        if (leaf_step_now(pleaf != kEmpty, node_ready, trig & 0xFFu)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
             for (uint32_t k = 0; k < cnt; k++) {

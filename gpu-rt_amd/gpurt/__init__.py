@@ -145,7 +145,7 @@ class Bvh:
         _lib.check(_lib.load().gpurt_bvh_refit(self._h, _ptr(verts), verts.shape[1] * 4,
                                                _stream(self.device)))
 
-    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger=0):
+    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger=0, refill_trigger=0):
         """stats: optional int64 device tensor of 4 counters (wide nodes visited,
         triangles tested, stack spills, reserved), accumulated by the counting kernel
         variant."""
@@ -156,7 +156,7 @@ class Bvh:
                                  "on the BVH's device")
         return QueryOpts(stack_depth, waves_per_cu,
                          ctypes.c_void_p(stats.data_ptr()) if stats is not None else None,
-                         leaf_trigger, 0)
+                         leaf_trigger, refill_trigger)
 
     def closest_point(self, q, with_bary=True, stack_depth=0, waves_per_cu=0, stats=None,
                       out=None, leaf_trigger=0):
@@ -177,7 +177,7 @@ class Bvh:
         return prim, pt, dist, bary
 
     def trace_closest(self, rays, with_uv=True, stack_depth=0, waves_per_cu=0, stats=None,
-                      out=None, leaf_trigger=0):
+                      out=None, leaf_trigger=0, refill_trigger=0):
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
@@ -187,17 +187,18 @@ class Bvh:
             uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
         else:
             prim, t, uv = out
-        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_trace_closest(self._h, _ptr(rays), n, _ptr(prim), _ptr(t),
                                                    _ptr(uv), ctypes.byref(opts), _stream(dev)))
         return prim, t, uv
 
-    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0):
+    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0,
+                  refill_trigger=0):
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
         occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
-        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_trace_any(self._h, _ptr(rays), n, _ptr(occ),
                                                ctypes.byref(opts), _stream(dev)))
         return occ
@@ -235,7 +236,8 @@ class Batch:
     """
 
     def __init__(self, ctx, meshes, node_width=4, leaf_size=4, treelet_size=0, stack_depth=0,
-                 waves_per_cu=0, with_bary=True, with_uv=True, node_format=0, leaf_trigger=0):
+                 waves_per_cu=0, with_bary=True, with_uv=True, node_format=0, leaf_trigger=0,
+                 refill_trigger=0):
         lib = _lib.load()
         self.ctx = ctx
         dev = ctx.device
@@ -278,7 +280,7 @@ class Batch:
         self._arr = arr
         self._meshes = meshes
         bo = BuildOpts(node_width, leaf_size, treelet_size, 0, node_format)
-        qo = QueryOpts(stack_depth, waves_per_cu, None, leaf_trigger, 0)
+        qo = QueryOpts(stack_depth, waves_per_cu, None, leaf_trigger, refill_trigger)
         torch.cuda.synchronize(dev)  # inputs must be resident before the first (eager) run
         h = ctypes.c_void_p()
         _lib.check(lib.gpurt_batch_create(ctx.handle, arr, len(meshes), ctypes.byref(bo),
@@ -376,15 +378,15 @@ class Tlas:
     def device(self):
         return self.ctx.device
 
-    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger):
+    def _qopts(self, stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger=0):
         if stats is not None and (stats.dtype != torch.int64 or stats.numel() < 4):
             raise ValueError("stats must be an int64 tensor of >= 4 elements")
         return QueryOpts(stack_depth, waves_per_cu,
                          ctypes.c_void_p(stats.data_ptr()) if stats is not None else None,
-                         leaf_trigger, 0)
+                         leaf_trigger, refill_trigger)
 
     def trace_closest(self, rays, with_uv=True, stack_depth=0, waves_per_cu=0, stats=None,
-                      out=None, leaf_trigger=0):
+                      out=None, leaf_trigger=0, refill_trigger=0):
         """Returns (obj, prim, t, uv): obj = custom index of the hit instance (-1: miss)."""
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
@@ -396,18 +398,19 @@ class Tlas:
             uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
         else:
             obj, prim, t, uv = out
-        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_tlas_trace_closest(self._h, _ptr(rays), n, _ptr(obj), _ptr(prim),
                                                         _ptr(t), _ptr(uv), ctypes.byref(opts),
                                                         _stream(dev)))
         return obj, prim, t, uv
 
-    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0):
+    def trace_any(self, rays, stack_depth=0, waves_per_cu=0, stats=None, out=None, leaf_trigger=0,
+                  refill_trigger=0):
         dev = self.device
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
         occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
-        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
+        opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_tlas_trace_any(self._h, _ptr(rays), n, _ptr(occ),
                                                     ctypes.byref(opts), _stream(dev)))
         return occ

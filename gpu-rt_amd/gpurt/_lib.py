@@ -51,7 +51,7 @@ class BuildOpts(ctypes.Structure):
 class QueryOpts(ctypes.Structure):
     _fields_ = [("stack_depth", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
                 ("stats", ctypes.c_void_p), ("leaf_trigger", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("refill_trigger", ctypes.c_int32)]
 
 
 class BvhInfo(ctypes.Structure):

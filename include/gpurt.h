@@ -91,7 +91,9 @@ typedef struct gpurt_query_opts {
     int32_t leaf_trigger; /* lanes (of a 64-lane wave) holding a postponed leaf at which
                              the wave switches to leaf tests, 1..64 (0 = library
                              default); changes speed only                             */
-    int32_t reserved;
+    int32_t refill_trigger; /* idle lanes (of 64) at which a wave of the ray kernels
+                             writes its finished results and fetches new rays, 1..64
+                             (0 = library default); changes speed only              */
 } gpurt_query_opts;
 
 typedef struct gpurt_bvh_info {
