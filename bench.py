@@ -68,8 +68,10 @@ def measured_traffic(args, n_tris, n_rays, node_format):
     if any(k not in m for k in keys):
         return None
     if (m["n_tris"], m["n_rays"], m["node_width"], m["stack_depth"], m["treelet_size"],
-            m["node_format"], m.get("leaf_size", 4), m.get("waves_per_cu", 0)) != \
-            (n_tris, n_rays, args.width, args.stack, args.treelet, node_format, args.leaf, args.waves):
+            m["node_format"], m.get("leaf_size", 4), m.get("waves_per_cu", 0), m.get("leaf_trigger", 0),
+            m.get("refill_trigger", 0)) != \
+            (n_tris, n_rays, args.width, args.stack, args.treelet, node_format, args.leaf, args.waves,
+             args.leaf_trigger, args.refill_trigger):
         return None
     rd = m["fetch_kb_raw"] * m["calib_kib"] / m["calib_fetch_kb_raw"]
     wr = m["write_kb_raw"] * m["calib_kib"] / m["calib_write_kb_raw"]

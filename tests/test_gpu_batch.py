@@ -39,13 +39,14 @@ def _dev(meshes, dev):
     return [{k: torch.from_numpy(v).to(dev) for k, v in m.items()} for m in meshes]
 
 
-@pytest.mark.parametrize("fmt", [1, 2])
-@pytest.mark.parametrize("width", [2, 4, 8])
-def test_batch_matches_oracle_and_plain(ctx, dev, width, fmt):
+@pytest.mark.parametrize("fmt,width,sched", [(1, 2, {}), (1, 4, {}), (1, 8, {}), (2, 2, {}), (2, 4, {}),
+                                             (2, 8, {}), (2, 4, dict(leaf_trigger=1, refill_trigger=64)),
+                                             (1, 4, dict(leaf_trigger=64, refill_trigger=1))])
+def test_batch_matches_oracle_and_plain(ctx, dev, width, fmt, sched):
     host = _small_meshes()
     dm = _dev(host, dev)
-    print("batch width", width, "format", fmt, "meshes", len(host), flush=True)
-    batch = gpurt.Batch(ctx, dm, node_width=width, node_format=fmt)
+    print("batch width", width, "format", fmt, "meshes", len(host), sched, flush=True)
+    batch = gpurt.Batch(ctx, dm, node_width=width, node_format=fmt, **sched)
     res = batch.run()
     torch.cuda.synchronize()
     for i, (h, r) in enumerate(zip(host, res)):

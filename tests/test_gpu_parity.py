@@ -73,7 +73,7 @@ def test_cpq_leaf_sizes(ctx, dev, leaf, fmt):
 def _ray_check(ctx, dev, V, F, R, RA=None, **kw):
     bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **{k: v for k, v in kw.items() if k in
                                                       ("node_width", "leaf_size", "node_format")})
-    qkw = {k: v for k, v in kw.items() if k in ("stack_depth",)}
+    qkw = {k: v for k, v in kw.items() if k in ("stack_depth", "waves_per_cu", "leaf_trigger", "refill_trigger")}
     prim, t, uv = _np(*bvh.trace_closest(_t(R, dev), **qkw))
     ep, et, euv = O.ray_closest_brute_f32(V, F, R)
     assert np.array_equal(prim, ep), f"{(prim != ep).sum()} ids differ"
@@ -93,6 +93,17 @@ def test_rays_small_scene_bitexact(ctx, dev, width, fmt):
     prim = _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], node_width=width,
                       node_format=fmt)
     assert 0.2 < (prim >= 0).mean() < 0.9
+
+
+@pytest.mark.parametrize("refill", [1, 13, 64])
+@pytest.mark.parametrize("leaf", [1, 64])
+def test_wave_scheduling_options_keep_answers(ctx, dev, leaf, refill):
+    """leaf_trigger (parked leaves before a leaf step) and refill_trigger (idle lanes before
+    finished rays are written and new ones fetched) reorder work inside a wave only."""
+    w = W.rays_scene(n_rays=8192, grid=101, n_spheres=6)
+    print("rays leaf_trigger", leaf, "refill_trigger", refill, flush=True)
+    _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], leaf_trigger=leaf,
+               refill_trigger=refill, waves_per_cu=4)
 
 
 @pytest.mark.parametrize("stride", [3, 4, 12])
@@ -223,6 +234,10 @@ def test_bad_options_are_rejected(ctx, dev):
     bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev))
     with pytest.raises(gpurt.GpurtError):
         bvh.closest_point(_t(w["queries"], dev), stack_depth=24)
+    rays = torch.zeros((4, 8), device=dev)
+    for bad in (dict(refill_trigger=65), dict(refill_trigger=-1), dict(leaf_trigger=65)):
+        with pytest.raises(gpurt.GpurtError):
+            bvh.trace_closest(rays, **bad)
 
 
 def test_empty_query_sets(ctx, dev):

@@ -21,20 +21,21 @@ def _np(*ts):
     return [t.cpu().numpy() if t is not None else None for t in ts]
 
 
-def _check(ctx, dev, sc, rays, rays_any, custom=None, mask=None, **bkw):
+def _check(ctx, dev, sc, rays, rays_any, custom=None, mask=None, qkw=None, **bkw):
     blas = [gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), **bkw) for V, F in sc["meshes"]]
     n = sc["xf"].shape[0]
     custom = list(range(100, 100 + n)) if custom is None else custom
     mask = [0xFF] * n if mask is None else mask
     inst = [(blas[m], sc["xf"][i].reshape(3, 4), custom[i], mask[i]) for i, m in enumerate(sc["mesh_of"])]
     tlas = gpurt.Tlas(ctx, inst)
-    obj, prim, t, uv = _np(*tlas.trace_closest(_t(rays, dev)))
+    qkw = qkw or {}
+    obj, prim, t, uv = _np(*tlas.trace_closest(_t(rays, dev), **qkw))
     eo, ep, et, euv = O.inst_ray_brute_f32(sc["xf"], sc["mesh_of"], sc["meshes"], rays, custom=custom, mask=mask)
     assert np.array_equal(prim, ep), f"{(prim != ep).sum()} prim ids differ"
     assert np.array_equal(obj, eo), f"{(obj != eo).sum()} object ids differ"
     assert np.array_equal(t, et)
     assert np.array_equal(uv, euv)
-    occ, = _np(tlas.trace_any(_t(rays_any, dev)))
+    occ, = _np(tlas.trace_any(_t(rays_any, dev), **qkw))
     eocc = O.inst_ray_brute_f32(sc["xf"], sc["mesh_of"], sc["meshes"], rays_any, custom=custom, mask=mask,
                                 any_hit=True)
     assert np.array_equal(occ, eocc)
@@ -48,6 +49,15 @@ def test_tlas_matches_instanced_brute_force(ctx, dev, width, fmt):
     print("tlas width", width, "format", fmt, sc["meta"], flush=True)
     obj, _ = _check(ctx, dev, sc, sc["rays"], sc["rays_any"], node_width=width, node_format=fmt)
     assert len(set(obj.tolist())) > 5  # rays hit many different instances
+
+
+@pytest.mark.parametrize("refill", [1, 13, 64])
+@pytest.mark.parametrize("leaf", [1, 64])
+def test_tlas_wave_scheduling_options_keep_answers(ctx, dev, leaf, refill):
+    sc = W.instanced_scene(n_rays=6000, grid=41, n_spheres=12, sphere_level=2, seed=5)
+    print("tlas leaf_trigger", leaf, "refill_trigger", refill, flush=True)
+    _check(ctx, dev, sc, sc["rays"], sc["rays_any"],
+           qkw=dict(leaf_trigger=leaf, refill_trigger=refill, waves_per_cu=4))
 
 
 def test_tlas_masks_and_custom_indices(ctx, dev):
