@@ -449,28 +449,40 @@ def tracer_bench(args, torch, gpurt, W, O, dev, ctx):
 
 
 def profile_mode(args, torch, gpurt, W, dev, ctx):
-    """Short run for hardware-counter passes: `--profile-reps` closest-hit launches on
-    rays_scene, then the same number of elementwise adds over 256 MiB (a plain streaming
-    kernel that reads 256 MiB and writes 256 MiB, whose FETCH_SIZE/WRITE_SIZE calibrate
-    the counters)."""
-    w = W.rays_scene()
+    """Short run for hardware-counter passes: `--profile-reps` launches of the kernel
+    `--profile-workload` names (closest-hit rays on rays_scene, or closest-point queries
+    on cpq_small / cpq_large), then the same number of elementwise adds over 256 MiB (a
+    plain streaming kernel that reads 256 MiB and writes 256 MiB, whose
+    FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
+    kind = args.profile_workload
+    w = W.rays_scene() if kind == "rays_scene" else getattr(W, kind)()
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
-    rays = torch.from_numpy(w["rays"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
                     node_format=args.format, leaf_size=args.leaf)
-    n = rays.shape[0]
-    out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
-           torch.empty((n, 2), device=dev))
-    for _ in range(args.profile_reps):
-        bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+    if kind == "rays_scene":
+        rays = torch.from_numpy(w["rays"]).to(dev)
+        n = rays.shape[0]
+        out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
+               torch.empty((n, 2), device=dev))
+        for _ in range(args.profile_reps):
+            bvh.trace_closest(rays, out=out, stack_depth=args.stack, waves_per_cu=args.waves,
+                              leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+    else:
+        q = torch.from_numpy(w["queries"]).to(dev)
+        n = q.shape[0]
+        out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty((n, 3), device=dev),
+               torch.empty(n, device=dev), torch.empty((n, 3), device=dev))
+        for _ in range(args.profile_reps):
+            bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves,
+                              leaf_trigger=args.leaf_trigger)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(args.profile_reps):
         torch.add(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    print(json.dumps({"profile": "rays_scene closest-hit", "launches": args.profile_reps,
-                      "rays": n, "calib_bytes_read": src.numel() * 4,
+    print(json.dumps({"profile": kind, "launches": args.profile_reps,
+                      "units": n, "calib_bytes_read": src.numel() * 4,
                       "calib_bytes_written": dst.numel() * 4}), flush=True)
 
 
@@ -509,6 +521,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--profile-workload", default="rays_scene", choices=["rays_scene", "cpq_small", "cpq_large"],
+                    help="kernel of the --profile run: closest-hit rays or closest-point queries")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box "

@@ -94,6 +94,11 @@ struct ChunkSource {
         }
         return got;
     }
+    // wave-uniform: no lane of this wave will get another query
+    __device__ __forceinline__ bool exhausted() const { return !more; }
+    __device__ __forceinline__ static Handle shfl(const Handle& h, uint32_t from) {
+        return Handle{(uint32_t)__shfl(h.m, (int)from), (uint32_t)__shfl(h.i, (int)from)};
+    }
     __device__ __forceinline__ uint32_t root(const Handle& h) const { return h.m; }
     __device__ __forceinline__ void ray(const Handle& h, float4& r0, float4& r1) const {
         const float4* r = meshes[h.m].rays;
@@ -158,9 +163,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
 template <int W, bool Q, int S, int BS>
 static void launch_batch_kind(bool rays, unsigned grid, hipStream_t s, const BatchArgs& a) {
     if (rays)
-        hipLaunchKernelGGL((k_batch_ray<W, Q, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+        hipLaunchKernelGGL((k_batch_ray<W, Q, S, BS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kGroupKeyBytes) * BS, s, a);
     else
-        hipLaunchKernelGGL((k_batch_cpq<W, Q, S, BS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+        hipLaunchKernelGGL((k_batch_cpq<W, Q, S, BS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kGroupKeyBytes) * BS, s, a);
 }
 
 template <int W, bool Q>
@@ -416,8 +421,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     b->stack = S;
     b->format = bp.format;
     if (qopts && qopts->leaf_trigger) b->leaf_trigger = (uint32_t)qopts->leaf_trigger;
-    // bits 8-15: the ray kernel's refill trigger (half a wave by default, as for single meshes)
-    b->leaf_trigger |= (qopts && qopts->refill_trigger ? (uint32_t)qopts->refill_trigger : 32u) << 8;
+    // bits 8-15: the ray kernel's refill trigger (16 lanes by default, as for single meshes)
+    b->leaf_trigger |= (qopts && qopts->refill_trigger ? (uint32_t)qopts->refill_trigger : 16u) << 8;
     b->host_meshes.assign(meshes, meshes + n_meshes);
     int rc = build_alloc(segs.data(), (int)n_meshes, W, leaf, tsize, b->build);
     if (rc) {
@@ -444,7 +449,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     const bool any_f32 = !rays_use_q8(bp.format) || !cpq_use_q8(bp.format);
     const int hint = W == 8 ? (any_f32 ? WavesHint<8, false>::value : WavesHint<8, true>::value)
                             : WavesHint<4, false>::value;
-    int blocks_per_cu = std::min(ctx->lds_per_cu / (S * BS * 8), std::min(32, 4 * hint) / waves_per_block);
+    int blocks_per_cu = std::min(ctx->lds_per_cu / ((S * 8 + kGroupKeyBytes) * BS), std::min(32, 4 * hint) / waves_per_block);
     if (qopts && qopts->waves_per_cu > 0) blocks_per_cu = std::max(1, qopts->waves_per_cu / waves_per_block);
     blocks_per_cu = std::max(1, blocks_per_cu);
     b->grid = (unsigned)std::max<size_t>(1, (size_t)ctx->n_cu * blocks_per_cu);

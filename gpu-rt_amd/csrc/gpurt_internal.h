@@ -235,9 +235,13 @@ struct TravScratch {
     uint32_t* counters = nullptr;
     uint2* spill = nullptr;
 };
-// waves_per_simd: what the kernel is compiled for (0: the single-level kernels' WavesHint)
+// LDS per lane beyond the stack slots of the single-level kernels: the 64-bit result key
+// and the member count of work donation (traverse.h)
+constexpr int kGroupKeyBytes = 12;
+// waves_per_simd: what the kernel is compiled for (0: the single-level kernels' WavesHint);
+// lds_extra: LDS bytes per lane beyond the stack slots
 int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
-                   size_t n, TravPlan& p, int waves_per_simd = 0);
+                   size_t n, TravPlan& p, int waves_per_simd = 0, int lds_extra = kGroupKeyBytes);
 int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t);
 void free_traversal_scratch(TravScratch& t, hipStream_t s);
 // the same scratch in memory the caller holds

@@ -96,13 +96,13 @@ template <int W, bool Q, int S, int BS, bool STATS>
 static void launch_kind(QKind kind, const TravArgs& a, unsigned grid, hipStream_t s) {
     switch (kind) {
         case QKind::Cpq:
-            hipLaunchKernelGGL((k_cpq<W, Q, S, BS, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_cpq<W, Q, S, BS, STATS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kGroupKeyBytes) * BS, s, a);
             break;
         case QKind::Closest:
-            hipLaunchKernelGGL((k_ray<W, Q, S, BS, false, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_ray<W, Q, S, BS, false, STATS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kGroupKeyBytes) * BS, s, a);
             break;
         case QKind::Any:
-            hipLaunchKernelGGL((k_ray<W, Q, S, BS, true, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+            hipLaunchKernelGGL((k_ray<W, Q, S, BS, true, STATS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kGroupKeyBytes) * BS, s, a);
             break;
     }
 }
@@ -122,21 +122,20 @@ static int dispatch_stack(int S, QKind kind, const TravArgs& a, unsigned grid, h
 // Defaults per query kind, from sweeps on the rays_scene and closest-point workloads:
 // the parked-leaf share that switches a wave to leaf steps, the idle-lane share at which
 // a ray wave writes its finished rays and fetches new ones (a refill costs the whole wave
-// one pass through the ray setup, so it pays to gather several finished lanes; closest-hit
-// rays run long and wait for half the wave, any-hit rays end early and refill sooner),
-// and resident waves per CU (0 = as many as registers and LDS allow; 7 per SIMD leaves
-// the L2 less crowded).  rays_scene, 1M + 1M rays: closest 3.29 -> 3.57, any hit
-// 4.51 -> 5.00 G rays/s against a refill on every finished lane.
-static uint32_t default_leaf_trigger(QKind kind) {
-    return kind == QKind::Any ? 32u : kind == QKind::Closest ? 28u : 32u;
-}
-static uint32_t default_refill_trigger(QKind kind) { return kind == QKind::Any ? 8u : 32u; }
-static int default_waves_per_cu(QKind kind) { return kind == QKind::Cpq ? 0 : 28; }
+// one pass through the ray setup, so it pays to gather several finished lanes), and
+// resident waves per CU (fewer waves leave the L2 less crowded; with work donation the
+// end of a launch no longer needs extra waves to hide its idle lanes).  rays_scene, 1M +
+// 1M rays: closest 3.29 -> 4.05, any hit 4.51 -> 5.70 G rays/s against a refill on every
+// finished lane at 8 waves/SIMD; closest-point queries near a 1.3M-triangle surface: 0.88
+// -> 1.06 G queries/s at 4 waves/SIMD.
+static uint32_t default_leaf_trigger(QKind) { return 32u; }
+static uint32_t default_refill_trigger(QKind) { return 16u; }
+static int default_waves_per_cu(QKind kind) { return kind == QKind::Cpq ? 16 : 24; }
 
 static int block_for_stack(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 
 int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
-                   size_t n, TravPlan& p, int waves_per_simd) {
+                   size_t n, TravPlan& p, int waves_per_simd, int lds_extra) {
     if (n > 0xFFFFFFFFull - 1024u) return set_error(GPURT_E_INVALID, "too many queries in one call");
     // default: the smallest LDS stack (most resident waves); 8-wide nodes push more
     // entries per visit, so they default to 16
@@ -150,7 +149,7 @@ int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need
     const int hint = waves_per_simd > 0 ? waves_per_simd
                      : width == 8 ? (q8 ? WavesHint<8, true>::value : WavesHint<8, false>::value)
                                   : (q8 ? WavesHint<4, true>::value : WavesHint<4, false>::value);
-    int blocks_per_cu = std::min(ctx->lds_per_cu / (p.S * p.BS * 8), std::min(32, 4 * hint) / waves_per_block);
+    int blocks_per_cu = std::min(ctx->lds_per_cu / ((p.S * 8 + lds_extra) * p.BS), std::min(32, 4 * hint) / waves_per_block);
     if (opts && opts->waves_per_cu > 0)
         blocks_per_cu = std::max(1, opts->waves_per_cu / waves_per_block);
     else if (default_waves_per_cu(kind) > 0)

@@ -575,7 +575,7 @@ static int tlas_plan(gpurt_tlas T, size_t n, bool any, const gpurt_query_opts* o
     // worst-case stack: the top levels' entries plus one BLAS's
     const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
     return plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
-                          W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
+                          W == 8 ? kTlasWaves<8> : kTlasWaves<4>, 0);
 }
 
 size_t tlas_scratch_bytes(gpurt_tlas T, size_t n) {

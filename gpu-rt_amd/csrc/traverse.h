@@ -292,6 +292,8 @@ constexpr float kTSlack = 0x1p-20f;             // relative slack on ray box exi
 //   typename Handle;                         which query (index, or mesh and index)
 //   bool refill(bool has, Handle& h);        called by all lanes together; gives lanes
 //                                            with has == false their next query
+//   bool exhausted();                        (wave-uniform) no further query for the wave
+//   static Handle shfl(Handle, uint32_t l);  lane l's handle (a wave shuffle)
 //   uint32_t root(const Handle&);            root node of the query's tree
 //   ray(const Handle&, float4&, float4&)     (rays) origin, tmin / direction, tmax
 //   V3 point(const Handle&)                  (closest point) the query point
@@ -306,28 +308,6 @@ __device__ __forceinline__ bool leaf_step_now(bool parked, bool node_ready, uint
     if (__ballot(node_ready) == 0) return true;
     return (uint32_t)__popcll(pk) * 64u >= (uint32_t)__popcll(__ballot(true)) * trig;
 }
-
-// Plain source: lane g takes queries g, g + stride, g + 2 stride, ... of one array.
-struct StrideSource {
-    using Handle = uint32_t;
-    const float4* rays;
-    const float* pts;
-    uint32_t n, next, stride;
-    __device__ __forceinline__ bool refill(bool has, Handle& h) {
-        if (has || next >= n) return false;
-        h = next;
-        next += stride;
-        return true;
-    }
-    __device__ __forceinline__ uint32_t root(Handle) const { return 0u; }
-    __device__ __forceinline__ void ray(Handle h, float4& r0, float4& r1) const {
-        r0 = rays[2 * (size_t)h];
-        r1 = rays[2 * (size_t)h + 1];
-    }
-    __device__ __forceinline__ V3 point(Handle h) const {
-        return v3(pts[3 * (size_t)h], pts[3 * (size_t)h + 1], pts[3 * (size_t)h + 2]);
-    }
-};
 
 // Plain source with load balancing: lane g takes query g first; the queries after the
 // first `total` are cut into 64-query chunks dealt round-robin to nq = min(kQueues,
@@ -389,6 +369,9 @@ struct QueueSource {
         }
         return got;
     }
+    // wave-uniform: no lane of this wave will get another query
+    __device__ __forceinline__ bool exhausted() const { return started && !more; }
+    __device__ __forceinline__ static Handle shfl(Handle h, uint32_t from) { return __shfl(h, (int)from); }
     __device__ __forceinline__ uint32_t root(Handle) const { return 0u; }
     __device__ __forceinline__ void ray(Handle h, float4& r0, float4& r1) const {
         r0 = rays[2 * (size_t)h];
@@ -398,6 +381,85 @@ struct QueueSource {
         return v3(pts[3 * (size_t)h], pts[3 * (size_t)h + 1], pts[3 * (size_t)h + 2]);
     }
 };
+
+// ------------------------------------------------------------------------------------
+// Work donation at the end of a launch
+//
+// Once a wave's source has no more queries, its lanes without a query would idle while
+// the last queries finish one node at a time (measured: a launch of 262,144 rays takes
+// 82% as long as one of 1,048,576).  Instead an idle lane takes the top stack entry of
+// a lane that still traverses and keeps other work: both then work on the same query,
+// each on its own subtrees, the helper starting from the best value the donor had.
+// The lanes working on one query form its group -- the query's own lane (the owner) and
+// its helpers; helpers donate further in the same way.  A member merges each better
+// result into the owner's 64-bit key in LDS with an atomic min and adopts the key when
+// it is better than its own (so every member culls with the group's best).  The key orders
+// (value, primitive id) lexicographically (result_key), so the merge keeps the
+// lowest-id tie rule.  The owner's LDS count holds the members still traversing (one at
+// the start, one more per helper, one less per member that finishes); when it reaches
+// zero, the member that found the winning key writes the result (the owner when nothing
+// was found) and all members go idle.  Every subtree stays with exactly one lane, and
+// every lane culls only against a value that some primitive (or the query's own limit)
+// reached, so the answer is the one a single lane finds.
+// ------------------------------------------------------------------------------------
+
+// position of the n-th (from 0) set bit of m; n < popcount(m)
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (n >= c) {
+            n -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t lane = __lane_id();
+    return lane ? (~0ull >> (64u - lane)) : 0ull;
+}
+
+// this wave's 64 group keys and member counts: after the block's S * BS stack slots,
+// BS keys (8 bytes) then BS counts (4 bytes)
+template <int S, int BS>
+__device__ __forceinline__ unsigned long long* group_keys() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+    return reinterpret_cast<unsigned long long*>(lds_stack + S * BS) + w;
+}
+template <int S, int BS>
+__device__ __forceinline__ uint32_t* group_counts() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned long long*>(lds_stack + S * BS) + BS) + w;
+}
+
+// Pairs idle lanes with donors (called by all lanes together): the r-th idle lane, in
+// lane order, takes from the r-th donor.  Returns the donor lane for a recipient (take),
+// else this lane; give marks a donor that has been paired.
+__device__ __forceinline__ uint32_t pair_donors(bool idle, bool donor, bool& take, bool& give) {
+    const uint64_t I = __ballot(idle), D = __ballot(donor);
+    const uint32_t k = min((uint32_t)__popcll(I), (uint32_t)__popcll(D));
+    const uint64_t below = lanes_below();
+    const uint32_t ri = (uint32_t)__popcll(I & below), rd = (uint32_t)__popcll(D & below);
+    give = donor && rd < k;
+    take = idle && ri < k;
+    return take ? nth_set_bit(D, ri) : __lane_id();
+}
+
+// (value, id) as one integer that orders like the pair: the float's bits mapped to an
+// order-preserving integer (negative values have their bits inverted, the rest get the
+// sign bit set), with -0 first turned into +0 since the traversal treats them as equal
+__device__ __forceinline__ unsigned long long result_key(float value, uint32_t id) {
+    const uint32_t b = __float_as_uint(value + 0.0f);
+    const uint32_t ord = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return (unsigned long long)ord << 32 | id;
+}
+__device__ __forceinline__ float key_value(unsigned long long key) {
+    const uint32_t ord = (uint32_t)(key >> 32);
+    return __uint_as_float((ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord);
+}
 
 template <typename NodeT, int S, int BS, bool ANY, bool STATS, typename Src, typename Out>
 __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
@@ -412,17 +474,73 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
     float tmin = 0.f, best = 0.f;
     uint32_t bid = 0xFFFFFFFFu, bpos = 0, cur = 0, pleaf = kEmpty;
     bool hit_any = false, have = false;
+    // work donation: the lane whose ray this lane works on, whether the ray is shared,
+    // whether this lane's part is finished and waits for the rest of the group, and
+    // (wave-uniform) whether the source has run dry
+    const uint32_t lane = __lane_id();
+    uint32_t owner = lane;
+    bool grouped = false, wait = false, drained = false;
+    bool mine = true;  // (closest hit) bid was found by this lane, so bpos locates it
+    unsigned long long* const gkey = group_keys<S, BS>();
+    uint32_t* const gcnt = group_counts<S, BS>();
     // the trigger word: bits 0-7 the leaf trigger, bits 8-15 the refill trigger
     const uint32_t leaf_trig = trig & 0xFFu;
     const uint32_t refill_trig = max((trig >> 8) & 0xFFu, 1u);
     for (;;) {
+        bool done;
+        if (!drained) {
+            done = has && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        } else {
+            // groups exist only once the source has run dry (wave-uniform branch)
+            if (grouped && has && !wait) {
+                // the best hit of the whole group so far, every iteration: any hit ends the
+                // search; a closer closest hit tightens this lane's culls
+                const unsigned long long g = gkey[owner];
+                if (ANY) {
+                    if (g == 0ull) {
+                        have = false;
+                        pleaf = kEmpty;
+                        st.sp = 0;
+                    }
+                } else if (g < result_key(best, bid)) {
+                    best = key_value(g);
+                    bid = (uint32_t)g;
+                    mine = false;
+                }
+            }
+            done = has && !wait && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+            if (grouped && done) {
+                atomicMin(gkey + owner, ANY ? (hit_any ? 0ull : ~0ull) : result_key(best, bid));
+                atomicSub(gcnt + owner, 1u);
+                wait = true;
+                done = false;
+            }
+        }
+        if (wait && gcnt[owner] == 0u) {  // the last member has finished
+            const unsigned long long fin = gkey[owner];
+            const bool write = ANY || (uint32_t)fin == 0xFFFFFFFFu
+                                   ? lane == owner
+                                   : mine && bid != 0xFFFFFFFFu && result_key(best, bid) == fin;
+            if (write) {
+                float bu = 0.f, bv = 0.f;
+                if (!ANY && bid != 0xFFFFFFFFu) {
+                    const float4 A = tris[3 * (size_t)bpos];
+                    const float4 B = tris[3 * (size_t)bpos + 1];
+                    const float4 C = tris[3 * (size_t)bpos + 2];
+                    float tt;
+                    (void)ray_tri(o, d, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+                }
+                out(h, bid, best, bu, bv, ANY ? fin == 0ull : hit_any);
+            }
+            has = wait = grouped = false;
+            owner = lane;
+        }
         // A lane whose ray is finished keeps its slot until refill_trig lanes of the wave are
-        // idle (or all are): the results are then written and new rays fetched for all of
-        // them at once, so the wave pays for the refill code once per batch rather than
-        // every time some lane finishes.
-        bool done = has && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        // idle (or all are, or the source has run dry): the results are then written and
+        // new rays fetched for all of them at once, so the wave pays for the refill code
+        // once per batch rather than every time some lane finishes.
         const uint32_t n_idle = (uint32_t)__popcll(__ballot(!has || done));
-        if (n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
+        if (drained || n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
             if (done) {
                 // barycentrics of the winner, recomputed once with the same arithmetic
                 float bu = 0.f, bv = 0.f;
@@ -437,31 +555,76 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
                 has = false;
                 done = false;
             }
-            if (src.refill(has, h)) {
-                float4 r0, r1;
-                src.ray(h, r0, r1);
-                o = v3(r0.x, r0.y, r0.z);
-                d = v3(r1.x, r1.y, r1.z);
-                tmin = r0.w;
-                best = r1.w;
-                const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
-                const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
-                const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
-                inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
-                oix = -o.x * inv.x;
-                oiy = -o.y * inv.y;
-                oiz = -o.z * inv.z;
-                bid = 0xFFFFFFFFu;
-                hit_any = false;
-                st.sp = 0;
-                cur = src.root(h);
-                pleaf = kEmpty;
-                have = true;
-                has = true;
+            if (!drained) {
+                if (src.refill(has, h)) {
+                    float4 r0, r1;
+                    src.ray(h, r0, r1);
+                    o = v3(r0.x, r0.y, r0.z);
+                    d = v3(r1.x, r1.y, r1.z);
+                    tmin = r0.w;
+                    best = r1.w;
+                    const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+                    const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+                    const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+                    inv = v3(1.0f / x, 1.0f / y, 1.0f / z);
+                    oix = -o.x * inv.x;
+                    oiy = -o.y * inv.y;
+                    oiz = -o.z * inv.z;
+                    bid = 0xFFFFFFFFu;
+                    hit_any = false;
+                    st.sp = 0;
+                    cur = src.root(h);
+                    pleaf = kEmpty;
+                    have = true;
+                    has = true;
+                    mine = true;
+                    gkey[lane] = ~0ull;
+                    gcnt[lane] = 1u;
+                }
+                drained = src.exhausted();
             }
             if (!__any(has)) break;
         }
-        if (!has || done) continue;
+        if (drained) {
+            // donation: an idle lane takes the top stack entry of a lane with other work left
+            const bool idle = !has;
+            const bool donor = has && !wait && !done && st.sp >= ((have || pleaf != kEmpty) ? 1 : 2);
+            if (__ballot(idle) != 0 && __ballot(donor) != 0) {
+                bool take, give;
+                const uint32_t from = pair_donors(idle, donor, take, give);
+                uint2 e = make_uint2(0u, 0u);
+                if (give) {
+                    e = st.pop();
+                    if (!ANY && !grouped && bid != 0xFFFFFFFFu) atomicMin(gkey + owner, result_key(best, bid));
+                    grouped = true;
+                }
+                e.x = __shfl(e.x, (int)from);
+                e.y = __shfl(e.y, (int)from);
+                owner = __shfl(owner, (int)from);
+                h = Src::shfl(h, from);
+                o = v3(__shfl(o.x, (int)from), __shfl(o.y, (int)from), __shfl(o.z, (int)from));
+                d = v3(__shfl(d.x, (int)from), __shfl(d.y, (int)from), __shfl(d.z, (int)from));
+                inv = v3(__shfl(inv.x, (int)from), __shfl(inv.y, (int)from), __shfl(inv.z, (int)from));
+                oix = __shfl(oix, (int)from);
+                oiy = __shfl(oiy, (int)from);
+                oiz = __shfl(oiz, (int)from);
+                tmin = __shfl(tmin, (int)from);
+                best = __shfl(best, (int)from);
+                if (take) {
+                    atomicAdd(gcnt + owner, 1u);
+                    has = true;
+                    grouped = true;
+                    bid = 0xFFFFFFFFu;
+                    mine = true;
+                    hit_any = false;
+                    st.sp = 0;
+                    pleaf = kEmpty;
+                    cur = e.x;
+                    have = __uint_as_float(e.y) <= best;
+                }
+            }
+        }
+        if (!has || done || wait) continue;
         if (!have && st.sp > 0) {
             const uint2 e = st.pop();
             if (__uint_as_float(e.y) <= best) {
@@ -477,6 +640,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
         if (leaf_step_now(pleaf != kEmpty, node_ready, leaf_trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
+            bool better = false;
             for (uint32_t k = 0; k < cnt; k++) {
                 const float4 A = tris[3 * (size_t)(f + k)];
                 const float4 B = tris[3 * (size_t)(f + k) + 1];
@@ -494,9 +658,13 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
                     best = t;
                     bid = id;
                     bpos = f + k;
+                    mine = true;
+                    better = true;
                 }
             }
             pleaf = kEmpty;
+            // share the lane's best with the rest of its group
+            if (grouped && (ANY ? hit_any : better)) atomicMin(gkey + owner, ANY ? 0ull : result_key(best, bid));
             if (ANY && hit_any) {
                 have = false;  // done: the result is written in the next iteration
             } else if (have && child_is_leaf(cur)) {
@@ -547,27 +715,100 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
     float best = INFINITY, thr = INFINITY;
     uint32_t bid = 0xFFFFFFFFu, cur = 0, pleaf = kEmpty;
     bool have = false;
+    // work donation state, as in ray_stream
+    const uint32_t lane = __lane_id();
+    uint32_t owner = lane;
+    bool grouped = false, wait = false, drained = false;
+    unsigned long long* const gkey = group_keys<S, BS>();
+    uint32_t* const gcnt = group_counts<S, BS>();
     for (;;) {
-        if (has && !have && pleaf == kEmpty && st.sp == 0) {
+        bool done;
+        if (!drained) {
+            done = has && !have && pleaf == kEmpty && st.sp == 0;
+        } else {
+            // groups exist only once the source has run dry (wave-uniform branch)
+            done = has && !wait && !have && pleaf == kEmpty && st.sp == 0;
+            if (grouped && done) {
+                atomicMin(gkey + owner, result_key(best, bid));
+                atomicSub(gcnt + owner, 1u);
+                wait = true;
+                done = false;
+            }
+        }
+        if (wait && gcnt[owner] == 0u) {  // the last member has finished
+            // the owner writes the group's winner (its id is the key's low word)
+            if (lane == owner) out(h, (uint32_t)gkey[owner]);
+            has = wait = grouped = false;
+            owner = lane;
+        }
+        if (done) {
             // only the winner's id leaves the traversal: its point, distance and
             // barycentrics are computed by a separate FP64 pass (cpq_finish_one)
             out(h, bid);
             has = false;
         }
-        if (src.refill(has, h)) {
-            q = src.point(h);
-            best = thr = INFINITY;
-            bid = 0xFFFFFFFFu;
-            st.sp = 0;
-            cur = src.root(h);
-            pleaf = kEmpty;
-            have = true;
-            has = true;
+        if (!drained) {
+            if (src.refill(has, h)) {
+                q = src.point(h);
+                best = thr = INFINITY;
+                bid = 0xFFFFFFFFu;
+                st.sp = 0;
+                cur = src.root(h);
+                pleaf = kEmpty;
+                have = true;
+                has = true;
+                gkey[lane] = ~0ull;
+                gcnt[lane] = 1u;
+            }
+            drained = src.exhausted();
         }
         if (!__any(has)) break;
-        if (!has) continue;
+        if (drained) {
+            // donation: an idle lane takes the top stack entry of a lane with other work left
+            const bool idle = !has;
+            const bool donor = has && !wait && st.sp >= ((have || pleaf != kEmpty) ? 1 : 2);
+            if (__ballot(idle) != 0 && __ballot(donor) != 0) {
+                bool take, give;
+                const uint32_t from = pair_donors(idle, donor, take, give);
+                uint2 e = make_uint2(0u, 0u);
+                if (give) {
+                    e = st.pop();
+                    if (!grouped && bid != 0xFFFFFFFFu) atomicMin(gkey + owner, result_key(best, bid));
+                    grouped = true;
+                }
+                e.x = __shfl(e.x, (int)from);
+                e.y = __shfl(e.y, (int)from);
+                owner = __shfl(owner, (int)from);
+                h = Src::shfl(h, from);
+                q = v3(__shfl(q.x, (int)from), __shfl(q.y, (int)from), __shfl(q.z, (int)from));
+                best = __shfl(best, (int)from);
+                thr = __shfl(thr, (int)from);
+                if (take) {
+                    atomicAdd(gcnt + owner, 1u);
+                    has = true;
+                    grouped = true;
+                    bid = 0xFFFFFFFFu;
+                    st.sp = 0;
+                    pleaf = kEmpty;
+                    cur = e.x;
+                    have = __uint_as_float(e.y) <= thr;
+                }
+            }
+        }
+        if (!has || wait) continue;
         if (!have && st.sp > 0) {
             const uint2 e = st.pop();
+            if (grouped) {
+                // the group's best so far, read beside the stack entry (a read of its own
+                // every iteration costs the long closest-point searches more than the
+                // fresher bound saves) tightens the culls
+                const unsigned long long g = gkey[owner];
+                if (g < result_key(best, bid)) {
+                    best = key_value(g);
+                    bid = (uint32_t)g;
+                    thr = best * kCullSlack;
+                }
+            }
             if (__uint_as_float(e.y) <= thr) {
                 cur = e.x;
                 have = true;
@@ -581,6 +822,7 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
         if (leaf_step_now(pleaf != kEmpty, node_ready, trig & 0xFFu)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
+            bool better = false;
             for (uint32_t k = 0; k < cnt; k++) {
                 const float4 A = tris[3 * (size_t)(f + k)];
                 const float4 B = tris[3 * (size_t)(f + k) + 1];
@@ -593,9 +835,12 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
                     best = d2;
                     bid = id;
                     thr = best * kCullSlack;
+                    better = true;
                 }
             }
             pleaf = kEmpty;
+            // share the lane's best with the rest of its group
+            if (grouped && better) atomicMin(gkey + owner, result_key(best, bid));
             if (have && child_is_leaf(cur)) {
                 pleaf = cur;
                 have = false;
