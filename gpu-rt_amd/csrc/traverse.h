@@ -581,7 +581,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
                     gkey[lane] = ~0ull;
                     gcnt[lane] = 1u;
                 }
-                drained = src.exhausted();
+                drained = !STATS && src.exhausted();  // counting kernels count one lane's visits
             }
             if (!__any(has)) break;
         }
@@ -760,7 +760,7 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
                 gkey[lane] = ~0ull;
                 gcnt[lane] = 1u;
             }
-            drained = src.exhausted();
+            drained = !STATS && src.exhausted();  // counting kernels count one lane's visits
         }
         if (!__any(has)) break;
         if (drained) {

@@ -173,6 +173,30 @@ def test_exact_ties_go_to_lowest_index(ctx, dev, fmt):
     assert p.tolist() == [0, 0] and t.tolist() == [1.0, 1.0]
 
 
+@pytest.mark.parametrize("n", [1, 2, 5, 70])
+def test_few_queries_shared_by_the_whole_wave(ctx, dev, n):
+    """A handful of queries leave most lanes of the wave idle from the start, so nearly
+    every subtree is handed to a helper lane (work donation) and the answers are merged
+    across lanes: still the brute-force answers, ties included."""
+    w = W.rays_scene(n_rays=n, grid=101, n_spheres=6)
+    print("few queries", n, flush=True)
+    _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"])
+    c = W.cpq_small(n_queries=n)
+    _cpq_check(ctx, dev, c["verts"], c["tris"], c["queries"])
+    # queries at the centre of the sphere: every triangle is nearly as close as the best
+    _cpq_check(ctx, dev, c["verts"], c["tris"], np.zeros((n, 3), np.float32) + np.float32(0.01))
+    # 15 copies of one triangle per leaf pair: the tie must resolve to index 0 across helpers
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    F = np.array([[2, 0, 1], [0, 1, 2], [1, 2, 0]] * 40, np.int32)
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev), leaf_size=1)
+    Q = np.tile(np.array([[0.25, 0.25, 1]], np.float32), (n, 1))
+    prim, = _np(bvh.closest_point(_t(Q, dev))[0])
+    assert (prim == 0).all()
+    R = W.pack_rays(np.tile([[0.2, 0.3, 1.0]], (n, 1)), np.tile([[0, 0, -1.0]], (n, 1)), 0.0, 10.0)
+    p, t, _ = _np(*bvh.trace_closest(_t(R, dev)))
+    assert (p == 0).all() and (t == 1.0).all()
+
+
 @pytest.mark.parametrize("fmt", FORMATS)
 def test_shared_vertices_and_edges_of_a_closed_mesh(ctx, dev, fmt):
     """Queries exactly at vertices and on edges of an icosphere: exact FP32 ties
