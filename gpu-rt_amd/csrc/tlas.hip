@@ -68,6 +68,28 @@ struct RaySetup {
     }
 };
 
+// Work donation state of the two-level kernels, after the block's S * BS stack slots:
+// BS (instance, id) keys (8 bytes), BS best-t words and BS member counts (4 bytes each).
+constexpr int kTlasGroupBytes = 16;
+template <int S, int BS>
+__device__ __forceinline__ unsigned long long* tlas_group_k() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+    return reinterpret_cast<unsigned long long*>(lds_stack + S * BS) + w;
+}
+template <int S, int BS>
+__device__ __forceinline__ uint32_t* tlas_group_t() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned long long*>(lds_stack + S * BS) + BS) + w;
+}
+template <int S, int BS>
+__device__ __forceinline__ uint32_t* tlas_group_count() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned long long*>(lds_stack + S * BS) + BS) + BS + w;
+}
+// t as an integer that orders like t (the high word of result_key), and back
+__device__ __forceinline__ uint32_t t_order(float t) { return (uint32_t)(result_key(t, 0u) >> 32); }
+__device__ __forceinline__ float t_from_order(uint32_t o) { return key_value((unsigned long long)o << 32); }
+
 template <typename NodeT, int S, int BS, bool ANY, bool STATS, typename Src, typename Out>
 __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_nodes,
                                                 const float4* __restrict__ top_tris,
@@ -87,6 +109,17 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
     int sp_base = 0;
     const NodeT* bnodes = top_nodes;
     const float4* btris = top_tris;
+    // work donation (traverse.h) with the two-level tie rule (t, instance, id): members
+    // share the group's best t through one LDS word (order-preserving bits; any hit: 0
+    // once a member hit), and when the group is complete the members holding that t
+    // merge (instance << 32 | id) through a second LDS key, whose minimum is the winner
+    const uint32_t lane = __lane_id();
+    uint32_t owner = lane;
+    bool grouped = false, wait = false, drained = false;
+    bool mine = true;  // (closest hit) binst/bid/bpos were found by this lane
+    uint32_t* const gt = tlas_group_t<S, BS>();
+    unsigned long long* const gk = tlas_group_k<S, BS>();
+    uint32_t* const gcnt = tlas_group_count<S, BS>();
     // the trigger word: bits 0-7 the leaf trigger, bits 8-15 the refill trigger
     const uint32_t leaf_trig = trig & 0xFFu;
     const uint32_t refill_trig = max((trig >> 8) & 0xFFu, 1u);
@@ -99,11 +132,70 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
             src.ray(h, r0, r1);
             r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
         }
+        bool done;
+        if (!drained) {
+            done = has && !in_inst && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        } else {
+            // groups exist only once the source has run dry (wave-uniform branch)
+            if (grouped && has && !wait) {
+                const uint32_t g = gt[owner];
+                if (ANY) {
+                    if (g == 0u) {  // a member hit: drop both levels
+                        have = false;
+                        pleaf = kEmpty;
+                        st.sp = 0;
+                        in_inst = false;
+                    }
+                } else if (g < t_order(best)) {  // a member's hit is closer: cull with it
+                    best = t_from_order(g);
+                    bid = binst = 0xFFFFFFFFu;
+                    mine = false;
+                }
+            }
+            done = has && !wait && !in_inst && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+            if (grouped && done) {
+                if (ANY ? hit_any : bid != 0xFFFFFFFFu) atomicMin(gt + owner, ANY ? 0u : t_order(best));
+                atomicSub(gcnt + owner, 1u);
+                wait = true;
+                done = false;
+            }
+        }
+        if (wait && gcnt[owner] == 0u) {  // the last member has finished
+            bool write, any_hit = false;
+            if (ANY) {
+                any_hit = gt[owner] == 0u;
+                write = lane == owner;
+            } else {
+                const bool cand = mine && bid != 0xFFFFFFFFu && t_order(best) == gt[owner];
+                const unsigned long long mk = (unsigned long long)binst << 32 | bid;
+                if (cand) atomicMin(gk + owner, mk);
+                const unsigned long long kf = gk[owner];
+                write = cand ? mk == kf : (kf == ~0ull && lane == owner);
+            }
+            if (write) {
+                float bu = 0.f, bv = 0.f;
+                uint32_t obj = 0xFFFFFFFFu;
+                if (!ANY && bid != 0xFFFFFFFFu) {
+                    const InstDev& I = inst[binst];
+                    V3 oo, od;
+                    to_object(I, r.o, r.d, oo, od);
+                    const float4 A = I.tris[3 * (size_t)bpos];
+                    const float4 B = I.tris[3 * (size_t)bpos + 1];
+                    const float4 C = I.tris[3 * (size_t)bpos + 2];
+                    float tt;
+                    (void)ray_tri(oo, od, v3(A.x, A.y, A.z), v3(B.x, B.y, B.z), v3(C.x, C.y, C.z), tt, bu, bv);
+                    obj = I.custom;
+                }
+                out(h, obj, bid, best, bu, bv, ANY ? any_hit : hit_any);
+            }
+            has = wait = grouped = false;
+            owner = lane;
+        }
         // a finished lane keeps its slot until refill_trig lanes of the wave are idle (or
-        // all are); results are then written and new rays fetched for all of them at once
-        bool done = has && !in_inst && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
+        // all are, or the source has run dry); results are then written and new rays
+        // fetched for all of them at once
         const uint32_t n_idle = (uint32_t)__popcll(__ballot(!has || done));
-        if (n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
+        if (drained || n_idle >= refill_trig || n_idle == (uint32_t)__popcll(__ballot(true))) {
             if (done) {
                 float bu = 0.f, bv = 0.f;
                 uint32_t obj = 0xFFFFFFFFu;
@@ -123,25 +215,84 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
                 has = false;
                 done = false;
             }
-            if (src.refill(has, h)) {
-                float4 r0, r1;
-                src.ray(h, r0, r1);
-                r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
-                tmin = r0.w;
-                best = r1.w;
-                bid = binst = 0xFFFFFFFFu;
-                hit_any = false;
-                st.sp = 0;
-                sp_base = 0;
-                in_inst = false;
-                cur = 0;
-                pleaf = kEmpty;
-                have = true;
-                has = true;
+            if (!drained) {
+                if (src.refill(has, h)) {
+                    float4 r0, r1;
+                    src.ray(h, r0, r1);
+                    r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
+                    tmin = r0.w;
+                    best = r1.w;
+                    bid = binst = 0xFFFFFFFFu;
+                    hit_any = false;
+                    st.sp = 0;
+                    sp_base = 0;
+                    in_inst = false;
+                    cur = 0;
+                    pleaf = kEmpty;
+                    have = true;
+                    has = true;
+                    mine = true;
+                    gt[lane] = ~0u;
+                    gk[lane] = ~0ull;
+                    gcnt[lane] = 1u;
+                }
+                drained = !STATS && src.exhausted();  // counting kernels count one lane's visits
             }
             if (!__any(has)) break;
         }
-        if (!has || done) continue;
+        if (drained) {
+            // donation: an idle lane takes the top stack entry of a lane with other work
+            // left -- a BLAS entry when the donor is inside an instance (the recipient then
+            // works in the same object space), else a top-level entry
+            const bool idle = !has;
+            const bool donor = has && !wait && !done && (!in_inst || st.sp > sp_base) &&
+                               st.sp >= ((have || pleaf != kEmpty) ? 1 : 2);
+            if (__ballot(idle) != 0 && __ballot(donor) != 0) {
+                bool take, give;
+                const uint32_t from = pair_donors(idle, donor, take, give);
+                uint2 e = make_uint2(0u, 0u);
+                if (give) {
+                    e = st.pop();
+                    if (!ANY && !grouped && bid != 0xFFFFFFFFu) atomicMin(gt + owner, t_order(best));
+                    grouped = true;
+                }
+                e.x = __shfl(e.x, (int)from);
+                e.y = __shfl(e.y, (int)from);
+                owner = __shfl(owner, (int)from);
+                h = Src::shfl(h, from);
+                r.o = v3(__shfl(r.o.x, (int)from), __shfl(r.o.y, (int)from), __shfl(r.o.z, (int)from));
+                r.d = v3(__shfl(r.d.x, (int)from), __shfl(r.d.y, (int)from), __shfl(r.d.z, (int)from));
+                r.inv = v3(__shfl(r.inv.x, (int)from), __shfl(r.inv.y, (int)from), __shfl(r.inv.z, (int)from));
+                r.oix = __shfl(r.oix, (int)from);
+                r.oiy = __shfl(r.oiy, (int)from);
+                r.oiz = __shfl(r.oiz, (int)from);
+                tmin = __shfl(tmin, (int)from);
+                best = __shfl(best, (int)from);
+                const bool d_in = __shfl((int)in_inst, (int)from) != 0;
+                const uint32_t d_inst = __shfl(cinst, (int)from);
+                if (take) {
+                    atomicAdd(gcnt + owner, 1u);
+                    has = true;
+                    grouped = true;
+                    mine = true;
+                    bid = binst = 0xFFFFFFFFu;
+                    hit_any = false;
+                    st.sp = 0;
+                    sp_base = 0;
+                    pleaf = kEmpty;
+                    in_inst = d_in;
+                    cinst = d_inst;
+                    if (d_in) {
+                        const InstDev& I = inst[d_inst];
+                        bnodes = reinterpret_cast<const NodeT*>(I.nodes);
+                        btris = I.tris;
+                    }
+                    cur = e.x;
+                    have = __uint_as_float(e.y) <= best;
+                }
+            }
+        }
+        if (!has || done || wait) continue;
         if (!have && st.sp > (in_inst ? sp_base : 0)) {
             const uint2 e = st.pop();
             if (__uint_as_float(e.y) <= best) {
@@ -175,6 +326,7 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
         if (leaf_step_now(pleaf != kEmpty, node_ready, leaf_trig)) {
             if (pleaf == kEmpty) continue;
             const uint32_t f = leaf_first(pleaf), cnt = leaf_count(pleaf);
+            bool better = false;
             for (uint32_t k = 0; k < cnt; k++) {
                 const float4 A = btris[3 * (size_t)(f + k)];
                 const float4 B = btris[3 * (size_t)(f + k) + 1];
@@ -193,9 +345,13 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
                     bid = id;
                     binst = cinst;
                     bpos = f + k;
+                    mine = true;
+                    better = true;
                 }
             }
             pleaf = kEmpty;
+            // share the lane's best with the rest of its group
+            if (grouped && (ANY ? hit_any : better)) atomicMin(gt + owner, ANY ? 0u : t_order(best));
             if (ANY && hit_any) {
                 // done: drop both levels; the world ray is restored before the result
                 have = false;
@@ -234,10 +390,12 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
 }
 
 // Resident waves per SIMD the two-level kernels are compiled for: the second level's
-// state (instance, stack base, BLAS pointers) does not fit 64 VGPRs without spilling, so
-// 4-wide nodes get up to 72 (7 waves); 8-wide nodes take what they need.
-template <int W>
-constexpr int kTlasWaves = W == 8 ? 4 : 7;
+// state (instance, stack base, BLAS pointers) and the work-donation state do not fit 64
+// VGPRs without spilling.  4-wide closest hit gets up to 85 (6 waves: at 72 the LDS stack
+// address spilled to scratch and every pop waited for it), 4-wide any hit fits 72 (7
+// waves, 8% faster than 6); 8-wide nodes take what they need.
+template <int W, bool ANY>
+constexpr int kTlasWaves = W == 8 ? 4 : (ANY ? 7 : 6);
 
 struct TlasArgs {
     const void* top_nodes;
@@ -260,7 +418,7 @@ struct TlasArgs {
 };
 
 template <int W, bool Q, int S, int BS, bool ANY, bool STATS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W>))) void k_tlas_ray(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W, ANY>))) void k_tlas_ray(
     TlasArgs a) {
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
@@ -297,9 +455,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W
 template <int W, bool Q, int S, int BS, bool STATS>
 static void launch_tlas_kind(bool any, const TlasArgs& a, unsigned grid, hipStream_t s) {
     if (any)
-        hipLaunchKernelGGL((k_tlas_ray<W, Q, S, BS, true, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+        hipLaunchKernelGGL((k_tlas_ray<W, Q, S, BS, true, STATS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kTlasGroupBytes) * BS, s, a);
     else
-        hipLaunchKernelGGL((k_tlas_ray<W, Q, S, BS, false, STATS>), dim3(grid), dim3(BS), S * BS * sizeof(uint2), s, a);
+        hipLaunchKernelGGL((k_tlas_ray<W, Q, S, BS, false, STATS>), dim3(grid), dim3(BS), (S * sizeof(uint2) + kTlasGroupBytes) * BS, s, a);
 }
 
 template <int W, bool Q, bool STATS>
@@ -553,16 +711,20 @@ void gpurt_tlas_destroy(gpurt_tlas t) {
 namespace gpurt_impl {
 
 // Defaults of the two-level kernels (opts fields left 0): every wave slot the register
-// budget allows (7 waves/SIMD), a leaf trigger of 32 lanes, and a refill when 16 lanes of
+// budget allows (kTlasWaves), a leaf trigger of 32 lanes, and a refill when 16 lanes of
 // a closest-hit wave are idle, or at once for any hit, whose rays in the two-level scene
 // are mostly short (measured on the 65-instance rays workload: any hit 3.38 -> 3.81 G
 // rays/s, closest 2.25 -> 2.37 against the single-level kernels' earlier defaults, and
 // closest 2.00 -> 2.18 from the refill trigger).
 constexpr int kTlasDefaultTrigger = 32;
 
+static int tlas_waves(int W, bool any) {
+    return W == 8 ? (any ? kTlasWaves<8, true> : kTlasWaves<8, false>) : (any ? kTlasWaves<4, true> : kTlasWaves<4, false>);
+}
+
 static gpurt_query_opts tlas_opts(const gpurt_query_opts* opts, int W, bool any) {
     gpurt_query_opts o = opts ? *opts : gpurt_query_opts{};
-    if (o.waves_per_cu == 0) o.waves_per_cu = 4 * (W == 8 ? kTlasWaves<8> : kTlasWaves<4>);
+    if (o.waves_per_cu == 0) o.waves_per_cu = 4 * tlas_waves(W, any);
     if (o.leaf_trigger == 0) o.leaf_trigger = kTlasDefaultTrigger;
     if (o.refill_trigger == 0) o.refill_trigger = any ? 1 : 16;
     return o;
@@ -575,7 +737,7 @@ static int tlas_plan(gpurt_tlas T, size_t n, bool any, const gpurt_query_opts* o
     // worst-case stack: the top levels' entries plus one BLAS's
     const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
     return plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
-                          W == 8 ? kTlasWaves<8> : kTlasWaves<4>, 0);
+                          tlas_waves(W, any), kTlasGroupBytes);
 }
 
 size_t tlas_scratch_bytes(gpurt_tlas T, size_t n) {
