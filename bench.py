@@ -127,11 +127,29 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     uv = torch.empty((n, 2), dtype=torch.float32, device=dev)
     occ = torch.empty(n, dtype=torch.uint8, device=dev)
 
-    def step():
+    def step_serial():
         bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
         bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
 
-    ms = timed(step, args.steps, args.warmup)
+    # The two passes are independent, so a step issues the any-hit pass on a second
+    # stream beside the closest-hit pass (the way the reference's ray-generation shader
+    # keeps both ray kinds in flight) and joins before the next step; the end of one
+    # launch, where few lanes are busy, then overlaps the other's work.
+    s_any = torch.cuda.Stream(dev)
+
+    def step_overlapped():
+        fork = torch.cuda.Event()
+        fork.record()
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+        with torch.cuda.stream(s_any):
+            s_any.wait_event(fork)
+            bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+            join = torch.cuda.Event()
+            join.record(s_any)
+        torch.cuda.current_stream().wait_event(join)
+
+    ms_serial = timed(step_serial, args.steps, args.warmup)
+    ms = ms_serial if args.serial else timed(step_overlapped, args.steps, args.warmup)
     ms_closest = timed(lambda: bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger),
                        args.steps, 1)
     ms_any = timed(lambda: bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger), args.steps, 1)
@@ -195,7 +213,8 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
         "config": {"workload": "rays_scene", "n_tris": int(F.shape[0]), "n_rays": n,
-                   "passes": ["closest_hit", "any_hit"], "node_width": args.width,
+                   "passes": ["closest_hit", "any_hit"], "streams": 1 if args.serial else 2,
+                   "node_width": args.width,
                    "leaf_size": args.leaf, "stack_depth": args.stack, "treelet_size": args.treelet,
                    "node_format": info["node_format"], "waves_per_cu": args.waves,
                    "leaf_trigger": args.leaf_trigger, "refill_trigger": args.refill_trigger},
@@ -206,7 +225,10 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                                    f"{threads} threads); its single-thread build "
                                    f"({cpu_build_s:.1f}s) is not included",
                          "seconds": cpu_s, "build_s": cpu_build_s},
-        "extra_rays": {"closest_rays_per_s": n / (ms_closest * 1e-3),
+        "extra_rays": {"step": "closest-hit and any-hit passes one after the other" if args.serial else
+                       "any-hit pass on a second stream beside the closest-hit pass, joined every step",
+                       "serial_rays_per_s": 2 * n / (ms_serial * 1e-3), "ms_per_step_serial": ms_serial,
+                       "closest_rays_per_s": n / (ms_closest * 1e-3),
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
                        "hit_rate": hit_rate, "occluded_rate": occ_rate,
                        "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
@@ -499,6 +521,8 @@ def main():
     ap.add_argument("--leaf-trigger", type=int, default=0,
                     help="lanes (of 64) with a postponed leaf that switch a wave to leaf tests "
                          "(0 = library default)")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the headline step's two passes one after the other on one stream")
     ap.add_argument("--refill-trigger", type=int, default=0,
                     help="idle lanes (of 64) at which a ray wave writes its finished rays and "
                          "fetches new ones (0 = library default)")
