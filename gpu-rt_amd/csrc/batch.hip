@@ -450,7 +450,13 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     const int hint = W == 8 ? (any_f32 ? WavesHint<8, false>::value : WavesHint<8, true>::value)
                             : WavesHint<4, false>::value;
     int blocks_per_cu = std::min(ctx->lds_per_cu / ((S * 8 + kGroupKeyBytes) * BS), std::min(32, 4 * hint) / waves_per_block);
-    if (qopts && qopts->waves_per_cu > 0) blocks_per_cu = std::max(1, qopts->waves_per_cu / waves_per_block);
+    // default 4 waves/SIMD, as the single-mesh closest-point kernels, which dominate a
+    // batch (256-mesh batch: 36.1 -> 34.9 ms per replay against every slot registers allow)
+    constexpr int kBatchDefaultWaves = 16;
+    if (qopts && qopts->waves_per_cu > 0)
+        blocks_per_cu = std::max(1, qopts->waves_per_cu / waves_per_block);
+    else
+        blocks_per_cu = std::min(blocks_per_cu, std::max(1, kBatchDefaultWaves / waves_per_block));
     blocks_per_cu = std::max(1, blocks_per_cu);
     b->grid = (unsigned)std::max<size_t>(1, (size_t)ctx->n_cu * blocks_per_cu);
     // deepest possible wide tree (kMaxLevels levels) bounds the stack: no overflow
