@@ -41,7 +41,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
-    QueueSource src{nullptr, a.q, a.n, gid, total, a.counter};
+    QueueSource src{nullptr, a.q, a.n, first_query(gid, total), total, a.counter};
     auto out = [&](uint32_t qi, uint32_t id) { a.prim[qi] = id == 0xFFFFFFFFu ? -1 : (int32_t)id; };
     if (!cpq_stream<NodeOf<W, Q>, S, BS, STATS>(nodes, a.tris, src, a.leaf_trigger, st, out, n_nodes, n_tris))
         gatomic_or(a.err, 1u);
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WavesHint<W,
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     unsigned long long n_nodes = 0, n_tris = 0;
-    QueueSource src{a.rays, nullptr, a.n, gid, total, a.counter};
+    QueueSource src{a.rays, nullptr, a.n, first_query(gid, total), total, a.counter};
     auto out = [&](uint32_t ri, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
             a.occ[ri] = any ? 1 : 0;
@@ -155,7 +155,10 @@ int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need
     else if (default_waves_per_cu(kind) > 0)
         blocks_per_cu = std::min(blocks_per_cu, std::max(1, default_waves_per_cu(kind) / waves_per_block));
     blocks_per_cu = std::max(blocks_per_cu, 1);
-    p.grid = (unsigned)std::min<size_t>(cdiv(n, p.BS), (size_t)ctx->n_cu * blocks_per_cu);
+    // up to kSpread lanes per query: a small set is spread over more waves (first_query),
+    // whose idle lanes take work from the busy ones once the queues are empty
+    constexpr size_t kSpread = 4;
+    p.grid = (unsigned)std::min<size_t>(cdiv(n * kSpread, p.BS), (size_t)ctx->n_cu * blocks_per_cu);
     if (p.grid == 0) p.grid = 1;
     p.total = p.grid * p.BS;
     // the part of the worst-case stack beyond the LDS slots spills to device memory

@@ -425,7 +425,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W
     LaneStack<S, BS> st;
     st.init(a.spill, a.spill_slots, gid, total);
     const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
-    QueueSource src{a.rays, nullptr, n, gid, total, a.counter};
+    QueueSource src{a.rays, nullptr, n, first_query(gid, total), total, a.counter};
     auto out = [&](uint32_t ri, uint32_t obj, uint32_t id, float t, float u, float v, bool any) {
         if (ANY) {
             a.occ[ri] = any ? 1 : 0;
