@@ -60,6 +60,27 @@ def test_tlas_wave_scheduling_options_keep_answers(ctx, dev, leaf, refill):
            qkw=dict(leaf_trigger=leaf, refill_trigger=refill, waves_per_cu=4))
 
 
+@pytest.mark.parametrize("n", [1, 3, 70])
+def test_tlas_few_rays_shared_by_the_whole_wave(ctx, dev, n):
+    """A handful of rays leave most lanes idle from the start, so subtrees of both levels
+    are handed to helper lanes (work donation) and merged by (t, instance, id): still the
+    instanced brute force's answers."""
+    sc = W.instanced_scene(n_rays=n, grid=41, n_spheres=12, sphere_level=2, seed=9)
+    print("tlas few rays", n, flush=True)
+    _check(ctx, dev, sc, sc["rays"], sc["rays_any"])
+    # coincident instances: equal t in several instances, the lowest instance must win
+    V, F = sc["meshes"][1]
+    blas = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev))
+    xf = np.eye(4, dtype=np.float32)[:3]
+    tlas = gpurt.Tlas(ctx, [(blas, xf, 10 + i, 0xFF) for i in range(6)])
+    R = W.pack_rays(np.tile([[0.05, 0.1, 5.0]], (n, 1)), np.tile([[0.0, 0.0, -1.0]], (n, 1)), 0.0, 100.0)
+    obj, prim, t, _ = _np(*tlas.trace_closest(_t(R, dev)))
+    eo, ep, et, _ = O.inst_ray_brute_f32(np.tile(xf.reshape(1, 12), (6, 1)), np.zeros(6, np.int32), [(V, F)], R,
+                                          custom=list(range(10, 16)))
+    assert np.array_equal(obj, eo) and np.array_equal(prim, ep) and np.array_equal(t, et)
+    assert (obj == 10).all()
+
+
 def test_tlas_masks_and_custom_indices(ctx, dev):
     sc = W.instanced_scene(n_rays=4000, grid=33, n_spheres=8, sphere_level=2, seed=6)
     n = sc["xf"].shape[0]
