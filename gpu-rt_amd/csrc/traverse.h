@@ -1,16 +1,18 @@
 // traverse.h -- BVH traversal shared by the plain kernels (query.hip) and the persistent
 // batch kernels (batch.hip).
 //
-// One lane per query.  A lane keeps a stack of (node-or-leaf reference, lower bound)
-// pairs: the first S entries live in LDS laid out [slot][lane] (consecutive lanes hit
-// consecutive banks at every depth), deeper entries spill to a device buffer that the
-// launcher sizes from the tree depth, so no traversal can overflow.  A wide node is
-// fetched with unconditional dwordx4 loads (FP32 boxes: 7 for W=4; 8-bit boxes: 4);
-// children are bounded, sorted by the bound with a sorting network, and pushed
-// far-to-near so the nearest is taken next without a stack round trip.  Culls compare against the best value with a small
-// relative slack and boxes are inflated at build time, so a child that can hold an
-// equal-or-better primitive is never culled: results do not depend on the tree's
-// shape, and ties resolve to the lowest primitive id.
+// One lane per query while the work queues last; at the end of a launch the lanes left
+// without a query help the busy ones (work donation, below).  A lane keeps a stack of
+// (node-or-leaf reference, lower bound) pairs: the first S entries live in LDS laid out
+// [slot][lane] (consecutive lanes hit consecutive banks at every depth), deeper entries
+// spill to a device buffer that the launcher sizes from the tree depth, so no traversal
+// can overflow.  A wide node is fetched with unconditional dwordx4 loads (FP32 boxes: 7
+// for W=4; 8-bit boxes: 4); children are bounded, sorted by the bound with a sorting
+// network, and pushed far-to-near so the nearest is taken next without a stack round
+// trip.  Culls compare against the best value with a small relative slack and boxes are
+// inflated at build time, so a child that can hold an equal-or-better primitive is never
+// culled: results do not depend on the tree's shape, and ties resolve to the lowest
+// primitive id.
 #pragma once
 
 #include "gpurt_internal.h"
