@@ -549,8 +549,8 @@ def main():
                     help="kernel of the --profile run: closest-hit rays or closest-point queries")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box "
-              "here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     import gpurt

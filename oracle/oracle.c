@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or calls
  * this file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * use it, and only as the checker (or, for the baseline, as the timed CPU port).
+ * use it, and only as the test oracle (or, for the baseline, as the timed CPU port).
  *
  * The reference (TheNumbat/GPU-RT, main branch) builds and traverses its BVH inside
  * the closed Vulkan driver (src/vk/vulkan.cpp:877 vkCmdBuildAccelerationStructuresKHR,

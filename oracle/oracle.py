@@ -1,7 +1,7 @@
 """numpy/ctypes wrapper of liboracle.so (oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, always as the checker or the timed CPU port -- never by the product
+cpu_baseline leg, always as the test oracle or the timed CPU port -- never by the product
 package.  See oracle.c for what each function restates and which reference lines it
 follows.
 """

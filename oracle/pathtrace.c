@@ -1,7 +1,7 @@
 /*
  * pathtrace.c -- CPU restatement of the reference's path tracer (TEST INFRASTRUCTURE ONLY).
  *
- * Only tests/ and bench.py's cpu_baseline leg use it, as the checker.  It restates the
+ * Only tests/ and bench.py's cpu_baseline leg use it, as the test oracle.  It restates the
  * ray-generation shader src/shaders/rt/rt.rgen in its own per-pixel form: for every pixel
  * a loop over samples and bounces (main, rt.rgen:611-677) calling
  *   make_camera_ray (597-609), trace_ray / visibility (257-291) -- here the FP32

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the test oracle.
 
 The reference has no tests, fixtures or CPU BVH (its BVH lives in the Vulkan driver),
 so the oracle is pinned by: known answers a plausible mistake breaks (wrong region,
