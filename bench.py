@@ -135,15 +135,20 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     # stream beside the closest-hit pass (the way the reference's ray-generation shader
     # keeps both ray kinds in flight) and joins before the next step; the end of one
     # launch, where few lanes are busy, then overlaps the other's work.
+    # Each concurrent pass launches 16 waves per CU (half of a CU's 32 wave slots) so that
+    # both are resident together, and refills at 8 idle lanes; alone, a pass keeps the
+    # library defaults.
     s_any = torch.cuda.Stream(dev)
+    waves_overlap = args.waves or 16
+    refill_overlap = args.refill_trigger or 8
 
     def step_overlapped():
         fork = torch.cuda.Event()
         fork.record()
-        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+        bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=waves_overlap, leaf_trigger=args.leaf_trigger, refill_trigger=refill_overlap)
         with torch.cuda.stream(s_any):
             s_any.wait_event(fork)
-            bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger)
+            bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=waves_overlap, leaf_trigger=args.leaf_trigger, refill_trigger=refill_overlap)
             join = torch.cuda.Event()
             join.record(s_any)
         torch.cuda.current_stream().wait_event(join)
@@ -226,7 +231,8 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                                    f"({cpu_build_s:.1f}s) is not included",
                          "seconds": cpu_s, "build_s": cpu_build_s},
         "extra_rays": {"step": "closest-hit and any-hit passes one after the other" if args.serial else
-                       "any-hit pass on a second stream beside the closest-hit pass, joined every step",
+                       "any-hit pass on a second stream beside the closest-hit pass, joined every step; "
+                       "each pass at %d waves per CU, refill trigger %d" % (waves_overlap, refill_overlap),
                        "serial_rays_per_s": 2 * n / (ms_serial * 1e-3), "ms_per_step_serial": ms_serial,
                        "closest_rays_per_s": n / (ms_closest * 1e-3),
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
