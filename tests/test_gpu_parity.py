@@ -106,6 +106,28 @@ def test_wave_scheduling_options_keep_answers(ctx, dev, leaf, refill):
                refill_trigger=refill, waves_per_cu=4)
 
 
+def test_repeated_launches_are_identical(ctx, dev):
+    """Work donation groups lanes by timing; the answers must not depend on it: repeated
+    launches (full grid, few queries, few waves) all equal the brute force."""
+    w = W.rays_scene(n_rays=20000, grid=101, n_spheres=6)
+    V, F, R = w["verts"], w["tris"], w["rays"]
+    bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev))
+    ep, et, euv = O.ray_closest_brute_f32(V, F, R)
+    eo = O.ray_any_brute_f32(V, F, w["rays_any"])
+    c = W.cpq_small(n_queries=3000)
+    cb = gpurt.Bvh(ctx, _t(c["verts"], dev), _t(c["tris"], dev))
+    cp = O.cpq_brute_f32(c["verts"], c["tris"], c["queries"])[0]
+    tr, tra, tq = _t(R, dev), _t(w["rays_any"], dev), _t(c["queries"], dev)
+    for it in range(6):
+        for kw in ({}, dict(waves_per_cu=4), dict(waves_per_cu=16, refill_trigger=8)):
+            p, t, uv = _np(*bvh.trace_closest(tr, **kw))
+            assert np.array_equal(p, ep) and np.array_equal(t, et) and np.array_equal(uv, euv), (it, kw)
+            occ, = _np(bvh.trace_any(tra, **kw))
+            assert np.array_equal(occ, eo), (it, kw)
+            prim, = _np(cb.closest_point(tq, **{k: v for k, v in kw.items() if k != "refill_trigger"})[0])
+            assert np.array_equal(prim, cp), (it, kw)
+
+
 @pytest.mark.parametrize("stride", [3, 4, 12])
 def test_vertex_strides(ctx, dev, stride):
     """12-, 16- and 48-byte vertex records (48 = the reference's Mesh::Vertex)."""
