@@ -711,11 +711,10 @@ void gpurt_tlas_destroy(gpurt_tlas t) {
 namespace gpurt_impl {
 
 // Defaults of the two-level kernels (opts fields left 0): every wave slot the register
-// budget allows (kTlasWaves), a leaf trigger of 32 lanes, and a refill when 16 lanes of
-// a closest-hit wave are idle, or at once for any hit, whose rays in the two-level scene
-// are mostly short (measured on the 65-instance rays workload: any hit 3.38 -> 3.81 G
-// rays/s, closest 2.25 -> 2.37 against the single-level kernels' earlier defaults, and
-// closest 2.00 -> 2.18 from the refill trigger).
+// budget allows (kTlasWaves), a leaf trigger of 32 lanes, and a refill when 24 lanes of a
+// closest-hit wave or 8 of an any-hit wave are idle (swept on the 65-instance rays
+// workload with work donation: closest 2.58 -> 2.70, any hit 4.17 -> 4.29 G rays/s
+// against 16 and 1).
 constexpr int kTlasDefaultTrigger = 32;
 
 static int tlas_waves(int W, bool any) {
@@ -726,7 +725,7 @@ static gpurt_query_opts tlas_opts(const gpurt_query_opts* opts, int W, bool any)
     gpurt_query_opts o = opts ? *opts : gpurt_query_opts{};
     if (o.waves_per_cu == 0) o.waves_per_cu = 4 * tlas_waves(W, any);
     if (o.leaf_trigger == 0) o.leaf_trigger = kTlasDefaultTrigger;
-    if (o.refill_trigger == 0) o.refill_trigger = any ? 1 : 16;
+    if (o.refill_trigger == 0) o.refill_trigger = any ? 8 : 24;
     return o;
 }
 
