@@ -242,6 +242,39 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
         const int byte = row * W + j;
         return (float)((w[Q0 + byte / 4] >> (8 * (byte % 4))) & 0xFFu);
     };
+    if constexpr (W % 4 == 0) {
+        // Near and far planes chosen once per node by the direction's signs.  The
+        // builder gives every non-empty child lo byte <= hi byte, and q -> fmaf(q, a, b)
+        // is non-decreasing for a >= 0 and non-increasing for a <= 0 (a has inv's sign),
+        // so the row picked is exactly the min / max of the two plane distances.  With
+        // W % 4 == 0 each row is whole words, so the choice is a word select per row word
+        // instead of a min and a max per child and axis.
+        constexpr int RW = W / 4;  // words per byte row
+        const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
+        uint32_t nw[3][RW], fw[3][RW];
+#pragma unroll
+        for (int k = 0; k < RW; k++) {
+            nw[0][k] = nx ? w[Q0 + 3 * RW + k] : w[Q0 + k];
+            fw[0][k] = nx ? w[Q0 + k] : w[Q0 + 3 * RW + k];
+            nw[1][k] = ny ? w[Q0 + 4 * RW + k] : w[Q0 + RW + k];
+            fw[1][k] = ny ? w[Q0 + RW + k] : w[Q0 + 4 * RW + k];
+            nw[2][k] = nz ? w[Q0 + 5 * RW + k] : w[Q0 + 2 * RW + k];
+            fw[2][k] = nz ? w[Q0 + 2 * RW + k] : w[Q0 + 5 * RW + k];
+        }
+        auto bt = [](uint32_t v, int j) -> float { return (float)((v >> (8 * (j % 4))) & 0xFFu); };
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const uint32_t ch = w[4 + j];
+            const float lo = fmaxf(fmaxf(fmaf(bt(nw[0][j / 4], j), ax, bx), fmaf(bt(nw[1][j / 4], j), ay, by)),
+                                   fmaxf(fmaf(bt(nw[2][j / 4], j), az, bz), tmin));
+            const float hi = fminf(fminf(fmaf(bt(fw[0][j / 4], j), ax, bx), fmaf(bt(fw[1][j / 4], j), ay, by)),
+                                   fminf(fmaf(bt(fw[2][j / 4], j), az, bz), best));
+            const bool hb = (ch != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+            tn[j] = hb ? lo : INFINITY;
+            c[j] = hb ? ch : kEmpty;
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < W; j++) {
         const uint32_t ch = w[4 + j];
