@@ -443,7 +443,9 @@ struct QueueSource {
 // lowest-id tie rule.  The owner's LDS count holds the members still traversing (one at
 // the start, one more per helper, one less per member that finishes); when it reaches
 // zero, the member that found the winning key writes the result (the owner when nothing
-// was found) and all members go idle.  Every subtree stays with exactly one lane, and
+// was found) and all members go idle.  For closest points only the id leaves the
+// traversal and it is the key's low word, so the owner writes it and a helper is free to
+// take more work as soon as its own subtrees are done.  Every subtree stays with exactly one lane, and
 // every lane culls only against a value that some primitive (or the query's own limit)
 // reached, so the answer is the one a single lane finds.
 // ------------------------------------------------------------------------------------
@@ -776,8 +778,17 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
             if (grouped && done) {
                 atomicMin(gkey + owner, result_key(best, bid));
                 atomicSub(gcnt + owner, 1u);
-                wait = true;
                 done = false;
+                if (lane == owner) {
+                    wait = true;  // the owner holds the query and writes the group's result
+                } else {
+                    // a helper's best is in the group key, so it is free at once to take
+                    // more of any group's work (one query near the centre of a closed
+                    // surface can otherwise keep its wave busy many times longer than
+                    // every other query of the launch together)
+                    has = grouped = false;
+                    owner = lane;
+                }
             }
         }
         if (wait && gcnt[owner] == 0u) {  // the last member has finished
