@@ -557,10 +557,20 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
             }
             done = has && !wait && !have && pleaf == kEmpty && (st.sp == 0 || (ANY && hit_any));
             if (grouped && done) {
-                atomicMin(gkey + owner, ANY ? (hit_any ? 0ull : ~0ull) : result_key(best, bid));
+                const unsigned long long k = ANY ? (hit_any ? 0ull : ~0ull) : result_key(best, bid);
+                atomicMin(gkey + owner, k);
                 atomicSub(gcnt + owner, 1u);
-                wait = true;
                 done = false;
+                // A helper that cannot write the result leaves at once to take more work:
+                // any-hit results are written by the owner, and a closest-hit helper is the
+                // writer only if it found a hit whose key is still the group's (keys only
+                // fall, and each primitive lies in one subtree).
+                if (lane != owner && (ANY || !mine || bid == 0xFFFFFFFFu || gkey[owner] < k)) {
+                    has = grouped = false;
+                    owner = lane;
+                } else {
+                    wait = true;
+                }
             }
         }
         if (wait && gcnt[owner] == 0u) {  // the last member has finished
