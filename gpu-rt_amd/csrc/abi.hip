@@ -110,6 +110,16 @@ int gpurt_ctx_create(int device, gpurt_ctx* out) {
         delete c;
         return set_error(GPURT_E_DEVICE, "context init failed");
     }
+    // the fixed traversal scratch of gpurt_trace_closest_noalloc: its device address,
+    // once per device (this device is current here)
+    {
+        const int rc = noalloc_scratch_init(device);
+        if (rc) {
+            (void)hipFree(c->err);
+            delete c;
+            return rc;
+        }
+    }
     // keep freed stream-ordered allocations (traversal spill stacks) in the pool
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
@@ -273,6 +283,19 @@ int gpurt_trace_closest(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* p
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     return launch_ray(b, rays, n, prim, t, uv, nullptr, false, opts, (hipStream_t)stream);
+}
+
+int gpurt_trace_closest_noalloc(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
+                                float* uv, const gpurt_query_opts* opts, void* stream) {
+    if (!b) return set_error(GPURT_E_INVALID, "null bvh");
+    if (n == 0) return GPURT_OK;
+    if (!rays || !prim || !t) return set_error(GPURT_E_INVALID, "null ray/output pointer");
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    if (opts && opts->stats) return set_error(GPURT_E_UNSUPPORTED, "stats are not counted by the no-allocation variant");
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    return launch_ray_noalloc(b, rays, n, prim, t, uv, opts, (hipStream_t)stream);
 }
 
 int gpurt_trace_any(gpurt_bvh b, const gpurt_ray* rays, size_t n, uint8_t* occluded,

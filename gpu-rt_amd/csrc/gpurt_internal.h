@@ -255,6 +255,15 @@ int launch_cpq(gpurt_bvh_s* b, const float* q, size_t n, int32_t* prim, float* p
 int launch_ray(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
                float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, hipStream_t s);
 
+// gpurt_trace_closest_noalloc (query.hip): the closest-hit launch with its traversal
+// scratch in a fixed per-device array of kNoAllocScratchBytes, set up once per device by
+// noalloc_scratch_init (with that device current; called by gpurt_ctx_create)
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+int noalloc_scratch_init(int device);
+int launch_ray_noalloc(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t, float* uv,
+                       const gpurt_query_opts* opts, hipStream_t s);
+
 // backward (backward.hip)
 int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim, const float* g,
                    float* g_q, float* g_verts, hipStream_t s);

@@ -5,6 +5,7 @@
 // (lane-refill) traversal of traverse.h over the queries with a grid sized to the waves
 // that fit on the chip at once: lane g takes queries g, g + total, g + 2 total, ...
 #include <algorithm>
+#include <mutex>
 
 #include "traverse.h"
 
@@ -204,8 +205,58 @@ void free_traversal_scratch(TravScratch& t, hipStream_t s) {
     t.mem = nullptr;
 }
 
+// ------------------------------------------------------------------------------------
+// fixed scratch of gpurt_trace_closest_noalloc
+// ------------------------------------------------------------------------------------
+
+// The queue counters and stack spill area of the no-allocation entry point.  Each device
+// holds its own copy from the first context created on it until the process ends; its
+// address is taken once per device by noalloc_scratch_init (hipGetSymbolAddress), and
+// only that address is handed to kernels (the array's name is not a device address in
+// host code).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return set_error(GPURT_E_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GPURT_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    }
+    g_scratch_ptr[device] = ptr;
+    return GPURT_OK;
+}
+
+static void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// Fits a plan into `bytes` of scratch: the spill area grows with the resident lanes, so
+// a deep tree is traversed with fewer resident blocks (whole blocks, at least one).
+static int fit_plan_to_scratch(TravPlan& p, size_t bytes) {
+    if (traversal_scratch_bytes(p) <= bytes) return GPURT_OK;
+    const size_t per_block = (size_t)p.spill_slots * p.BS * sizeof(uint2);
+    const size_t room = bytes - kCounterBytes;
+    const size_t blocks = room / per_block;
+    if (blocks == 0)
+        return set_error(GPURT_E_INVALID, "tree too deep for the fixed traversal scratch of "
+                                          "gpurt_trace_closest_noalloc (64 MiB)");
+    p.grid = (unsigned)std::min<size_t>(p.grid, blocks);
+    p.total = p.grid * p.BS;
+    return GPURT_OK;
+}
+
+// fixed_scratch: memory of `fixed_bytes` the caller holds for the queue counters and
+// stack spill (the grid is fitted into it); nullptr: a stream-ordered allocation per call
 static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpurt_query_opts* opts,
-                     hipStream_t s) {
+                     hipStream_t s, void* fixed_scratch = nullptr, size_t fixed_bytes = 0) {
     if (n == 0) return GPURT_OK;
     const bool q8 = kind == QKind::Cpq ? cpq_use_q8(b->format) : rays_use_q8(b->format);
     // worst case stack: (W-1) entries per level above the last plus W for the last
@@ -213,7 +264,14 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
     int rc = plan_traversal(b->ctx, b->width, q8, kind, (b->width - 1) * b->depth + 1, opts, n, p);
     if (rc) return rc;
     TravScratch t;
-    if ((rc = alloc_traversal_scratch(p, s, t))) return rc;
+    if (fixed_scratch) {
+        if ((rc = fit_plan_to_scratch(p, fixed_bytes))) return rc;
+        carve_traversal_scratch(p, fixed_scratch, t);
+        clear_traversal_counters(t, s);
+        t.mem = nullptr;  // not freed below
+    } else if ((rc = alloc_traversal_scratch(p, s, t))) {
+        return rc;
+    }
     a.tris = b->tris;
     a.root = 0;
     a.n = (uint32_t)n;
@@ -282,6 +340,18 @@ int launch_ray(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, f
     a.uv = uv;
     a.occ = occ;
     return run_query(b, any ? QKind::Any : QKind::Closest, a, n, opts, s);
+}
+
+int launch_ray_noalloc(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t, float* uv,
+                       const gpurt_query_opts* opts, hipStream_t s) {
+    void* mem = noalloc_scratch(b->ctx->device);
+    if (!mem) return set_error(GPURT_E_DEVICE, "fixed traversal scratch not set up on this device");
+    TravArgs a{};
+    a.rays = reinterpret_cast<const float4*>(rays);
+    a.prim = prim;
+    a.t = t;
+    a.uv = uv;
+    return run_query(b, QKind::Closest, a, n, opts, s, mem, kNoAllocScratchBytes);
 }
 
 }  // namespace gpurt_impl

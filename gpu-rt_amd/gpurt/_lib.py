@@ -28,7 +28,8 @@ _ERR_NAMES = {
 EXPORTS = (
     "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_bvh_build", "gpurt_bvh_refit",
     "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_closest_point", "gpurt_trace_closest",
-    "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd", "gpurt_batch_create",
+    "gpurt_trace_closest_noalloc", "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd",
+    "gpurt_batch_create",
     "gpurt_batch_run", "gpurt_batch_set_grads", "gpurt_batch_run_backward", "gpurt_batch_destroy",
     "gpurt_tlas_build", "gpurt_tlas_destroy", "gpurt_tlas_trace_closest", "gpurt_tlas_trace_any",
     "gpurt_scene_create", "gpurt_scene_destroy", "gpurt_render", "gpurt_scene_light_pdf",
@@ -123,6 +124,7 @@ def load():
         "gpurt_bvh_get_info": (I, [P, ctypes.POINTER(BvhInfo)]),
         "gpurt_closest_point": (I, [P, P, S, P, P, P, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_trace_closest": (I, [P, P, S, P, P, P, ctypes.POINTER(QueryOpts), P]),
+        "gpurt_trace_closest_noalloc": (I, [P, P, S, P, P, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_trace_any": (I, [P, P, S, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_closest_point_bwd": (I, [P, P, S, P, P, P, P, P, P, P, P]),
         "gpurt_trace_bwd": (I, [P, P, S, P, P, P, P, P, P, P, P]),

@@ -159,6 +159,21 @@ GPURT_API int gpurt_closest_point(gpurt_bvh bvh, const float* q, size_t n, int32
 GPURT_API int gpurt_trace_closest(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, int32_t* prim,
                         float* t, float* uv, const gpurt_query_opts* opts, void* stream);
 
+/* gpurt_trace_closest without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same arguments, results
+ * identical bit for bit to gpurt_trace_closest.  The traversal's scratch (work-queue
+ * counters and the stack spill area) is a fixed 64 MiB array per device, held from the
+ * first gpurt_ctx_create on that device until the process ends; the launch is fitted into
+ * it (a tree whose worst-case stack spills more than it holds runs with fewer resident
+ * waves), and GPURT_E_INVALID is returned when not even one block's spill fits (a tree
+ * thousands of levels deep) or n exceeds 2^32 - 1025.  The counters are cleared by a
+ * kernel on `stream` (no memset node).  Every call on one device shares that array:
+ * calls that may overlap on one device (different streams, or host threads) race and
+ * must be serialized by the caller.  opts->stats must be null. */
+GPURT_API int gpurt_trace_closest_noalloc(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, int32_t* prim,
+                                          float* t, float* uv, const gpurt_query_opts* opts, void* stream);
+
 /* Any hit (occlusion) for n rays: occluded[i] = 1 iff some triangle is hit with t in
  * [tmin, tmax].  Replaces visibility()'s traceRayEXT with TerminateOnFirstHit |
  * Opaque | SkipClosestHitShader and the payload.hit preset/clear

@@ -69,6 +69,7 @@ def test_null_arguments_return_invalid_without_device_work():
     assert lib.gpurt_closest_point_bwd(None, None, 1, None, None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_trace_bwd(None, None, 1, None, None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_bvh_get_info(None, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_trace_closest_noalloc(None, None, 1, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_ctx_create(0, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_scene_create(None, None, 1, None, ctypes.byref(out)) == _lib.GPURT_E_INVALID
     assert lib.gpurt_render(None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
