@@ -7,15 +7,22 @@ Headline workload (one "step" = one pass of the hot path over one batch):
   per second over both passes, inputs resident in HBM, BVH built before timing.
 
 The same JSON line carries:
-  roofline      achieved algorithmic GB/s of the closest-hit traversal kernel (bytes per
-                ray = 32 in + 16 out + 128 per wide node visited + 36 per triangle tested,
-                visits counted by the kernel's counting variant) over its mean duration
-                (HIP events on the launch stream), against the 8 TB/s HBM3E peak.
+  roofline      achieved algorithmic GB/s of the closest-hit traversal kernel over its
+                mean duration (HIP events on the launch stream), against the 8 TB/s HBM3E
+                peak.  Algorithmic bytes, the same formula for every traversal leg:
+                  query I/O + node_bytes per wide node visited + 36 per triangle tested
+                (node_bytes = the traversed node encoding: 64 for the 8-bit 4-wide nodes
+                rays use, 128 for the FP32 4-wide nodes closest points use; 36 = three
+                float3 vertices), visits counted by the kernel's counting variant on the
+                same BVH and inputs.  I/O: closest hit 32 in + 16 out, any hit 32 + 1,
+                closest point 12 in + 20 out + 12 barycentrics.  Backward: 144 B per
+                closest-point query, 180 B per ray.
   cpu_baseline  the oracle's CPU LBVH (C, pthreads on every host core) on a bounded
                 sample of the same rays.
-  extra         the other workloads: closest-point queries/s (5,120 / 5,242,880 tris),
-                forward and forward+backward on the differentiable 1.3M-tri workload,
-                build times.
+  extra         the other workloads, each with the same roofline fields: closest-point
+                queries/s (5,120 and 5,242,880 triangles), any hit, forward and backward
+                on the differentiable 1.3M-triangle workload, the 256-mesh batch against
+                the plain per-mesh path (default and autotuned parameters), build times.
 """
 import argparse
 import json
@@ -32,6 +39,15 @@ sys.path.insert(0, ROOT)
 METRIC = "closest-pt queries/s + rays/s, 1M-tri, fwd & fwd+bwd, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 TRI_BYTES = 36  # three float3 vertices per triangle tested
+CPQ_BWD_BYTES = 144  # g, prim, bary, pt, dist, q, index gather, g_q out, 3 x 2 x 12 vertex RMW
+RAY_BWD_BYTES = 180  # the same for a ray, plus 36 B of vertex gathers for the normal
+
+
+def roof(byts, ms, kernel):
+    """Roofline entry: algorithmic bytes per launch over the launch's mean time."""
+    gbs = byts / (ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "bytes_per_launch": int(byts), "kernel_ms": ms,
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
 
 def log(*a):
@@ -103,6 +119,12 @@ def fp64_cpq(O, V, F, Q, prim, dist, every):
     return {"sample": int(idx.size), "id_agreement": float(same.mean()),
             "dist_max_rel_err": float(rel.max()) if rel.size else 0.0,
             "dist_p999_rel_err": float(np.quantile(rel, 0.999)) if rel.size else 0.0}
+
+
+def roof_kernel(args, info, kind):
+    q8 = info["node_format"] in (2, 3) if kind != "cpq" else info["node_format"] == 2
+    name = {"closest": "k_ray<W=%d,%s,closest>", "any": "k_ray<W=%d,%s,any>", "cpq": "k_cpq<W=%d,%s>+k_cpq_finish"}[kind]
+    return name % (args.width, "q8" if q8 else "f32")
 
 
 def ray_bench(args, torch, gpurt, W, O, dev, ctx):
@@ -197,23 +219,11 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     copy_ms = timed(lambda: dst.copy_(src), 10, 2)
     copy_gbs = 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9
     del src, dst
-    roof = {
-        "kernel": "k_ray<W=%d,%s,closest>" % (args.width, "q8" if info["node_format"] in (2, 3) else "f32"),
-        "bound": "hbm",
-        "achieved": bytes_closest / (ms_closest * 1e-3) / 1e9,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "traffic": None,
-        "bytes_per_launch": bytes_closest,
-        "nodes_per_ray": nodes / n,
-        "tris_per_ray": tris / n,
-        "stack_spills_per_ray": spills / n,
-        "kernel_ms": ms_closest,
-    }
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["copy_achievable_GBps"] = copy_gbs
-    roof["frac_of_copy"] = roof["achieved"] / copy_gbs
-    roof["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
+    rl = roof(bytes_closest, ms_closest, roof_kernel(args, info, "closest"))
+    rl.update({"nodes_per_ray": nodes / n, "tris_per_ray": tris / n, "stack_spills_per_ray": spills / n})
+    rl["copy_achievable_GBps"] = copy_gbs
+    rl["frac_of_copy"] = rl["achieved"] / copy_gbs
+    rl["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
     res = {
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
@@ -223,7 +233,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                    "leaf_size": args.leaf, "stack_depth": args.stack, "treelet_size": args.treelet,
                    "node_format": info["node_format"], "waves_per_cu": args.waves,
                    "leaf_trigger": args.leaf_trigger, "refill_trigger": args.refill_trigger},
-        "roofline": roof,
+        "roofline": rl,
         "cpu_baseline": {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
                          "sample": f"{passes} full passes of the {n} closest-hit + {n} any-hit "
                                    f"rays of rays_scene on the oracle's C LBVH (leaf 4, "
@@ -238,13 +248,15 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
                        "hit_rate": hit_rate, "occluded_rate": occ_rate,
                        "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
+                       "any_roofline": roof(n * (32 + 1) + nodes_a * info["node_bytes"] + tris_a * TRI_BYTES,
+                                            ms_any, roof_kernel(args, info, "any")),
                        "bvh": info, "parity_all_rays_vs_cpu_lbvh": parity, "fp64_check": fp64},
     }
     del bvh
     return res
 
 
-def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0):
+def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0, parity_every=0):
     V, F, Q = w["verts"], w["tris"], w["queries"]
     tv = torch.from_numpy(V).to(dev)
     tf = torch.from_numpy(F).to(dev)
@@ -262,13 +274,27 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0):
     bvh.closest_point(q, out=out, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, stats=st)
     torch.cuda.synchronize()
     nodes, tris, _, _ = st.cpu().tolist()
-    byts = n * (12 + 20 + 12) + nodes * bvh.info()["cpq_node_bytes"] + tris * TRI_BYTES
+    info = bvh.info()
+    byts = n * (12 + 20 + 12) + nodes * info["cpq_node_bytes"] + tris * TRI_BYTES
+    prim = out[0].cpu().numpy()
     del bvh
-    fp64 = fp64_cpq(O, V, F, Q, out[0].cpu().numpy(), out[2].cpu().numpy(), fp64_every) if fp64_every else None
+    parity = None
+    if parity_every:
+        # every parity_every-th query against the oracle's CPU LBVH (equal to brute force)
+        idx = np.arange(0, n, parity_every)
+        cb = O.CpuBvh(V, F, 4)
+        ep, ept, ed, eb = cb.closest_point(Q[idx])
+        parity = {"sample": int(idx.size),
+                  "bitexact_vs_cpu_lbvh": bool(np.array_equal(prim[idx], ep) and
+                                               np.array_equal(out[2].cpu().numpy()[idx], ed) and
+                                               np.array_equal(out[1].cpu().numpy()[idx], ept) and
+                                               np.array_equal(out[3].cpu().numpy()[idx], eb))}
+    fp64 = fp64_cpq(O, V, F, Q, prim, out[2].cpu().numpy(), fp64_every) if fp64_every else None
     return {"workload": name, "n_tris": int(F.shape[0]), "n_queries": n,
             "queries_per_s": n / (ms * 1e-3), "ms": ms, "gpu_build_ms": build_ms,
             "nodes_per_query": nodes / n, "tris_per_query": tris / n,
-            "algorithmic_GBps": byts / (ms * 1e-3) / 1e9, "fp64_check": fp64}
+            "roofline": roof(byts, ms, roof_kernel(args, info, "cpq")),
+            "parity": parity, "fp64_check": fp64}
 
 
 def diff_bench(args, torch, gpurt, W, dev, ctx):
@@ -301,10 +327,24 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
 
     ms_f = timed(fwd, args.steps, 2)
     ms_fb = timed(fwd_bwd, args.steps, 2)
+    # the backward launches alone, on saved forward outputs
+    prim, pt, dist, bary = bvh.closest_point(q0)
+    rays = gpurt.make_rays(o0, d0)
+    rp, rt, ruv = bvh.trace_closest(rays)
+    g1 = torch.ones(nq, device=dev)
+    g2 = torch.ones(nr, device=dev)
+    gq = torch.empty_like(q0)
+    go = torch.empty((nr, 3), device=dev)
+    gv = torch.zeros((tv.shape[0], 3), device=dev)
+    ms_cb = timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2)
+    ms_rb = timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)
     del bvh
     return {"workload": "diff", "n_tris": int(F.shape[0]), "n_queries": nq, "n_rays": nr,
             "fwd_units_per_s": (nq + nr) / (ms_f * 1e-3), "fwd_ms": ms_f,
-            "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb}
+            "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb,
+            "fwd_bwd_over_fwd": ms_fb / ms_f,
+            "cpq_bwd_roofline": roof(nq * CPQ_BWD_BYTES, ms_cb, "k_cpq_bwd"),
+            "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "k_ray_bwd")}
 
 
 def batched_bench(args, torch, gpurt, W, dev, ctx):
@@ -339,6 +379,25 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
     ms_p = timed(plain, 2, 1)
     torch.cuda.synchronize()
+    # the autotuner over the space the batched workload names (node width {2,4,8}, stack
+    # depth {16,32,64}, waves per CU), untimed; then the tuned batch is timed like the
+    # default one
+    tuned = None
+    if not args.no_tune:
+        from gpurt import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune_batch(ctx, meshes, use_cache=False, reps=3,
+                                    space={"node_width": (2, 4, 8), "stack_depth": (16, 32, 64),
+                                           "waves_per_cu": (8, 16, 24), "treelet_size": (args.batch_treelet,),
+                                           "node_format": (args.format,)})
+        tune_s = time.perf_counter() - t0
+        tbatch = gpurt.Batch(ctx, meshes, leaf_size=args.leaf, **tuned["build"], **tuned["query"])
+        ms_t = timed(tbatch.run, max(3, args.steps // 4), 1)
+        tres = tbatch.run()
+        torch.cuda.synchronize()
+        parity_tuned = all(torch.equal(a["q_prim"], b["q_prim"]) and torch.equal(a["r_t"], b["r_t"])
+                           for a, b in zip(tres, res))
+        tbatch.close()
     parity = all(torch.equal(plain_out[i][0], r["q_prim"]) and torch.equal(plain_out[i][1], r["q_dist"])
                  and torch.equal(plain_out[i][2], r["r_prim"]) and torch.equal(plain_out[i][3], r["r_t"])
                  for i, r in enumerate(res))
@@ -387,6 +446,11 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "fwd_bwd_batched_units_per_s": (nq + nr) / (ms_bfb * 1e-3), "fwd_bwd_batched_ms": ms_bfb,
             "fwd_bwd_plain_units_per_s": (nq + nr) / (ms_pfb * 1e-3), "fwd_bwd_plain_ms": ms_pfb,
             "fwd_bwd_batched_over_plain": ms_pfb / ms_bfb, "batch_create_s": create_s,
+            "autotuned": None if tuned is None else {
+                "build": tuned["build"], "query": tuned["query"], "candidates": tuned["candidates"],
+                "tuner_s": tune_s, "tuner_default_ms": tuned["default_ms"], "tuner_best_ms": tuned["objective_ms"],
+                "batched_ms": ms_t, "batched_units_per_s": (nq + nr) / (ms_t * 1e-3),
+                "batched_over_plain": ms_p / ms_t, "parity_vs_default": bool(parity_tuned)},
             "parity_batched_vs_plain": bool(parity), "parity_bwd_batched_vs_plain": bool(parity_bwd),
             "treelet_size": args.batch_treelet,
             "note": "each step rebuilds all 256 BVHs and answers every query; fwd+bwd adds the "
@@ -483,12 +547,24 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     plain streaming kernel that reads 256 MiB and writes 256 MiB, whose
     FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
     kind = args.profile_workload
-    w = W.rays_scene() if kind == "rays_scene" else getattr(W, kind)()
+    w = W.rays_scene() if kind == "rays_scene" else W.diff() if kind == "diff_bwd" else getattr(W, kind)()
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
                     node_format=args.format, leaf_size=args.leaf)
-    if kind == "rays_scene":
+    if kind == "diff_bwd":
+        q = torch.from_numpy(w["queries"]).to(dev)
+        rays = gpurt.make_rays(torch.from_numpy(w["ray_o"]).to(dev), torch.from_numpy(w["ray_d"]).to(dev))
+        n = q.shape[0] + rays.shape[0]
+        prim, pt, dist, bary = bvh.closest_point(q)
+        rp, rt, ruv = bvh.trace_closest(rays)
+        g1, g2 = torch.ones(q.shape[0], device=dev), torch.ones(rays.shape[0], device=dev)
+        gq, go = torch.empty_like(q), torch.empty((rays.shape[0], 3), device=dev)
+        gv = torch.zeros((tv.shape[0], 3), device=dev)
+        for _ in range(args.profile_reps):
+            bvh.closest_point_bwd(q, prim, bary, pt, dist, g1, gq, gv)
+            bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv)
+    elif kind == "rays_scene":
         rays = torch.from_numpy(w["rays"]).to(dev)
         n = rays.shape[0]
         out = (torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, device=dev),
@@ -543,7 +619,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: repeat full passes until this much CPU time")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--large", action="store_true", help="also time the 5.2M-tri CPQ workload")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 5.2M-triangle clustered closest-point workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched-vs-plain leg (for profiling it)")
@@ -551,8 +628,13 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
-    ap.add_argument("--profile-workload", default="rays_scene", choices=["rays_scene", "cpq_small", "cpq_large"],
-                    help="kernel of the --profile run: closest-hit rays or closest-point queries")
+    ap.add_argument("--profile-workload", default="rays_scene",
+                    choices=["rays_scene", "cpq_small", "cpq_large", "diff_bwd"],
+                    help="kernel of the --profile run: closest-hit rays, closest-point queries, or the "
+                         "backward passes of the differentiable workload")
+    ap.add_argument("--no-tune", action="store_true", help="batched leg: skip the autotuner sweep")
+    ap.add_argument("--only", choices=["diff", "cpq_large", "batched", "tracer"],
+                    help="print only this leg (for profiling it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path; "
@@ -570,10 +652,17 @@ def main():
     if args.profile:
         profile_mode(args, torch, gpurt, W, dev, ctx)
         return
-    if args.only_batched:
+    if args.only_batched or args.only == "batched":
         print(json.dumps(batched_bench(args, torch, gpurt, W, dev, ctx)), flush=True)
         return
-    if args.only_tracer:
+    if args.only == "diff":
+        print(json.dumps(diff_bench(args, torch, gpurt, W, dev, ctx)), flush=True)
+        return
+    if args.only == "cpq_large":
+        print(json.dumps(cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx, O, fp64_every=1024,
+                                   parity_every=64)), flush=True)
+        return
+    if args.only_tracer or args.only == "tracer":
         print(json.dumps(tracer_bench(args, torch, gpurt, W, O, dev, ctx)), flush=True)
         return
     r = ray_bench(args, torch, gpurt, W, O, dev, ctx)
@@ -590,9 +679,10 @@ def main():
         if not args.no_batched:
             log("[bench] batched vs plain")
             extra["batched"] = batched_bench(args, torch, gpurt, W, dev, ctx)
-        if args.large:
+        if not args.no_large:
             log("[bench] cpq_large")
-            extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx, O, 1024)
+            extra["cpq_large"] = cpq_bench("cpq_large", W.cpq_large(), args, torch, gpurt, dev, ctx, O,
+                                           fp64_every=1024, parity_every=64)
     line = {
         "metric": METRIC,
         "value": r["value"],

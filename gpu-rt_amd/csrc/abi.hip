@@ -43,6 +43,73 @@ __global__ void k_compact_verts(uint32_t n, const float* __restrict__ v, uint64_
     out[3 * (size_t)i + 2] = p[2];
 }
 
+void note_use(gpurt_bvh_s* b, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(b->use_mu);
+    // work recorded into a graph runs when the graph is launched: no event can mark it
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        b->uses_lost = true;
+        return;
+    }
+    for (auto& u : b->uses)
+        if (u.first == s) {
+            if (hipEventRecord(u.second, s) != hipSuccess) {
+                (void)hipGetLastError();
+                b->uses_lost = true;
+            }
+            return;
+        }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess || hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        if (e) (void)hipEventDestroy(e);
+        b->uses_lost = true;
+        return;
+    }
+    b->uses.emplace_back(s, e);
+}
+
+// note_use without creating an event (the no-allocation entry point): a stream not seen
+// before makes destroy wait for the whole device instead
+static void note_use_existing(gpurt_bvh_s* b, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(b->use_mu);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        b->uses_lost = true;
+        return;
+    }
+    for (auto& u : b->uses)
+        if (u.first == s) {
+            if (hipEventRecord(u.second, s) != hipSuccess) {
+                (void)hipGetLastError();
+                b->uses_lost = true;
+            }
+            return;
+        }
+    b->uses_lost = true;
+}
+
+void wait_uses(gpurt_bvh_s* b, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(b->use_mu);
+    for (auto& u : b->uses)
+        if (hipStreamWaitEvent(s, u.second, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            b->uses_lost = true;
+        }
+}
+
+// frees a BVH's buffers in order on stream s (all come from the stream-ordered pool)
+static void free_bvh_buffers(gpurt_bvh_s* b, hipStream_t s) {
+    for (void* p : {b->nodes, b->qnodes, (void*)b->tris, (void*)b->tri_idx, (void*)b->verts, (void*)b->leaf_order})
+        if (p) (void)hipFreeAsync(p, s);
+    b->nodes = b->qnodes = nullptr;
+    b->tris = nullptr;
+    b->tri_idx = b->leaf_order = nullptr;
+    b->verts = nullptr;
+}
+
 int compact_verts(const float* verts, size_t stride_floats, size_t n, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_compact_verts, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, verts,
                        (uint64_t)stride_floats, out);
@@ -105,7 +172,9 @@ int gpurt_ctx_create(int device, gpurt_ctx* out) {
         delete c;
         return set_error(GPURT_E_OOM, "context allocation failed");
     }
-    if (hipMemset(c->err, 0, 256) != hipSuccess) {
+    if (hipMemset(c->err, 0, 256) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->release, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
         (void)hipFree(c->err);
         delete c;
         return set_error(GPURT_E_DEVICE, "context init failed");
@@ -115,6 +184,7 @@ int gpurt_ctx_create(int device, gpurt_ctx* out) {
     {
         const int rc = noalloc_scratch_init(device);
         if (rc) {
+            (void)hipStreamDestroy(c->release);
             (void)hipFree(c->err);
             delete c;
             return rc;
@@ -135,8 +205,36 @@ void gpurt_ctx_destroy(gpurt_ctx ctx) {
     if (!ctx) return;
     DeviceGuard g(ctx->device);
     (void)hipDeviceSynchronize();
+    {
+        // handles still open: their device memory goes with the context
+        std::lock_guard<std::mutex> lk(ctx->live_mu);
+        for (gpurt_tlas_s* t : ctx->tlases) tlas_orphan(t);
+        ctx->tlases.clear();
+        for (gpurt_bvh_s* b : ctx->bvhs) {
+            free_bvh_buffers(b, ctx->release);
+            for (auto& u : b->uses) (void)hipEventDestroy(u.second);
+            b->uses.clear();
+            b->ctx = nullptr;
+        }
+        ctx->bvhs.clear();
+    }
+    if (ctx->release) (void)hipStreamSynchronize(ctx->release);
+    (void)hipGetLastError();
+    if (ctx->release) (void)hipStreamDestroy(ctx->release);
     if (ctx->err) (void)hipFree(ctx->err);
     delete ctx;
+}
+
+int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear) {
+    if (!ctx || !flags) return set_error(GPURT_E_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    GPURT_HIP(hipDeviceSynchronize());
+    uint32_t f = 0;
+    GPURT_HIP(hipMemcpy(&f, ctx->err, sizeof(f), hipMemcpyDeviceToHost));
+    if (clear) GPURT_HIP(hipMemset(ctx->err, 0, sizeof(f)));
+    *flags = f;
+    return GPURT_OK;
 }
 
 static int check_mesh(const float* verts, size_t vstride_bytes, size_t n_verts,
@@ -174,32 +272,29 @@ int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, siz
     auto* b = new gpurt_bvh_s();
     b->ctx = ctx;
     b->n_verts = (int64_t)n_verts;
-    rc = build_segments(ctx, &sd, 1, bp.width, bp.leaf_size, bp.treelet_size, bp.treelet_rounds, bp.format, s, b);
-    if (rc) {
+    // on failure the buffers made so far are freed in order on the build stream
+    auto fail = [&](int code) {
+        free_bvh_buffers(b, s);
         delete b;
-        return rc;
-    }
+        return code;
+    };
+    rc = build_segments(ctx, &sd, 1, bp.width, bp.leaf_size, bp.treelet_size, bp.treelet_rounds, bp.format, s, b);
+    if (rc) return fail(rc);
     // keep the index buffer and compact positions for the backward passes and refit
-    if (hipMalloc((void**)&b->tri_idx, sizeof(uint32_t) * 3 * n_tris) != hipSuccess ||
-        hipMalloc((void**)&b->verts, sizeof(float) * 3 * n_verts) != hipSuccess) {
+    if (hipMallocAsync((void**)&b->tri_idx, sizeof(uint32_t) * 3 * n_tris, s) != hipSuccess ||
+        hipMallocAsync((void**)&b->verts, sizeof(float) * 3 * n_verts, s) != hipSuccess) {
         (void)hipGetLastError();
-        gpurt_bvh_destroy(b);
-        return set_error(GPURT_E_OOM, "hipMalloc of mesh copy failed");
+        return fail(set_error(GPURT_E_OOM, "allocation of the mesh copy failed"));
     }
     if (hipMemcpyAsync(b->tri_idx, tri_idx, sizeof(uint32_t) * 3 * n_tris, hipMemcpyDeviceToDevice, s) !=
-        hipSuccess) {
-        rc = hip_error(hipGetLastError(), "copying indices");
-        gpurt_bvh_destroy(b);
-        return rc;
-    }
-    if ((rc = compact_verts(verts, vstride_bytes / 4, n_verts, b->verts, s)) != GPURT_OK) {
-        gpurt_bvh_destroy(b);
-        return rc;
-    }
-    if (hipStreamSynchronize(s) != hipSuccess) {
-        rc = hip_error(hipGetLastError(), "build");
-        gpurt_bvh_destroy(b);
-        return rc;
+        hipSuccess)
+        return fail(hip_error(hipGetLastError(), "copying indices"));
+    if ((rc = compact_verts(verts, vstride_bytes / 4, n_verts, b->verts, s)) != GPURT_OK) return fail(rc);
+    // the handle is complete when this returns, so queries on any stream may follow
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(hip_error(hipGetLastError(), "build"));
+    {
+        std::lock_guard<std::mutex> lk(ctx->live_mu);
+        ctx->bvhs.insert(b);
     }
     *out = b;
     return GPURT_OK;
@@ -211,6 +306,8 @@ int gpurt_bvh_refit(gpurt_bvh bvh, const float* verts, size_t vstride_bytes, voi
         return set_error(GPURT_E_INVALID, "vstride_bytes must be a multiple of 4 and >= 12");
     DeviceGuard g(bvh->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    // a refit rewrites the tree: it waits for earlier queries on other streams
+    wait_uses(bvh, (hipStream_t)stream);
     int rc = launch_refit(bvh, verts, vstride_bytes / 4, (hipStream_t)stream);
     if (rc) return rc;
     GPURT_HIP(hipStreamSynchronize((hipStream_t)stream));
@@ -219,14 +316,22 @@ int gpurt_bvh_refit(gpurt_bvh bvh, const float* verts, size_t vstride_bytes, voi
 
 void gpurt_bvh_destroy(gpurt_bvh b) {
     if (!b) return;
+    if (!b->ctx) {  // its context was destroyed first and released its device memory
+        delete b;
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(b->ctx->live_mu);
+        b->ctx->bvhs.erase(b);
+    }
     DeviceGuard g(b->ctx->device);
-    (void)hipDeviceSynchronize();
-    if (b->nodes) (void)hipFree(b->nodes);
-    if (b->qnodes) (void)hipFree(b->qnodes);
-    if (b->tris) (void)hipFree(b->tris);
-    if (b->tri_idx) (void)hipFree(b->tri_idx);
-    if (b->verts) (void)hipFree(b->verts);
-    if (b->leaf_order) (void)hipFree(b->leaf_order);
+    hipStream_t rs = b->ctx->release;
+    // uses that no event marks (graph capture, a failed record): wait for the device
+    if (b->uses_lost) (void)hipDeviceSynchronize();
+    wait_uses(b, rs);
+    free_bvh_buffers(b, rs);
+    for (auto& u : b->uses) (void)hipEventDestroy(u.second);
+    (void)hipGetLastError();
     delete b;
 }
 
@@ -244,6 +349,23 @@ int gpurt_bvh_get_info(gpurt_bvh b, gpurt_bvh_info* out) {
     out->leaf_size = b->leaf_size;
     out->sah_cost = b->sah_cost;
     out->node_format = b->format;
+    return GPURT_OK;
+}
+
+int gpurt_bvh_export(gpurt_bvh b, const gpurt_bvh_arrays* out, void* stream) {
+    if (!b || !out) return set_error(GPURT_E_INVALID, "null argument");
+    if (out->qnodes && !b->qnodes) return set_error(GPURT_E_INVALID, "this BVH keeps no 8-bit nodes (node format 1)");
+    DeviceGuard g(b->ctx->device);
+    if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)b->n_nodes, nt = (size_t)b->n_tris;
+    if (out->nodes) GPURT_HIP(hipMemcpyAsync(out->nodes, b->nodes, node_bytes(b->width) * nn, hipMemcpyDeviceToDevice, s));
+    if (out->qnodes)
+        GPURT_HIP(hipMemcpyAsync(out->qnodes, b->qnodes, qnode_bytes(b->width) * nn, hipMemcpyDeviceToDevice, s));
+    if (out->tris) GPURT_HIP(hipMemcpyAsync(out->tris, b->tris, sizeof(float4) * 3 * nt, hipMemcpyDeviceToDevice, s));
+    if (out->leaf_order)
+        GPURT_HIP(hipMemcpyAsync(out->leaf_order, b->leaf_order, sizeof(uint32_t) * nt, hipMemcpyDeviceToDevice, s));
+    note_use(b, s);
     return GPURT_OK;
 }
 
@@ -270,7 +392,9 @@ int gpurt_closest_point(gpurt_bvh b, const float* q, size_t n, int32_t* prim, fl
     if (rc) return rc;
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_cpq(b, q, n, prim, pt, dist, bary, opts, (hipStream_t)stream);
+    const int rc2 = launch_cpq(b, q, n, prim, pt, dist, bary, opts, (hipStream_t)stream);
+    note_use(b, (hipStream_t)stream);
+    return rc2;
 }
 
 int gpurt_trace_closest(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
@@ -282,7 +406,9 @@ int gpurt_trace_closest(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* p
     if (rc) return rc;
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_ray(b, rays, n, prim, t, uv, nullptr, false, opts, (hipStream_t)stream);
+    const int rc2 = launch_ray(b, rays, n, prim, t, uv, nullptr, false, opts, (hipStream_t)stream);
+    note_use(b, (hipStream_t)stream);
+    return rc2;
 }
 
 int gpurt_trace_closest_noalloc(gpurt_bvh b, const gpurt_ray* rays, size_t n, int32_t* prim, float* t,
@@ -295,7 +421,9 @@ int gpurt_trace_closest_noalloc(gpurt_bvh b, const gpurt_ray* rays, size_t n, in
     if (opts && opts->stats) return set_error(GPURT_E_UNSUPPORTED, "stats are not counted by the no-allocation variant");
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_ray_noalloc(b, rays, n, prim, t, uv, opts, (hipStream_t)stream);
+    const int rc2 = launch_ray_noalloc(b, rays, n, prim, t, uv, opts, (hipStream_t)stream);
+    note_use_existing(b, (hipStream_t)stream);
+    return rc2;
 }
 
 int gpurt_trace_any(gpurt_bvh b, const gpurt_ray* rays, size_t n, uint8_t* occluded,
@@ -307,7 +435,9 @@ int gpurt_trace_any(gpurt_bvh b, const gpurt_ray* rays, size_t n, uint8_t* occlu
     if (rc) return rc;
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_ray(b, rays, n, nullptr, nullptr, nullptr, occluded, true, opts, (hipStream_t)stream);
+    const int rc2 = launch_ray(b, rays, n, nullptr, nullptr, nullptr, occluded, true, opts, (hipStream_t)stream);
+    note_use(b, (hipStream_t)stream);
+    return rc2;
 }
 
 int gpurt_closest_point_bwd(gpurt_bvh b, const float* q, size_t n, const int32_t* prim,
@@ -320,7 +450,9 @@ int gpurt_closest_point_bwd(gpurt_bvh b, const float* q, size_t n, const int32_t
     if (n > 0xFFFFFFFFull) return set_error(GPURT_E_INVALID, "too many queries in one call");
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_cpq_bwd(b, q, n, prim, g_dist, g_q, g_verts, (hipStream_t)stream);
+    const int rc2 = launch_cpq_bwd(b, q, n, prim, g_dist, g_q, g_verts, (hipStream_t)stream);
+    note_use(b, (hipStream_t)stream);
+    return rc2;
 }
 
 int gpurt_trace_bwd(gpurt_bvh b, const gpurt_ray* rays, size_t n, const int32_t* prim,
@@ -333,7 +465,9 @@ int gpurt_trace_bwd(gpurt_bvh b, const gpurt_ray* rays, size_t n, const int32_t*
     if (n > 0xFFFFFFFFull) return set_error(GPURT_E_INVALID, "too many rays in one call");
     DeviceGuard g(b->ctx->device);
     if (!g.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
-    return launch_ray_bwd(b, rays, n, prim, t, uv, g_t, g_origin, g_dir, g_verts, (hipStream_t)stream);
+    const int rc2 = launch_ray_bwd(b, rays, n, prim, t, uv, g_t, g_origin, g_dir, g_verts, (hipStream_t)stream);
+    note_use(b, (hipStream_t)stream);
+    return rc2;
 }
 
 }  // extern "C"

@@ -8,8 +8,10 @@
 //   hit t     t = n.(v0 - o)/(n.d), n = (v1-v0) x (v2-v0), b = (1-u-v, u, v) fixed:
 //             dt/do = -n/(n.d),  dt/dd = -t n/(n.d),  dt/dv_j = b_j n/(n.d)
 //
-// Vertex positions are read with a stride of `vs` floats; vertex gradients are n x 3
-// and accumulated with hardware FP32 atomic adds.
+// Vertex positions are read with a stride of `vs` floats; vertex gradients are n x 3.
+// The terms of one query or ray are formed here; the plain (backward.hip) and batch
+// (batch.hip) kernels sort them by vertex and add each vertex's terms in a fixed order,
+// without atomics, so vertex gradients are the same bits on every run.
 #pragma once
 
 #include "gpurt_internal.h"
@@ -21,17 +23,27 @@ __device__ __forceinline__ V3 vert_at(const float* __restrict__ verts, uint64_t 
     return v3(verts[vs * v], verts[vs * v + 1], verts[vs * v + 2]);
 }
 
+// The three vertex terms of one query or ray: vertex ids and the float3 added to each
+// (weight 0 or a dead query: nothing to add, vertex id kNoVertex).
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
+struct CornerTerms {
+    uint32_t v[3];
+    float c[3][3];
+};
+
 // Query i of a closest-point backward.  The saved id selects the triangle (no
 // re-traversal); its closest point, offset q - p and barycentrics are recomputed in FP64
 // (closest_pt_tri_f64_core, the computation the forward pass reports from), so the
 // direction (q - p)/|q - p| stays accurate however close the query is to the surface --
 // a point rounded to FP32 would be off by ~1e-7 of the coordinates, a large angle when
 // |q - p| is tiny.  The saved pt / dist / bary are not needed.  Distance 0 contributes
-// nothing (the gradient is undefined on the surface).
-__device__ __forceinline__ void cpq_bwd_one(size_t i, const float* __restrict__ q, const int32_t* __restrict__ prim,
-                                            const float* __restrict__ g, const uint32_t* __restrict__ tri_idx,
-                                            uint32_t n_tris, const float* __restrict__ verts, uint64_t vs,
-                                            float* __restrict__ g_q, float* g_verts) {
+// nothing (the gradient is undefined on the surface).  Writes g_q[i] (when g_q) and
+// returns the vertex terms -b_j g_q.
+__device__ __forceinline__ CornerTerms cpq_bwd_terms(size_t i, const float* __restrict__ q,
+                                                     const int32_t* __restrict__ prim, const float* __restrict__ g,
+                                                     const uint32_t* __restrict__ tri_idx, uint32_t n_tris,
+                                                     const float* __restrict__ verts, uint64_t vs,
+                                                     float* __restrict__ g_q) {
     const int32_t p = prim[i];
     float gx = 0.f, gy = 0.f, gz = 0.f;
     double b[3] = {0.0, 0.0, 0.0};
@@ -56,26 +68,27 @@ __device__ __forceinline__ void cpq_bwd_one(size_t i, const float* __restrict__ 
         g_q[3 * i + 1] = gy;
         g_q[3 * i + 2] = gz;
     }
-    if (g_verts && live) {
+    CornerTerms t;
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float bj = (float)b[j];
-            if (bj == 0.0f) continue;
-            const size_t v = tri_idx[3 * (size_t)p + j];
-            unsafeAtomicAdd(&g_verts[3 * v], -bj * gx);
-            unsafeAtomicAdd(&g_verts[3 * v + 1], -bj * gy);
-            unsafeAtomicAdd(&g_verts[3 * v + 2], -bj * gz);
-        }
+    for (int j = 0; j < 3; j++) {
+        const float bj = (float)b[j];
+        t.v[j] = live && bj != 0.0f ? tri_idx[3 * (size_t)p + j] : kNoVertex;
+        t.c[j][0] = -bj * gx;
+        t.c[j][1] = -bj * gy;
+        t.c[j][2] = -bj * gz;
     }
+    return t;
 }
 
 // Ray i of a hit-distance backward (misses and rays parallel to the plane contribute
-// nothing).
-__device__ __forceinline__ void ray_bwd_one(size_t i, const float4* __restrict__ rays, const int32_t* __restrict__ prim,
-                                            const float* __restrict__ t, const float* __restrict__ uv,
-                                            const float* __restrict__ g, const uint32_t* __restrict__ tri_idx,
-                                            uint32_t n_tris, const float* __restrict__ verts, uint64_t vs,
-                                            float* __restrict__ g_o, float* __restrict__ g_d, float* g_verts) {
+// nothing).  Writes g_o[i] / g_d[i] (when given) and returns the vertex terms
+// b_j n / (n.d) scaled by g (vertex ids kNoVertex without uv).
+__device__ __forceinline__ CornerTerms ray_bwd_terms(size_t i, const float4* __restrict__ rays,
+                                                     const int32_t* __restrict__ prim, const float* __restrict__ t,
+                                                     const float* __restrict__ uv, const float* __restrict__ g,
+                                                     const uint32_t* __restrict__ tri_idx, uint32_t n_tris,
+                                                     const float* __restrict__ verts, uint64_t vs,
+                                                     float* __restrict__ g_o, float* __restrict__ g_d) {
     const int32_t p = prim[i];
     V3 go = v3(0.f, 0.f, 0.f), gd = v3(0.f, 0.f, 0.f);
     V3 nrm = v3(0.f, 0.f, 0.f);
@@ -110,18 +123,23 @@ __device__ __forceinline__ void ray_bwd_one(size_t i, const float4* __restrict__
         g_d[3 * i + 1] = gd.y;
         g_d[3 * i + 2] = gd.z;
     }
-    if (g_verts && uv && live) {
+    CornerTerms ct;
+    float bw[3] = {0.f, 0.f, 0.f};
+    if (uv && live) {
         const float u = uv[2 * i], v = uv[2 * i + 1];
-        const float bw[3] = {(1.0f - u) - v, u, v};
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float w = bw[j] * s;
-            if (w == 0.0f) continue;
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j]], nrm.x * w);
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j] + 1], nrm.y * w);
-            unsafeAtomicAdd(&g_verts[3 * (size_t)vi[j] + 2], nrm.z * w);
-        }
+        bw[0] = (1.0f - u) - v;
+        bw[1] = u;
+        bw[2] = v;
     }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const float w = bw[j] * s;
+        ct.v[j] = uv && live && w != 0.0f ? vi[j] : kNoVertex;
+        ct.c[j][0] = nrm.x * w;
+        ct.c[j][1] = nrm.y * w;
+        ct.c[j][2] = nrm.z * w;
+    }
+    return ct;
 }
 
 }  // namespace gpurt_impl

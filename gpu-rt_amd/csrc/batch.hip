@@ -237,26 +237,67 @@ __global__ __launch_bounds__(256) void k_batch_cpq_finish(const BatchMeshDev* __
                    md.q_bary);
 }
 
-// One wave per 64-query chunk of the forward chunk list.
+// Vertex terms of the batched backward, one wave per 64-item chunk of the forward chunk
+// list (closest-point chunks, then ray chunks): item (chunk c, lane l) owns term slots
+// 3 (64 c + l) .. + 2, keyed by (mesh << vbits | vertex), or `none` when it adds nothing
+// (no such item, no vertex gradient requested, a zero weight).  Query and ray gradients
+// are written here.
 template <bool RAYS>
-__global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restrict__ meshes,
-                                                   const BatchGradDev* __restrict__ grads,
-                                                   const uint2* __restrict__ chunks, uint32_t n_chunks) {
+__global__ __launch_bounds__(256) void k_batch_bwd_terms(const BatchMeshDev* __restrict__ meshes,
+                                                         const BatchGradDev* __restrict__ grads,
+                                                         const uint2* __restrict__ chunks, uint32_t n_chunks,
+                                                         uint32_t first_chunk, int vbits, uint64_t none,
+                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                         float* __restrict__ terms) {
     const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (c >= n_chunks) return;
     const uint2 ch = chunks[c];
     const BatchMeshDev& md = meshes[ch.x];
     const BatchGradDev& gd = grads[ch.x];
     const uint32_t i = ch.y + (threadIdx.x & 63u);
+    CornerTerms t{{kNoVertex, kNoVertex, kNoVertex}, {}};
     if (RAYS) {
-        if (i >= md.n_r || !gd.g_t) return;
-        ray_bwd_one(i, md.rays, md.r_prim, md.r_t, md.r_uv, gd.g_t, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride,
-                    gd.g_origin, gd.g_dir, gd.g_verts);
+        if (i < md.n_r && gd.g_t)
+            t = ray_bwd_terms(i, md.rays, md.r_prim, md.r_t, md.r_uv, gd.g_t, gd.tri_idx, gd.n_tris, gd.verts,
+                              gd.vstride, gd.g_origin, gd.g_dir);
     } else {
-        if (i >= md.n_q || !gd.g_dist) return;
-        cpq_bwd_one(i, md.q, md.q_prim, gd.g_dist, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride, gd.g_q,
-                    gd.g_verts);
+        if (i < md.n_q && gd.g_dist)
+            t = cpq_bwd_terms(i, md.q, md.q_prim, gd.g_dist, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride, gd.g_q);
     }
+    const size_t k0 = 3 * (64 * ((size_t)first_chunk + c) + (threadIdx.x & 63u));
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const size_t k = k0 + j;
+        keys[k] = (t.v[j] == kNoVertex || !gd.g_verts) ? none : ((uint64_t)ch.x << vbits | t.v[j]);
+        vals[k] = (uint32_t)k;
+        terms[3 * k] = t.c[j][0];
+        terms[3 * k + 1] = t.c[j][1];
+        terms[3 * k + 2] = t.c[j][2];
+    }
+}
+
+// One thread per sorted term; the first term of a (mesh, vertex) run adds the run, in
+// term order, to that mesh's vertex gradient (cleared by k_batch_zero_vgrads).
+__global__ __launch_bounds__(kBlock) void k_batch_vertex_runs(uint32_t m, const uint64_t* __restrict__ keys,
+                                                              const uint32_t* __restrict__ vals,
+                                                              const float* __restrict__ terms, int vbits,
+                                                              uint64_t none, const BatchGradDev* __restrict__ grads) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const uint64_t key = keys[p];
+    if (key >= none || (p > 0 && keys[p - 1] == key)) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (uint32_t r = p; r < m && keys[r] == key; r++) {
+        const size_t k = vals[r];
+        sx += terms[3 * k];
+        sy += terms[3 * k + 1];
+        sz += terms[3 * k + 2];
+    }
+    float* g = grads[key >> vbits].g_verts;
+    const size_t v = key & ((1ull << vbits) - 1);
+    g[3 * v] += sx;
+    g[3 * v + 1] += sy;
+    g[3 * v + 2] += sz;
 }
 
 }  // namespace gpurt_impl
@@ -265,6 +306,7 @@ using namespace gpurt_impl;
 
 struct gpurt_batch_s {
     gpurt_ctx ctx = nullptr;
+    int device = 0;  // the context's device (destroy does not read the context)
     int width = 4, leaf_size = 4, stack = 8, block = 256, format = kFormatF32;
     uint32_t leaf_trigger = 32;
     BuildBufs build;
@@ -283,6 +325,7 @@ struct gpurt_batch_s {
     std::vector<gpurt_batch_mesh> host_meshes;
     BatchGradDev* grads_d = nullptr;
     uint2* zero_segs_d = nullptr;
+    void* terms_mem = nullptr;  // vertex terms of the backward and their sort buffers
     hipGraph_t bgraph = nullptr;
     hipGraphExec_t bexec = nullptr;
     void drop_backward() {
@@ -290,6 +333,8 @@ struct gpurt_batch_s {
         if (bgraph) (void)hipGraphDestroy(bgraph);
         if (grads_d) (void)hipFree(grads_d);
         if (zero_segs_d) (void)hipFree(zero_segs_d);
+        if (terms_mem) (void)hipFree(terms_mem);
+        terms_mem = nullptr;
         bexec = nullptr;
         bgraph = nullptr;
         grads_d = nullptr;
@@ -416,6 +461,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     auto* b = new gpurt_batch_s();
     b->ctx = ctx;
+    b->device = ctx->device;
     b->width = W;
     b->leaf_size = leaf;
     b->stack = S;
@@ -498,7 +544,8 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
         delete b;
         return rc;
     }
-    // one eager run validates the inputs (vertex indices) before the handle is returned
+    // one eager run validates the inputs (vertex indices, finite positions) before the
+    // handle is returned
     uint32_t h_counters[4] = {0, 0, 0, 0};
     uint32_t h_last = 0;
     if (hipGraphLaunch(b->exec, b->cap) != hipSuccess ||
@@ -510,6 +557,7 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
         return rc;
     }
     if (h_counters[1] & 1u) rc = set_error(GPURT_E_INVALID, "triangle index >= n_verts");
+    else if (h_counters[1] & 4u) rc = set_error(GPURT_E_INVALID, "non-finite vertex position");
     else if (h_counters[1] & 2u) rc = set_error(GPURT_E_DEVICE, "wide-node capacity exceeded");
     else if (h_last) rc = set_error(GPURT_E_UNSUPPORTED, "BVH deeper than the supported level count");
     if (rc) {
@@ -547,15 +595,39 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         if (g.g_verts)
             for (uint64_t f = 0; f < 3 * (uint64_t)hm.n_verts; f += kZeroSeg) segs.push_back(make_uint2((uint32_t)m, (uint32_t)f));
     }
+    // vertex terms: 3 per item slot of every chunk, keyed by (mesh, vertex)
+    const uint32_t n_chunks = b->n_cpq_chunks + b->n_ray_chunks;
+    const uint64_t m64 = 3ull * 64 * n_chunks;
+    if (m64 > 0xFFFFFFFFull - kBlock) return set_error(GPURT_E_INVALID, "too many queries for a batched backward");
+    const uint32_t m = (uint32_t)m64;
+    uint32_t max_verts = 1;
+    for (const gpurt_batch_mesh& hm : b->host_meshes) max_verts = std::max<uint32_t>(max_verts, (uint32_t)hm.n_verts);
+    int vbits = 1, mbits = 1;
+    while (vbits < 32 && (max_verts - 1) >> vbits) vbits++;
+    while (mbits < 32 && (n_meshes >> mbits)) mbits++;  // n_meshes itself marks "no vertex"
+    if (vbits + mbits > 63) return set_error(GPURT_E_INVALID, "batch too large for a batched backward");
+    const uint64_t none = (uint64_t)n_meshes << vbits;
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t tbytes = 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) + up(12 * (size_t)m) +
+                          up(4 * radix_sort_scratch_words(std::max<uint32_t>(m, 1)));
     DevGuard dg(b->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     b->drop_backward();
     if (hipMalloc((void**)&b->grads_d, sizeof(BatchGradDev) * n_meshes) != hipSuccess ||
-        (!segs.empty() && hipMalloc((void**)&b->zero_segs_d, sizeof(uint2) * segs.size()) != hipSuccess)) {
+        (!segs.empty() && hipMalloc((void**)&b->zero_segs_d, sizeof(uint2) * segs.size()) != hipSuccess) ||
+        (m && hipMalloc(&b->terms_mem, tbytes) != hipSuccess)) {
         (void)hipGetLastError();
         b->drop_backward();
         return set_error(GPURT_E_OOM, "batch gradient allocation failed");
     }
+    char* tp = static_cast<char*>(b->terms_mem);
+    uint64_t* k0 = reinterpret_cast<uint64_t*>(tp);
+    uint64_t* k1 = reinterpret_cast<uint64_t*>(tp + up(8 * (size_t)m));
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m));
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m) + up(4 * (size_t)m));
+    float* terms = reinterpret_cast<float*>(tp + 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m));
+    uint32_t* sort_scratch = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) +
+                                                         up(12 * (size_t)m));
     if (hipMemcpy(b->grads_d, gd.data(), sizeof(BatchGradDev) * n_meshes, hipMemcpyHostToDevice) != hipSuccess ||
         (!segs.empty() &&
          hipMemcpy(b->zero_segs_d, segs.data(), sizeof(uint2) * segs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
@@ -572,11 +644,18 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         hipLaunchKernelGGL(k_batch_zero_vgrads, dim3((unsigned)segs.size()), dim3(256), 0, b->cap, b->grads_d,
                            b->zero_segs_d);
     if (b->n_cpq_chunks)
-        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
-                           b->grads_d, b->chunks_d, b->n_cpq_chunks);
+        hipLaunchKernelGGL(k_batch_bwd_terms<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap,
+                           b->meshes_d, b->grads_d, b->chunks_d, b->n_cpq_chunks, 0u, vbits, none, k0, v0, terms);
     if (b->n_ray_chunks)
-        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
-                           b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks);
+        hipLaunchKernelGGL(k_batch_bwd_terms<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap,
+                           b->meshes_d, b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks, b->n_cpq_chunks,
+                           vbits, none, k0, v0, terms);
+    if (m) {
+        // each (mesh, vertex)'s terms in chunk order, then summed without atomics
+        const bool in1 = radix_sort_pairs(k0, v0, k1, v1, m, vbits + mbits, sort_scratch, b->cap);
+        hipLaunchKernelGGL(k_batch_vertex_runs, dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m, in1 ? k1 : k0,
+                           in1 ? v1 : v0, terms, vbits, none, b->grads_d);
+    }
     hipError_t el = hipGetLastError();
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(b->cap, &g);
@@ -601,7 +680,7 @@ int gpurt_batch_run_backward(gpurt_batch b, void* stream) {
 
 void gpurt_batch_destroy(gpurt_batch b) {
     if (!b) return;
-    DevGuard dg(b->ctx->device);
+    DevGuard dg(b->device);
     (void)hipDeviceSynchronize();
     delete b;
 }

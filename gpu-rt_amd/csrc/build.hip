@@ -134,15 +134,21 @@ __global__ __launch_bounds__(kBlock) void k_prep(const SegDesc* __restrict__ seg
         }
         if (bad) atomicOr(&counters[1], 1u);
         float v[3][3];
+        bool finite = true;
         for (int k = 0; k < 3; k++) {
             const float* p = sd.verts + (size_t)vi[k] * sd.vstride;
             v[k][0] = p[0]; v[k][1] = p[1]; v[k][2] = p[2];
+            finite = finite && isfinite(v[k][0]) && isfinite(v[k][1]) && isfinite(v[k][2]);
         }
+        // a non-finite position is reported (the build fails with GPURT_E_INVALID); the
+        // triangle's box is then the origin so every later stage works on finite boxes
+        if (!finite) atomicOr(&counters[1], 4u);
         for (int a = 0; a < 3; a++) {
             for (int k = 0; k < 3; k++) {
-                lo[a] = fminf(lo[a], v[k][a]);
-                hi[a] = fmaxf(hi[a], v[k][a]);
-                amax = fmaxf(amax, fabsf(v[k][a]));
+                const float x = finite ? v[k][a] : 0.0f;
+                lo[a] = fminf(lo[a], x);
+                hi[a] = fmaxf(hi[a], x);
+                amax = fmaxf(amax, fabsf(x));
             }
             cen[a] = (lo[a] + hi[a]) * 0.5f;
             cenx[a] = cen[a];
@@ -346,6 +352,35 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
         for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
         __syncthreads();
     }
+}
+
+size_t radix_sort_scratch_words(uint32_t n) {
+    const uint32_t nblocks = cdiv(n, kSortTile);
+    return 256 * (size_t)nblocks + cdiv(256 * (size_t)nblocks, kScanTile);
+}
+
+bool radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint32_t n, int key_bits,
+                      uint32_t* scratch, hipStream_t s) {
+    const uint32_t nblocks = cdiv(n, kSortTile);
+    uint32_t* hist = scratch;  // 256 digit counts per block, digit-major
+    uint32_t* scan_tmp = scratch + 256 * (size_t)nblocks;
+    const uint32_t hist_n = 256u * nblocks;
+    const unsigned scan_tiles = cdiv(hist_n, kScanTile);
+    uint64_t *kin = k0, *kout = k1;
+    uint32_t *vin = v0, *vout = v1;
+    bool in1 = false;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kBlock), 0, s, kin, n, shift, hist, nblocks);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(kBlock), 0, s, hist, hist_n, scan_tmp);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, scan_tmp, scan_tiles);
+        hipLaunchKernelGGL(k_scan_apply, dim3(scan_tiles), dim3(kBlock), 0, s, hist, hist_n, scan_tmp);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kBlock), 0, s, kin, vin, kout, vout, n, shift, hist,
+                           nblocks);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        in1 = !in1;
+    }
+    return in1;
 }
 
 // ------------------------------------------------------------------------------------
@@ -694,7 +729,7 @@ __global__ void k_init_levels(int n_seg, const uint32_t* __restrict__ seg_root, 
 
 BuildBufs::~BuildBufs() {
     for (void* p : allocs)
-        if (p) (void)hipFree(p);
+        if (p) (void)(pooled ? hipFreeAsync(p, pool_stream) : hipFree(p));
 }
 
 void* BuildBufs::release(void* p) {
@@ -730,8 +765,7 @@ static void carve_all(BuildBufs& b, Carver& c, size_t NI) {
     c.take(b.keys1, N);
     c.take(b.vals0, N);
     c.take(b.vals1, N);
-    c.take(b.hist, 256 * (size_t)b.nblocks_sort);
-    c.take(b.scan_tmp, cdiv(256 * (size_t)b.nblocks_sort, kScanTile));
+    c.take(b.hist, radix_sort_scratch_words(N));  // the sort's digit counts and scan partials
     c.take(b.saabb, 2 * (size_t)N);
     c.take(b.left, NI);
     c.take(b.right, NI);
@@ -773,46 +807,41 @@ int build_alloc(const SegDesc* host_segs, int n_seg, int W, int leaf_size, int t
     Carver sizing;
     carve_all(b, sizing, NI);
     void* scratch = nullptr;
-    if (hipMalloc(&scratch, sizing.off) != hipSuccess) {
+    if (b.alloc(&scratch, sizing.off) != hipSuccess) {
         (void)hipGetLastError();
-        return set_error(GPURT_E_OOM, "hipMalloc of " + std::to_string(sizing.off) + " bytes of build scratch failed");
+        return set_error(GPURT_E_OOM, "allocation of " + std::to_string(sizing.off) + " bytes of build scratch failed");
     }
-    b.allocs.push_back(scratch);
     Carver c;
     c.base = static_cast<char*>(scratch);
     carve_all(b, c, NI);
     void* nodes = nullptr;
     void* tris = nullptr;
     void* order = nullptr;
-    if (hipMalloc(&nodes, node_bytes(W) * b.capacity) != hipSuccess ||
-        hipMalloc(&tris, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
-        hipMalloc(&order, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
+    if (b.alloc(&nodes, node_bytes(W) * b.capacity) != hipSuccess ||
+        b.alloc(&tris, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
+        b.alloc(&order, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
         (void)hipGetLastError();
-        for (void* p : {nodes, tris, order})
-            if (p) (void)hipFree(p);
-        return set_error(GPURT_E_OOM, "hipMalloc of BVH storage failed");
+        return set_error(GPURT_E_OOM, "allocation of BVH storage failed");  // b frees what it holds
     }
-    b.allocs.push_back(nodes);
-    b.allocs.push_back(tris);
-    b.allocs.push_back(order);
     b.nodes = nodes;
     b.tris = static_cast<float4*>(tris);
     b.leaf_order = static_cast<uint32_t*>(order);
     if (treelet_size > 0) {
         void* t2 = nullptr;
         void* o2 = nullptr;
-        if (hipMalloc(&t2, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
-            hipMalloc(&o2, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
+        if (b.alloc(&t2, sizeof(float4) * 3 * (size_t)N) != hipSuccess ||
+            b.alloc(&o2, sizeof(uint32_t) * (size_t)N) != hipSuccess) {
             (void)hipGetLastError();
-            if (t2) (void)hipFree(t2);
-            return set_error(GPURT_E_OOM, "hipMalloc of re-pack buffers failed");
+            return set_error(GPURT_E_OOM, "allocation of re-pack buffers failed");
         }
-        b.allocs.push_back(t2);
-        b.allocs.push_back(o2);
         b.tris2 = static_cast<float4*>(t2);
         b.leaf_order2 = static_cast<uint32_t*>(o2);
     }
-    GPURT_HIP(hipMemcpy(b.segs, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice));
+    if (b.pooled) {
+        GPURT_HIP(hipMemcpyAsync(b.segs, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice, b.pool_stream));
+    } else {
+        GPURT_HIP(hipMemcpy(b.segs, host_segs, sizeof(SegDesc) * n_seg, hipMemcpyHostToDevice));
+    }
     return GPURT_OK;
 }
 
@@ -840,21 +869,10 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
     while ((1u << seg_bits) < (uint32_t)n_seg) seg_bits++;
     const int key_bits = 30 + seg_bits;
     uint64_t* kin = b.keys0;
-    uint64_t* kout = b.keys1;
     uint32_t* vin = b.vals0;
-    uint32_t* vout = b.vals1;
-    const uint32_t hist_n = 256u * b.nblocks_sort;
-    const unsigned scan_tiles = cdiv(hist_n, kScanTile);
-    for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, N, shift, b.hist,
-                           b.nblocks_sort);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(kBlock), 0, s, b.hist, hist_n, b.scan_tmp);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, b.scan_tmp, scan_tiles);
-        hipLaunchKernelGGL(k_scan_apply, dim3(scan_tiles), dim3(kBlock), 0, s, b.hist, hist_n, b.scan_tmp);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(b.nblocks_sort), dim3(kBlock), 0, s, kin, vin, kout,
-                           vout, N, shift, b.hist, b.nblocks_sort);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
+    if (radix_sort_pairs(b.keys0, b.vals0, b.keys1, b.vals1, N, key_bits, b.hist, s)) {
+        kin = b.keys1;
+        vin = b.vals1;
     }
     hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, vin, b.aabb_lo, b.aabb_hi, b.tri_v,
                        b.saabb, b.tris, b.leaf_order);
@@ -937,7 +955,12 @@ int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, int leaf_size,
                    int treelet_size, int treelet_rounds, int format, hipStream_t s, gpurt_bvh_s* out) {
     (void)ctx;
+    // every buffer from the stream-ordered pool on the build stream: the products become
+    // the BVH's (freed in stream order by gpurt_bvh_destroy), the scratch is freed on the
+    // stream when b goes out of scope
     BuildBufs b;
+    b.pooled = true;
+    b.pool_stream = s;
     int rc;
     if ((rc = build_alloc(host_segs, n_seg, W, leaf_size, treelet_size, b))) return rc;
     if ((rc = build_enqueue_binary(b, treelet_size, treelet_rounds, s))) return rc;
@@ -960,6 +983,7 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     GPURT_HIP(hipStreamSynchronize(s));
     if (!done) return set_error(GPURT_E_UNSUPPORTED, "BVH deeper than the supported level count");
     if (h_counters[1] & 1u) return set_error(GPURT_E_INVALID, "triangle index >= n_verts");
+    if (h_counters[1] & 4u) return set_error(GPURT_E_INVALID, "non-finite vertex position");
     if (h_counters[1] & 2u) return set_error(GPURT_E_DEVICE, "wide-node capacity exceeded");
     int depth = 0;
     while (depth <= kMaxLevels && h_lvl[depth] > 0) depth++;
@@ -967,12 +991,11 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     const uint32_t n_nodes = h_counters[0];
     void* nodes = b.nodes;
     void* tight = nullptr;
-    if (hipMalloc(&tight, node_bytes(W) * n_nodes) == hipSuccess) {
-        if (hipMemcpyAsync(tight, nodes, node_bytes(W) * n_nodes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
-            hipStreamSynchronize(s) == hipSuccess) {
-            nodes = tight;  // b frees the large array
+    if (hipMallocAsync(&tight, node_bytes(W) * n_nodes, s) == hipSuccess) {
+        if (hipMemcpyAsync(tight, nodes, node_bytes(W) * n_nodes, hipMemcpyDeviceToDevice, s) == hipSuccess) {
+            nodes = tight;  // b frees the large array (after the copy, in stream order)
         } else {
-            (void)hipFree(tight);
+            (void)hipFreeAsync(tight, s);
             (void)hipGetLastError();
             b.release(nodes);
         }
@@ -980,6 +1003,7 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
         (void)hipGetLastError();
         b.release(nodes);
     }
+    out->nodes = nodes;  // the caller frees the products on failure from here on
     uint32_t u = st[6];
     u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
     float scale;
@@ -1004,7 +1028,7 @@ int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int W, in
     out->nodes = nodes;
     out->format = format;
     if (has_qnodes(format)) {
-        GPURT_HIP(hipMalloc(&out->qnodes, qnode_bytes(W) * std::max<uint32_t>(n_nodes, 1u)));
+        GPURT_HIP(hipMallocAsync(&out->qnodes, qnode_bytes(W) * std::max<uint32_t>(n_nodes, 1u), s));
         launch_quantize(W, nodes, out->qnodes, n_nodes, n_nodes, nullptr, s);
         GPURT_HIP(hipGetLastError());
     }

@@ -21,7 +21,10 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gpurt.h"
@@ -122,6 +125,14 @@ struct gpurt_ctx_s {
     int n_cu = 256;
     int lds_per_cu = 160 * 1024;
     uint32_t* err = nullptr;  // sticky device error bits (bit 0: traversal stack overflow)
+    // stream on which destroyed handles release their memory, after waiting for the
+    // recorded uses of the handle (stream-ordered, no host wait)
+    hipStream_t release = nullptr;
+    // BVHs and TLASes still open on the context: gpurt_ctx_destroy releases their device
+    // memory, and destroying such a handle afterwards frees only its host part
+    std::mutex live_mu;
+    std::set<gpurt_bvh_s*> bvhs;
+    std::set<gpurt_tlas_s*> tlases;
 };
 
 struct gpurt_bvh_s {
@@ -144,6 +155,12 @@ struct gpurt_bvh_s {
     float* verts = nullptr;       // 3 * n_verts compact positions, original order
     uint32_t* leaf_order = nullptr;  // packed position -> original triangle (refit)
     std::vector<uint32_t> level_cnt;  // wide nodes per level (level l starts at sum of previous)
+    // Every buffer above comes from the stream-ordered pool (hipMallocAsync on the build
+    // stream), so gpurt_bvh_destroy frees it with hipFreeAsync on ctx->release after
+    // waiting for the last use recorded on each stream that used the handle.
+    std::mutex use_mu;
+    std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
+    bool uses_lost = false;  // an event could not be recorded: destroy waits for the device
 };
 
 namespace gpurt_impl {
@@ -151,6 +168,12 @@ namespace gpurt_impl {
 // error plumbing (abi.hip)
 int set_error(int code, const std::string& msg);
 int hip_error(hipError_t e, const char* what);
+
+// use tracking of a BVH handle (abi.hip): note_use records, after work that reads `b`
+// was enqueued on `s`, an event there (one event per stream, re-recorded); wait_uses
+// makes stream `s` wait for every recorded use
+void note_use(gpurt_bvh_s* b, hipStream_t s);
+void wait_uses(gpurt_bvh_s* b, hipStream_t s);
 
 #define GPURT_HIP(call)                                            \
     do {                                                           \
@@ -177,7 +200,6 @@ struct BuildBufs {
     float4 *aabb_lo = nullptr, *aabb_hi = nullptr, *tri_v = nullptr;
     uint32_t* segstat_rep = nullptr;  // per-mesh statistics, replicated for atomics
     uint32_t* segstat = nullptr;      // per-mesh statistics, reduced
-    uint32_t* scan_tmp = nullptr;
     uint64_t *keys0 = nullptr, *keys1 = nullptr;
     uint32_t *vals0 = nullptr, *vals1 = nullptr, *hist = nullptr;
     float4* saabb = nullptr;  // sorted triangle boxes (lo, hi)
@@ -202,6 +224,16 @@ struct BuildBufs {
     float4* final_tris() const { return repacked ? tris2 : tris; }
     uint32_t* final_order() const { return repacked ? leaf_order2 : leaf_order; }
     std::vector<void*> allocs;  // freed with the object unless release()d
+    // stream-ordered allocations (hipMallocAsync / hipFreeAsync on pool_stream) when set,
+    // else hipMalloc / hipFree (the batch keeps its buffers for its lifetime)
+    hipStream_t pool_stream = nullptr;
+    bool pooled = false;
+    hipError_t alloc(void** p, size_t bytes) {
+        *p = nullptr;
+        const hipError_t e = pooled ? hipMallocAsync(p, bytes, pool_stream) : hipMalloc(p, bytes);
+        if (e == hipSuccess) allocs.push_back(*p);
+        return e;
+    }
     BuildBufs() = default;
     BuildBufs(const BuildBufs&) = delete;
     BuildBufs& operator=(const BuildBufs&) = delete;
@@ -213,6 +245,13 @@ int build_alloc(const SegDesc* host_segs, int n_seg, int width, int leaf_size, i
 int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hipStream_t s);
 int build_enqueue_levels(BuildBufs& b, int leaf_size, int level_begin, int level_end, hipStream_t s);
 int run_treelets(BuildBufs& b, int treelet_size, int rounds, int key_bits, int nbits, hipStream_t s);
+// Stable LSD radix sort (build.hip) of n (uint64 key, uint32 value) pairs on the low
+// key_bits bits, 8 bits per pass, ping-ponging between (k0, v0) and (k1, v1); returns
+// true when the sorted pairs end in (k1, v1).  scratch: radix_sort_scratch_words(n)
+// words.
+bool radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint32_t n, int key_bits,
+                      uint32_t* scratch, hipStream_t s);
+size_t radix_sort_scratch_words(uint32_t n);
 int build_segments(gpurt_ctx ctx, const SegDesc* host_segs, int n_seg, int width, int leaf_size,
                    int treelet_size, int treelet_rounds, int format, hipStream_t stream, gpurt_bvh_s* out);
 
@@ -284,6 +323,11 @@ int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t*
                   int32_t* prim, float* t, float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts,
                   hipStream_t s, void* scratch, size_t scratch_bytes);
 size_t tlas_scratch_bytes(gpurt_tlas T, size_t n);
+// records a use of the TLAS (and through it of its BLASes) on stream s (tlas.hip)
+void tlas_note_use(gpurt_tlas t, hipStream_t s);
+// gpurt_ctx_destroy's part for a TLAS still open: frees its device memory on the
+// context's release stream and detaches it from the context (tlas.hip)
+void tlas_orphan(gpurt_tlas t);
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 

@@ -3,6 +3,7 @@
 // The reference only rebuilds (VK_BUILD_ACCELERATION_STRUCTURE_MODE_BUILD_KHR,
 // src/vk/vulkan.cpp:832,915); an optimisation loop that moves vertices every step wants
 // the cheaper update.  Steps:
+//   k_refit_check   every new position finite, else GPURT_E_INVALID with the tree unchanged
 //   k_refit_tris    re-gather each packed triangle from the new vertex positions (its id
 //                   and leaf position are unchanged), compact copy of the positions for
 //                   the backward passes, and the new coordinate scale
@@ -27,7 +28,21 @@ __device__ __forceinline__ float o2f_r(uint32_t u) {
 }
 
 __global__ void k_refit_init(uint32_t* scale) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *scale = f2o_r(0.0f);
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        scale[0] = f2o_r(0.0f);
+        scale[1] = 0u;  // set when a position is not finite
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_check(uint32_t n_verts, const float* __restrict__ v,
+                                                        uint64_t stride, uint32_t* flag) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    bool bad = false;
+    if (i < n_verts) {
+        const float* p = v + (size_t)i * stride;
+        bad = !(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]));
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) gatomic_or(flag, 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_refit_verts(uint32_t n_verts, const float* __restrict__ v,
@@ -122,6 +137,17 @@ int launch_refit(gpurt_bvh_s* b, const float* verts, size_t vstride_floats, hipS
     GPURT_HIP(hipMallocAsync(&tmp, 256, s));
     uint32_t* scale = static_cast<uint32_t*>(tmp);
     hipLaunchKernelGGL(k_refit_init, dim3(1), dim3(64), 0, s, scale);
+    // every new position must be finite; the tree is left as it was otherwise
+    hipLaunchKernelGGL(k_refit_check, dim3(cdiv(b->n_verts, kBlock)), dim3(kBlock), 0, s, (uint32_t)b->n_verts,
+                       verts, (uint64_t)vstride_floats, scale + 1);
+    uint32_t bad = 0;
+    hipError_t ce = hipMemcpyAsync(&bad, scale + 1, sizeof(bad), hipMemcpyDeviceToHost, s);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(s);
+    if (ce != hipSuccess || bad) {
+        (void)hipFreeAsync(tmp, s);
+        if (ce != hipSuccess) return hip_error(ce, "refit check");
+        return set_error(GPURT_E_INVALID, "non-finite vertex position");
+    }
     hipLaunchKernelGGL(k_refit_verts, dim3(cdiv(b->n_verts, kBlock)), dim3(kBlock), 0, s, (uint32_t)b->n_verts,
                        verts, (uint64_t)vstride_floats, b->verts, scale);
     hipLaunchKernelGGL(k_refit_tris, dim3(cdiv(b->n_tris, kBlock)), dim3(kBlock), 0, s, (uint32_t)b->n_tris,

@@ -90,12 +90,15 @@ __device__ __forceinline__ uint32_t* tlas_group_count() {
 __device__ __forceinline__ uint32_t t_order(float t) { return (uint32_t)(result_key(t, 0u) >> 32); }
 __device__ __forceinline__ float t_from_order(uint32_t o) { return key_value((unsigned long long)o << 32); }
 
+// `outside` collects (as a wave ballot) the lanes whose ray origin lies beyond `obound`
+// in some coordinate: outside the domain where the instance boxes' padding keeps the
+// answers exact (the kernel reports it through the context's status word)
 template <typename NodeT, int S, int BS, bool ANY, bool STATS, typename Src, typename Out>
 __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_nodes,
                                                 const float4* __restrict__ top_tris,
                                                 const InstDev* __restrict__ inst, Src& src, uint32_t trig,
                                                 LaneStack<S, BS>& st, Out out, unsigned long long& n_nodes,
-                                                unsigned long long& n_tris) {
+                                                unsigned long long& n_tris, float obound, uint64_t& outside) {
     constexpr int W = NodeT::kWidth;
     bool ok = true;
     typename Src::Handle h{};
@@ -222,6 +225,8 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
                     r.set(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
                     tmin = r0.w;
                     best = r1.w;
+                    empty_if_invalid_ray(r0, r1, tmin, best);
+                    outside |= __ballot(fmaxf(fmaxf(fabsf(r0.x), fabsf(r0.y)), fabsf(r0.z)) > obound);
                     bid = binst = 0xFFFFFFFFu;
                     hit_any = false;
                     st.sp = 0;
@@ -415,6 +420,7 @@ struct TlasArgs {
     unsigned long long* stats;
     uint32_t* err;
     uint32_t leaf_trigger;
+    float origin_bound;  // |origin| coordinates beyond this are outside the exact domain
 };
 
 template <int W, bool Q, int S, int BS, bool ANY, bool STATS>
@@ -441,10 +447,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W
         }
     };
     unsigned long long n_nodes = 0, n_tris = 0;
+    uint64_t outside = 0;
     if (!tlas_ray_stream<NodeOf<W, Q>, S, BS, ANY, STATS>(reinterpret_cast<const NodeOf<W, Q>*>(a.top_nodes),
                                                            a.top_tris, a.inst, src, a.leaf_trigger, st, out,
-                                                           n_nodes, n_tris))
+                                                           n_nodes, n_tris, a.origin_bound, outside))
         gatomic_or(a.err, 1u);
+    if (outside && __lane_id() == 0) gatomic_or(a.err, 2u);
     if (STATS) {
         atomicAdd(&a.stats[0], n_nodes);
         atomicAdd(&a.stats[1], n_tris);
@@ -506,11 +514,17 @@ struct gpurt_tlas_s {
     int width = 4;
     bool q8 = false;
     int max_blas_depth = 0;
+    float origin_bound = INFINITY;      // 2^8 x the largest |world box coordinate|
     std::vector<uint32_t> custom;       // custom index of every instance (host copy)
     std::vector<gpurt_bvh_s*> blas;     // BLAS of every instance
+    // frees in stream order on the context's release stream (see gpurt_tlas_destroy);
+    // a TLAS whose context is gone (tlas_orphan) holds no device memory
     ~gpurt_tlas_s() {
+        if (inst_d && ctx) {
+            if (top) wait_uses(top, ctx->release);
+            (void)hipFreeAsync(inst_d, ctx->release);
+        }
         if (top) gpurt_bvh_destroy(top);
-        if (inst_d) (void)hipFree(inst_d);
     }
 };
 
@@ -658,28 +672,28 @@ int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n, c
     T->width = W;
     T->q8 = q8;
     T->max_blas_depth = max_depth;
+    T->origin_bound = (float)std::ldexp(std::max(scale, 1e-30), 8);
     T->custom.resize(n);
     T->blas.resize(n);
     for (size_t i = 0; i < n; i++) {
         T->custom[i] = instances[i].custom_index;
         T->blas[i] = instances[i].blas;
     }
+    // stream-ordered allocations on the build stream; the instance table lives with the
+    // TLAS and is freed in stream order by gpurt_tlas_destroy
     float* vd = nullptr;
     uint32_t* id = nullptr;
-    if (hipMalloc((void**)&vd, sizeof(float) * verts.size()) != hipSuccess ||
-        hipMalloc((void**)&id, sizeof(uint32_t) * idx.size()) != hipSuccess ||
-        hipMalloc((void**)&T->inst_d, sizeof(InstDev) * n) != hipSuccess) {
-        (void)hipGetLastError();
-        if (vd) (void)hipFree(vd);
-        if (id) (void)hipFree(id);
-        delete T;
-        return set_error(GPURT_E_OOM, "TLAS allocation failed");
-    }
     rc = GPURT_OK;
-    if (hipMemcpy(vd, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(id, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(T->inst_d, dev.data(), sizeof(InstDev) * n, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMallocAsync((void**)&vd, sizeof(float) * verts.size(), s) != hipSuccess ||
+        hipMallocAsync((void**)&id, sizeof(uint32_t) * idx.size(), s) != hipSuccess ||
+        hipMallocAsync((void**)&T->inst_d, sizeof(InstDev) * n, s) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = set_error(GPURT_E_OOM, "TLAS allocation failed");
+    } else if (hipMemcpyAsync(vd, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(id, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(T->inst_d, dev.data(), sizeof(InstDev) * n, hipMemcpyHostToDevice, s) != hipSuccess) {
         rc = hip_error(hipGetLastError(), "TLAS upload");
+    }
     if (rc == GPURT_OK) {
         gpurt_build_opts to{};
         to.node_width = W;
@@ -689,11 +703,17 @@ int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n, c
         to.node_format = q8 ? kFormatQ8 : kFormatF32;
         rc = gpurt_bvh_build(ctx, vd, 12, 3 * n, id, n, &to, stream, &T->top);
     }
-    (void)hipFree(vd);
-    (void)hipFree(id);
+    if (vd) (void)hipFreeAsync(vd, s);
+    if (id) (void)hipFreeAsync(id, s);
+    // the host arrays above feed asynchronous copies: wait for them on every path
+    if (hipStreamSynchronize(s) != hipSuccess && rc == GPURT_OK) rc = hip_error(hipGetLastError(), "TLAS build");
     if (rc) {
         delete T;
         return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->live_mu);
+        ctx->tlases.insert(T);
     }
     *out = T;
     return GPURT_OK;
@@ -701,8 +721,36 @@ int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n, c
 
 void gpurt_tlas_destroy(gpurt_tlas t) {
     if (!t) return;
+    if (!t->ctx) {  // its context was destroyed first and released its device memory
+        delete t;
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(t->ctx->live_mu);
+        t->ctx->tlases.erase(t);
+    }
     TlasDevGuard dg(t->ctx->device);
-    (void)hipDeviceSynchronize();
+    // The TLAS's queries read its BLASes: the release stream waits for them and then
+    // counts as a use of every BLAS, so a BLAS destroyed afterwards is freed after them.
+    if (t->top) {
+        hipStream_t rs = t->ctx->release;
+        wait_uses(t->top, rs);
+        bool lost;
+        {
+            std::lock_guard<std::mutex> lk(t->top->use_mu);
+            lost = t->top->uses_lost;
+        }
+        std::vector<gpurt_bvh_s*> seen;
+        for (gpurt_bvh_s* b : t->blas) {
+            if (std::find(seen.begin(), seen.end(), b) != seen.end()) continue;
+            seen.push_back(b);
+            note_use(b, rs);
+            if (lost) {
+                std::lock_guard<std::mutex> lk(b->use_mu);
+                b->uses_lost = true;
+            }
+        }
+    }
     delete t;
 }
 
@@ -749,6 +797,16 @@ size_t tlas_scratch_bytes(gpurt_tlas T, size_t n) {
     return bytes;
 }
 
+void tlas_note_use(gpurt_tlas t, hipStream_t s) {
+    if (t && t->top) note_use(t->top, s);
+}
+
+void tlas_orphan(gpurt_tlas t) {
+    if (t->inst_d) (void)hipFreeAsync(t->inst_d, t->ctx->release);
+    t->inst_d = nullptr;
+    t->ctx = nullptr;
+}
+
 int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t* n_dev, int32_t* obj,
                   int32_t* prim, float* t, float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts,
                   hipStream_t stream, void* scratch, size_t scratch_bytes) {
@@ -793,6 +851,7 @@ int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t*
     a.stats = opts ? (unsigned long long*)opts->stats : nullptr;
     a.err = T->ctx->err;
     a.leaf_trigger = p.leaf_trigger;
+    a.origin_bound = T->origin_bound;
     const bool stats = a.stats != nullptr;
     bool launched = false;
     switch (W * 4 + (T->q8 ? 2 : 0) + (stats ? 1 : 0)) {
@@ -823,7 +882,12 @@ extern "C" {
 
 static int tlas_trace(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t,
                       float* uv, uint8_t* occ, bool any, const gpurt_query_opts* opts, void* stream) {
-    return tlas_trace_ex(T, rays, n, nullptr, obj, prim, t, uv, occ, any, opts, (hipStream_t)stream, nullptr, 0);
+    const int rc = tlas_trace_ex(T, rays, n, nullptr, obj, prim, t, uv, occ, any, opts, (hipStream_t)stream, nullptr, 0);
+    if (T && n) {
+        TlasDevGuard dg(T->ctx->device);
+        note_use(T->top, (hipStream_t)stream);
+    }
+    return rc;
 }
 
 int gpurt_tlas_trace_closest(gpurt_tlas T, const gpurt_ray* rays, size_t n, int32_t* obj, int32_t* prim, float* t,

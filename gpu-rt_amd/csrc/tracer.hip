@@ -678,6 +678,7 @@ struct Lane {
 
 struct gpurt_scene_s {
     gpurt_ctx ctx = nullptr;
+    int device = 0;  // the context's device (destroy does not read the context)
     gpurt_tlas tlas = nullptr;
     size_t n_objs = 0;
     uint32_t n_lights = 0, n_ltris = 0;
@@ -1026,6 +1027,7 @@ int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_ob
     hipStream_t s = (hipStream_t)stream;
     auto* S = new gpurt_scene_s();
     S->ctx = ctx;
+    S->device = ctx->device;
     S->tlas = tlas;
     S->n_objs = n_objects;
     S->n_lights = (uint32_t)lights.size();
@@ -1107,7 +1109,7 @@ int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_ob
 
 void gpurt_scene_destroy(gpurt_scene scene) {
     if (!scene) return;
-    DevScope dg(scene->ctx->device);
+    DevScope dg(scene->device);
     (void)hipDeviceSynchronize();
     delete scene;
 }
@@ -1242,6 +1244,7 @@ int gpurt_render(gpurt_scene S, const gpurt_render_params* prm, float* image, fl
         if (hipEventRecord(L.end, L.st) != hipSuccess && rc == GPURT_OK) rc = hip_error(hipGetLastError(), "join");
     }
     for (int g = 0; g < G; g++) (void)hipStreamWaitEvent(s, S->lanes[g].end, 0);
+    tlas_note_use(S->tlas, s);  // every traversal of the frame is ordered before this point
     if (rc) return rc;
     hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, kBlock)), dim3(kBlock), 0, s, P, (uint32_t)r.samples, (uint32_t)r.frame,
                        w.pix, (float4*)image);

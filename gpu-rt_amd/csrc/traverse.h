@@ -495,6 +495,23 @@ __device__ __forceinline__ uint32_t pair_donors(bool idle, bool donor, bool& tak
     return take ? nth_set_bit(D, ri) : __lane_id();
 }
 
+// Inputs that cannot be answered.  A ray with a non-finite origin or direction component,
+// a zero direction or a NaN bound is a miss: its interval is emptied (tmin = +inf, tmax =
+// -inf), so no box and no triangle passes the interval tests and the traversal ends
+// after the root.  A query point with a non-finite coordinate finds no triangle (its
+// cull bound is -inf), like a query on an empty mesh.  Without this, NaN entries would
+// pass the fmaxf / fminf box tests (they drop NaN operands) and visit every node.
+__device__ __forceinline__ void empty_if_invalid_ray(float4 r0, float4 r1, float& tmin, float& tmax) {
+    const bool ok = isfinite(r0.x) && isfinite(r0.y) && isfinite(r0.z) && isfinite(r1.x) && isfinite(r1.y) &&
+                    isfinite(r1.z) && (r1.x != 0.0f || r1.y != 0.0f || r1.z != 0.0f) && !isnan(r0.w) &&
+                    !isnan(r1.w);
+    if (!ok) {
+        tmin = INFINITY;
+        tmax = -INFINITY;
+    }
+}
+__device__ __forceinline__ bool finite_point(V3 q) { return isfinite(q.x) && isfinite(q.y) && isfinite(q.z); }
+
 // (value, id) as one integer that orders like the pair: the float's bits mapped to an
 // order-preserving integer (negative values have their bits inverted, the rest get the
 // sign bit set), with -0 first turned into +0 since the traversal treats them as equal
@@ -620,6 +637,7 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
                     d = v3(r1.x, r1.y, r1.z);
                     tmin = r0.w;
                     best = r1.w;
+                    empty_if_invalid_ray(r0, r1, tmin, best);
                     const float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
                     const float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
                     const float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
@@ -816,7 +834,7 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
         if (!drained) {
             if (src.refill(has, h)) {
                 q = src.point(h);
-                best = thr = INFINITY;
+                best = thr = finite_point(q) ? INFINITY : -INFINITY;
                 bid = 0xFFFFFFFFu;
                 st.sp = 0;
                 cur = src.root(h);

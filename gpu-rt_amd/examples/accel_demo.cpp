@@ -2,8 +2,11 @@
 // reference's 48-byte vertices, an Accel built from it, closest-hit / any-hit rays and
 // closest-point queries with known answers.  Prints "accel_demo ok" and exits 0 when
 // every answer matches, else prints the first mismatch and exits 1.
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/gpurt.hpp"
@@ -85,7 +88,48 @@ int main() {
             threw = e.code() == GPURT_E_UNSUPPORTED;
         }
         if (!threw) return fail("error path", 0);
-        std::printf("accel_demo ok\n");
+
+        // autotuner: a small space over these rays and queries, stored in a cache file and
+        // read back; a BVH built with the tuned options gives the same answers
+        char cache[] = "/tmp/gpurt_tune_XXXXXX";
+        const int fd = mkstemp(cache);
+        if (fd < 0) return fail("cache file", 0);
+        close(fd);
+        const int32_t widths[] = {2, 4}, stacks[] = {8, 16}, formats[] = {1, 2};
+        gpurt_tune_space sp{};
+        sp.node_width = widths;
+        sp.n_node_width = 2;
+        sp.stack_depth = stacks;
+        sp.n_stack_depth = 2;
+        sp.node_format = formats;
+        sp.n_node_format = 2;
+        const int32_t zero[] = {0};
+        sp.treelet_size = zero;
+        sp.n_treelet_size = 1;
+        sp.waves_per_cu = zero;
+        sp.n_waves_per_cu = 1;
+        sp.cache_path = cache;
+        for (auto& r : rays) r.tmax = 1e7f;
+        d_rays.upload(rays.data(), n);
+        const gpurt_tune_result tuned = gpurt::tune_queries(ctx, mesh, d_q.data(), 2, d_rays.data(), n, &sp);
+        if (tuned.candidates != 8 || tuned.from_cache) return fail("tuner candidates", tuned.candidates);
+        gpurt_tune_result loaded;
+        if (!gpurt::load_tuned(cache, tuned.key, loaded)) return fail("tuner cache load", 0);
+        if (loaded.build.node_width != tuned.build.node_width || loaded.query.stack_depth != tuned.query.stack_depth ||
+            loaded.build.node_format != tuned.build.node_format || !loaded.from_cache)
+            return fail("tuner cache contents", 0);
+        const gpurt_tune_result again = gpurt::tune_queries(ctx, mesh, d_q.data(), 2, d_rays.data(), n, &sp);
+        if (!again.from_cache || again.candidates != 0) return fail("tuner cache hit", 0);
+        std::remove(cache);
+        gpurt::Accel tuned_accel(ctx, mesh, &loaded.build);
+        tuned_accel.trace_closest(d_rays.data(), n, prim.data(), t.data(), uv.data(), &loaded.query);
+        gpurt::check_hip(hipDeviceSynchronize(), "sync");
+        prim.download(hp.data());
+        for (int i = 0; i < n; i++)
+            if (hp[i] != expect_prim[i]) return fail("tuned closest-hit prim", i);
+        std::printf("accel_demo ok (tuned: width %d, format %d, stack %d; %.3f ms vs %.3f ms default)\n",
+                    loaded.build.node_width, loaded.build.node_format, loaded.query.stack_depth, tuned.best_ms,
+                    tuned.default_ms);
         return 0;
     } catch (const std::exception& e) {
         std::printf("accel_demo exception: %s\n", e.what());

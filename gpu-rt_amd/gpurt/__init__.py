@@ -11,7 +11,9 @@ memory and the current HIP stream; every computation runs in the library's HIP k
     d = gpurt.closest_point_distance(bvh, q, verts)   # differentiable
     t = gpurt.ray_hit_t(bvh, rays_o, rays_d, verts)   # differentiable
 """
+import atexit
 import ctypes
+import weakref
 
 import torch
 
@@ -23,6 +25,26 @@ __all__ = ["Context", "Bvh", "Batch", "GpurtError", "closest_point_distance", "r
 
 EPS = 1e-5          # rtcommon.glsl:7 -- ray tmin of trace_ray / visibility
 LARGE_DIST = 1e7    # rtcommon.glsl:6 -- ray tmax of trace_ray
+
+
+# Every open library object, by kind.  At interpreter exit they are closed in dependency
+# order (scenes, batches, TLASes, BVHs, then contexts) before torch and the HIP runtime
+# shut down, instead of in whatever order garbage collection reaches them.
+_OPEN = {k: weakref.WeakSet() for k in ("scene", "batch", "tlas", "bvh", "ctx")}
+
+
+def _track(kind, obj):
+    _OPEN[kind].add(obj)
+
+
+@atexit.register
+def _close_all():
+    for kind in ("scene", "batch", "tlas", "bvh", "ctx"):
+        for obj in list(_OPEN[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 def _stream(device):
@@ -48,6 +70,17 @@ def _need(t, dtype, shape_tail, name, device):
     return t
 
 
+def _check_out(n, device, **arrays):
+    """Caller-supplied output tensors: n rows of the right dtype and width, contiguous, on
+    the device (None skips an optional output)."""
+    for name, (t, dtype, tail) in arrays.items():
+        if t is None:
+            continue
+        _need(t, dtype, tail, name, device)
+        if t.shape[0] != n:
+            raise ValueError(f"{name} has {t.shape[0]} rows for {n} queries")
+
+
 class Context:
     """Library context on one HIP device (gpurt_ctx_create)."""
 
@@ -59,10 +92,20 @@ class Context:
         h = ctypes.c_void_p()
         _lib.check(lib.gpurt_ctx_create(int(device), ctypes.byref(h)))
         self._h = h
+        _track("ctx", self)
 
     @property
     def handle(self):
         return self._h
+
+    def status(self, clear=True):
+        """Sticky conditions the kernels reported (gpurt_ctx_status): a set of
+        "stack_overflow", "tlas_outside_domain"; waits for the device."""
+        f = ctypes.c_uint32()
+        _lib.check(_lib.load().gpurt_ctx_status(self._h, ctypes.byref(f), int(bool(clear))))
+        names = {_lib.STATUS_STACK_OVERFLOW: "stack_overflow",
+                 _lib.STATUS_TLAS_OUTSIDE_DOMAIN: "tlas_outside_domain"}
+        return {n for bit, n in names.items() if f.value & bit}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -124,6 +167,7 @@ class Bvh:
                                        verts.shape[0], _ptr(tris), tris.shape[0],
                                        ctypes.byref(opts), _stream(dev), ctypes.byref(h)))
         self._h = h
+        _track("bvh", self)
 
     @property
     def handle(self):
@@ -137,6 +181,23 @@ class Bvh:
         inf = BvhInfo()
         _lib.check(_lib.load().gpurt_bvh_get_info(self._h, ctypes.byref(inf)))
         return {f: getattr(inf, f) for f, _ in BvhInfo._fields_}
+
+    def export(self):
+        """The built tree (gpurt_bvh_export) as device tensors: "nodes" (n_nodes, 8*W)
+        float32 FP32 wide nodes (child words and counts are the bit patterns of the last
+        2*W columns), "qnodes" (n_nodes, bytes) uint8 8-bit nodes or None, "tris"
+        (n_tris, 12) packed triangles, "leaf_order" (n_tris,) int32."""
+        inf = self.info()
+        W, nn, nt = inf["node_width"], inf["n_nodes"], inf["n_tris"]
+        dev = self.device
+        out = {"nodes": torch.empty((nn, 8 * W), dtype=torch.float32, device=dev),
+               "qnodes": (torch.empty((nn, {2: 48, 4: 64, 8: 96}[W]), dtype=torch.uint8, device=dev)
+                          if inf["node_format"] in (2, 3) else None),
+               "tris": torch.empty((nt, 12), dtype=torch.float32, device=dev),
+               "leaf_order": torch.empty(nt, dtype=torch.int32, device=dev)}
+        arr = _lib.BvhArrays(*(_ptr(out[k]) for k in ("nodes", "qnodes", "tris", "leaf_order")))
+        _lib.check(_lib.load().gpurt_bvh_export(self._h, ctypes.byref(arr), _stream(dev)))
+        return out
 
     def refit(self, verts):
         _need(verts, torch.float32, verts.shape[1:], "verts", self.device)
@@ -170,6 +231,8 @@ class Bvh:
             bary = torch.empty((n, 3), dtype=torch.float32, device=dev) if with_bary else None
         else:
             prim, pt, dist, bary = out
+            _check_out(n, dev, prim=(prim, torch.int32, ()), pt=(pt, torch.float32, (3,)),
+                       dist=(dist, torch.float32, ()), bary=(bary, torch.float32, (3,)))
         opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger)
         _lib.check(_lib.load().gpurt_closest_point(self._h, _ptr(q), n, _ptr(prim), _ptr(pt),
                                                    _ptr(dist), _ptr(bary), ctypes.byref(opts),
@@ -187,6 +250,8 @@ class Bvh:
             uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
         else:
             prim, t, uv = out
+            _check_out(n, dev, prim=(prim, torch.int32, ()), t=(t, torch.float32, ()),
+                       uv=(uv, torch.float32, (2,)))
         opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_trace_closest(self._h, _ptr(rays), n, _ptr(prim), _ptr(t),
                                                    _ptr(uv), ctypes.byref(opts), _stream(dev)))
@@ -198,19 +263,46 @@ class Bvh:
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
         occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
+        _check_out(n, dev, occluded=(occ, torch.uint8, ()))
         opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_trace_any(self._h, _ptr(rays), n, _ptr(occ),
                                                ctypes.byref(opts), _stream(dev)))
         return occ
 
+    def _check_bwd(self, n, per_query, g_verts):
+        """Shapes, dtypes and devices of a backward call's tensors: the kernels index
+        g_verts by this BVH's vertex indices and the per-query arrays by query."""
+        dev = self.device
+        _check_out(n, dev, **per_query)
+        if g_verts is not None:
+            _need(g_verts, torch.float32, (3,), "g_verts", dev)
+            if g_verts.shape[0] != self.n_verts:
+                raise ValueError(f"g_verts must be ({self.n_verts}, 3) for this BVH, "
+                                 f"got {tuple(g_verts.shape)}")
+
     def closest_point_bwd(self, q, prim, bary, pt, dist, g_dist, g_q=None, g_verts=None):
+        """Accumulates g_verts (n_verts, 3) and writes g_q (n, 3) from the forward pass's
+        saved ids (gpurt_closest_point_bwd)."""
+        _need(q, torch.float32, (3,), "q", self.device)
+        n = q.shape[0]
+        self._check_bwd(n, {"prim": (prim, torch.int32, ()), "g_dist": (g_dist, torch.float32, ()),
+                            "g_q": (g_q, torch.float32, (3,)), "bary": (bary, torch.float32, (3,)),
+                            "pt": (pt, torch.float32, (3,)), "dist": (dist, torch.float32, ())}, g_verts)
         _lib.check(_lib.load().gpurt_closest_point_bwd(
-            self._h, _ptr(q), q.shape[0], _ptr(prim), _ptr(bary), _ptr(pt), _ptr(dist),
+            self._h, _ptr(q), n, _ptr(prim), _ptr(bary), _ptr(pt), _ptr(dist),
             _ptr(g_dist), _ptr(g_q), _ptr(g_verts), _stream(self.device)))
 
     def trace_bwd(self, rays, prim, t, uv, g_t, g_origin=None, g_dir=None, g_verts=None):
+        """Accumulates g_verts (n_verts, 3) and writes g_origin / g_dir (n, 3) from the
+        forward pass's saved ids, t and (u, v) (gpurt_trace_bwd)."""
+        _need(rays, torch.float32, (8,), "rays", self.device)
+        n = rays.shape[0]
+        self._check_bwd(n, {"prim": (prim, torch.int32, ()), "t": (t, torch.float32, ()),
+                            "uv": (uv, torch.float32, (2,)), "g_t": (g_t, torch.float32, ()),
+                            "g_origin": (g_origin, torch.float32, (3,)),
+                            "g_dir": (g_dir, torch.float32, (3,))}, g_verts)
         _lib.check(_lib.load().gpurt_trace_bwd(
-            self._h, _ptr(rays), rays.shape[0], _ptr(prim), _ptr(t), _ptr(uv), _ptr(g_t),
+            self._h, _ptr(rays), n, _ptr(prim), _ptr(t), _ptr(uv), _ptr(g_t),
             _ptr(g_origin), _ptr(g_dir), _ptr(g_verts), _stream(self.device)))
 
     def close(self):
@@ -286,6 +378,7 @@ class Batch:
         _lib.check(lib.gpurt_batch_create(ctx.handle, arr, len(meshes), ctypes.byref(bo),
                                           ctypes.byref(qo), ctypes.byref(h)))
         self._h = h
+        _track("batch", self)
 
     def run(self):
         _lib.check(_lib.load().gpurt_batch_run(self._h, _stream(self.ctx.device)))
@@ -373,6 +466,7 @@ class Tlas:
         _lib.check(lib.gpurt_tlas_build(ctx.handle, arr, len(instances), ctypes.byref(opts),
                                         _stream(ctx.device), ctypes.byref(h)))
         self._h = h
+        _track("tlas", self)
 
     @property
     def device(self):
@@ -398,6 +492,8 @@ class Tlas:
             uv = torch.empty((n, 2), dtype=torch.float32, device=dev) if with_uv else None
         else:
             obj, prim, t, uv = out
+            _check_out(n, dev, obj=(obj, torch.int32, ()), prim=(prim, torch.int32, ()),
+                       t=(t, torch.float32, ()), uv=(uv, torch.float32, (2,)))
         opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_tlas_trace_closest(self._h, _ptr(rays), n, _ptr(obj), _ptr(prim),
                                                         _ptr(t), _ptr(uv), ctypes.byref(opts),
@@ -410,6 +506,7 @@ class Tlas:
         _need(rays, torch.float32, (8,), "rays", dev)
         n = rays.shape[0]
         occ = out if out is not None else torch.empty(n, dtype=torch.uint8, device=dev)
+        _check_out(n, dev, occluded=(occ, torch.uint8, ()))
         opts = self._qopts(stack_depth, waves_per_cu, stats, leaf_trigger, refill_trigger)
         _lib.check(_lib.load().gpurt_tlas_trace_any(self._h, _ptr(rays), n, _ptr(occ),
                                                     ctypes.byref(opts), _stream(dev)))
@@ -427,15 +524,34 @@ class Tlas:
             pass
 
 
+def _check_verts(bvh, verts):
+    """The differentiable wrappers return a gradient for `verts`, which must be the vertex
+    array the BVH was built on (or one of the same shape, e.g. after a refit)."""
+    if verts.dim() != 2 or verts.shape[1] < 3 or verts.shape[0] != bvh.n_verts:
+        raise ValueError(f"verts must be ({bvh.n_verts}, k>=3) like the BVH's vertex array, "
+                         f"got {tuple(verts.shape)}")
+
+
+def _vertex_grad(g_v, shape):
+    """The (n_verts, 3) position gradient laid out like verts: columns 0..2 of a zero
+    (n_verts, k) tensor for the 16- and 48-byte vertex records."""
+    if g_v is None or shape[1] == 3:
+        return g_v
+    out = torch.zeros(tuple(shape), dtype=g_v.dtype, device=g_v.device)
+    out[:, :3] = g_v
+    return out
+
+
 class _ClosestPointDistance(torch.autograd.Function):
     """d_i = min over triangles |q_i - p| ; backward via gpurt_closest_point_bwd."""
 
     @staticmethod
     def forward(ctx, bvh, q, verts):
+        _check_verts(bvh, verts)
         q = q.contiguous()
         prim, pt, dist, bary = bvh.closest_point(q)
         ctx.bvh = bvh
-        ctx.n_verts = verts.shape[0]
+        ctx.verts_shape = verts.shape
         ctx.save_for_backward(q, prim, pt, dist, bary)
         ctx.mark_non_differentiable(prim)
         return dist, prim
@@ -447,9 +563,9 @@ class _ClosestPointDistance(torch.autograd.Function):
         need_q, need_v = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
         g_dist = g_dist.contiguous().float()
         g_q = torch.empty_like(q) if need_q else None
-        g_v = torch.zeros((ctx.n_verts, 3), dtype=torch.float32, device=q.device) if need_v else None
+        g_v = torch.zeros((bvh.n_verts, 3), dtype=torch.float32, device=q.device) if need_v else None
         bvh.closest_point_bwd(q, prim, bary, pt, dist, g_dist, g_q, g_v)
-        return None, g_q, g_v
+        return None, g_q, _vertex_grad(g_v, ctx.verts_shape)
 
 
 class _RayHitT(torch.autograd.Function):
@@ -457,10 +573,11 @@ class _RayHitT(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, bvh, o, d, verts, tmin, tmax):
+        _check_verts(bvh, verts)
         rays = make_rays(o, d, tmin, tmax)
         prim, t, uv = bvh.trace_closest(rays)
         ctx.bvh = bvh
-        ctx.n_verts = verts.shape[0]
+        ctx.verts_shape = verts.shape
         ctx.save_for_backward(rays, prim, t, uv)
         ctx.mark_non_differentiable(prim)
         return t, prim
@@ -474,9 +591,9 @@ class _RayHitT(torch.autograd.Function):
         g_t = g_t.contiguous().float()
         g_o = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if need_o else None
         g_d = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if need_d else None
-        g_v = torch.zeros((ctx.n_verts, 3), dtype=torch.float32, device=rays.device) if need_v else None
+        g_v = torch.zeros((bvh.n_verts, 3), dtype=torch.float32, device=rays.device) if need_v else None
         bvh.trace_bwd(rays, prim, t, uv, g_t, g_o, g_d, g_v)
-        return None, g_o, g_d, g_v, None, None
+        return None, g_o, g_d, _vertex_grad(g_v, ctx.verts_shape), None, None
 
 
 def closest_point_distance(bvh, q, verts):

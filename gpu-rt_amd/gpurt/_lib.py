@@ -15,6 +15,8 @@ GPURT_E_OOM = -2
 GPURT_E_STACK_OVERFLOW = -3
 GPURT_E_DEVICE = -4
 GPURT_E_UNSUPPORTED = -5
+STATUS_STACK_OVERFLOW = 1
+STATUS_TLAS_OUTSIDE_DOMAIN = 2
 
 _ERR_NAMES = {
     GPURT_E_INVALID: "GPURT_E_INVALID",
@@ -26,11 +28,13 @@ _ERR_NAMES = {
 
 # Every symbol include/gpurt.h declares.
 EXPORTS = (
-    "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_bvh_build", "gpurt_bvh_refit",
-    "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_closest_point", "gpurt_trace_closest",
+    "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_ctx_status", "gpurt_bvh_build", "gpurt_bvh_refit",
+    "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_bvh_export", "gpurt_closest_point",
+    "gpurt_trace_closest",
     "gpurt_trace_closest_noalloc", "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd",
     "gpurt_batch_create",
     "gpurt_batch_run", "gpurt_batch_set_grads", "gpurt_batch_run_backward", "gpurt_batch_destroy",
+    "gpurt_tune_batch", "gpurt_tune_queries", "gpurt_tune_cache_load", "gpurt_tune_cache_store",
     "gpurt_tlas_build", "gpurt_tlas_destroy", "gpurt_tlas_trace_closest", "gpurt_tlas_trace_any",
     "gpurt_scene_create", "gpurt_scene_destroy", "gpurt_render", "gpurt_scene_light_pdf",
     "gpurt_last_error",
@@ -64,6 +68,11 @@ class BvhInfo(ctypes.Structure):
                 ("cpq_node_bytes", ctypes.c_int32)]
 
 
+class BvhArrays(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("qnodes", ctypes.c_void_p), ("tris", ctypes.c_void_p),
+                ("leaf_order", ctypes.c_void_p)]
+
+
 class BatchMesh(ctypes.Structure):
     _fields_ = [("verts", ctypes.c_void_p), ("vstride_bytes", ctypes.c_size_t),
                 ("n_verts", ctypes.c_size_t), ("tri_idx", ctypes.c_void_p),
@@ -73,6 +82,22 @@ class BatchMesh(ctypes.Structure):
                 ("rays", ctypes.c_void_p), ("n_r", ctypes.c_size_t),
                 ("r_prim", ctypes.c_void_p), ("r_t", ctypes.c_void_p),
                 ("r_uv", ctypes.c_void_p)]
+
+
+class TuneSpace(ctypes.Structure):
+    _fields_ = [("node_width", ctypes.c_void_p), ("n_node_width", ctypes.c_int32),
+                ("stack_depth", ctypes.c_void_p), ("n_stack_depth", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_void_p), ("n_waves_per_cu", ctypes.c_int32),
+                ("treelet_size", ctypes.c_void_p), ("n_treelet_size", ctypes.c_int32),
+                ("node_format", ctypes.c_void_p), ("n_node_format", ctypes.c_int32),
+                ("reps", ctypes.c_int32), ("builds_per_run", ctypes.c_float),
+                ("cache_path", ctypes.c_char_p)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("build", BuildOpts), ("query", QueryOpts), ("best_ms", ctypes.c_float),
+                ("default_ms", ctypes.c_float), ("candidates", ctypes.c_int32),
+                ("from_cache", ctypes.c_int32), ("key", ctypes.c_char * 128)]
 
 
 class Instance(ctypes.Structure):
@@ -118,10 +143,12 @@ def load():
     sig = {
         "gpurt_ctx_create": (I, [I, ctypes.POINTER(P)]),
         "gpurt_ctx_destroy": (V, [P]),
+        "gpurt_ctx_status": (I, [P, ctypes.POINTER(ctypes.c_uint32), I]),
         "gpurt_bvh_build": (I, [P, P, S, S, P, S, ctypes.POINTER(BuildOpts), P, ctypes.POINTER(P)]),
         "gpurt_bvh_refit": (I, [P, P, S, P]),
         "gpurt_bvh_destroy": (V, [P]),
         "gpurt_bvh_get_info": (I, [P, ctypes.POINTER(BvhInfo)]),
+        "gpurt_bvh_export": (I, [P, ctypes.POINTER(BvhArrays), P]),
         "gpurt_closest_point": (I, [P, P, S, P, P, P, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_trace_closest": (I, [P, P, S, P, P, P, ctypes.POINTER(QueryOpts), P]),
         "gpurt_trace_closest_noalloc": (I, [P, P, S, P, P, P, ctypes.POINTER(QueryOpts), P]),
@@ -133,6 +160,12 @@ def load():
         "gpurt_batch_run": (I, [P, P]),
         "gpurt_batch_set_grads": (I, [P, ctypes.POINTER(BatchGrad)]),
         "gpurt_batch_run_backward": (I, [P, P]),
+        "gpurt_tune_batch": (I, [P, ctypes.POINTER(BatchMesh), S, ctypes.POINTER(TuneSpace), P,
+                                 ctypes.POINTER(TuneResult)]),
+        "gpurt_tune_queries": (I, [P, P, S, S, P, S, P, S, P, S, ctypes.POINTER(TuneSpace), P,
+                                   ctypes.POINTER(TuneResult)]),
+        "gpurt_tune_cache_load": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(TuneResult)]),
+        "gpurt_tune_cache_store": (I, [ctypes.c_char_p, ctypes.POINTER(TuneResult)]),
         "gpurt_tlas_build": (I, [P, ctypes.POINTER(Instance), S, ctypes.POINTER(BuildOpts), P,
                                  ctypes.POINTER(P)]),
         "gpurt_tlas_destroy": (V, [P]),

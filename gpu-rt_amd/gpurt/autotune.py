@@ -1,12 +1,13 @@
-"""Autotuner for the traversal and build parameters.
+"""Autotuner for the traversal and build parameters (Python face of gpurt_tune_queries /
+gpurt_tune_batch, include/gpurt.h).
 
-Sweeps node width {2, 4, 8}, node encoding (FP32 or 8-bit child boxes), LDS stack depth
-{8, 16, 32, 64}, resident waves per CU and treelet size over a sample of the caller's own queries, times every candidate with HIP
-events on the current stream, and keeps the fastest.  Answers are independent of every
-parameter (the traversal is exact for any tree), so only speed is compared.  Winners
-are cached in a JSON file keyed by the device name and a coarse signature of the
-workload (log2 buckets of triangle and query counts, query kinds), so a second call with
-a similar workload costs nothing.
+The library sweeps node width, node encoding (FP32 or 8-bit child boxes), LDS stack
+depth, resident waves per CU and treelet size over the caller's own workload, times every
+candidate with HIP events on the current stream and keeps the fastest; the library
+defaults are timed beside it.  Answers are independent of every parameter (the traversal
+is exact for any tree), so only speed is compared.  Winners are kept in a text cache
+keyed by the device and log2 buckets of the workload's sizes, the same file C++ callers
+read with gpurt::load_tuned, so a second call with a similar workload costs nothing.
 
     best = autotune.tune_queries(ctx, verts, tris, queries=q, rays=r)
     bvh = gpurt.Bvh(ctx, verts, tris, **best["build"])
@@ -15,12 +16,12 @@ a similar workload costs nothing.
     best = autotune.tune_batch(ctx, meshes)
     batch = gpurt.Batch(ctx, meshes, **best["build"], **best["query"])
 """
-import itertools
-import json
-import math
+import ctypes
 import os
 
 import torch
+
+from . import _lib
 
 DEFAULT_SPACE = {
     "node_width": (2, 4, 8),
@@ -33,53 +34,51 @@ DEFAULT_SPACE = {
 
 def cache_path():
     return os.environ.get("GPURT_AUTOTUNE_CACHE",
-                          os.path.join(os.path.expanduser("~"), ".cache", "gpurt", "autotune.json"))
+                          os.path.join(os.path.expanduser("~"), ".cache", "gpurt", "autotune.txt"))
 
 
-def _bucket(n):
-    return int(math.floor(math.log2(max(1, int(n)))))
+def _space(space, path, reps, builds_per_run=0.0):
+    sp = dict(DEFAULT_SPACE, **(space or {}))
+    keep = []
+    s = _lib.TuneSpace()
+    for k in DEFAULT_SPACE:
+        arr = (ctypes.c_int32 * len(sp[k]))(*sp[k])
+        keep.append(arr)
+        setattr(s, k, ctypes.cast(arr, ctypes.c_void_p))
+        setattr(s, "n_" + k, len(sp[k]))
+    s.reps = int(reps)
+    s.builds_per_run = float(builds_per_run)
+    if path:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        s.cache_path = os.path.abspath(path).encode()
+    return s, keep
 
 
-def signature(device_name, n_tris, n_queries, n_rays, kind, objective):
-    """Cache key: device, log2 size buckets, query kinds and objective."""
-    return "|".join([device_name, kind, f"t{_bucket(n_tris)}", f"q{_bucket(n_queries)}",
-                     f"r{_bucket(n_rays)}", objective])
+def _result(r):
+    return {"build": {"node_width": r.build.node_width, "treelet_size": r.build.treelet_size,
+                      "node_format": r.build.node_format},
+            "query": {"stack_depth": r.query.stack_depth, "waves_per_cu": r.query.waves_per_cu},
+            "objective_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates,
+            "key": r.key.decode(), "cached": bool(r.from_cache)}
 
 
-def load_cache(path=None):
-    path = path or cache_path()
-    try:
-        with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
-        return {}
+def load(key, path=None):
+    """The cached result for `key`, or None (gpurt_tune_cache_load)."""
+    r = _lib.TuneResult()
+    rc = _lib.load().gpurt_tune_cache_load((path or cache_path()).encode(), key.encode(), ctypes.byref(r))
+    return _result(r) if rc == _lib.GPURT_OK else None
 
 
-def store_cache(key, value, path=None):
-    path = path or cache_path()
-    data = load_cache(path)
-    data[key] = value
-    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
-    tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(data, f, indent=1, sort_keys=True)
-    os.replace(tmp, path)
-
-
-def _time_ms(fn, reps=5):
-    fn()  # warm-up (first launch also allocates stream-ordered scratch)
-    torch.cuda.synchronize()
-    times = []
-    for _ in range(reps):
-        s = torch.cuda.Event(enable_timing=True)
-        e = torch.cuda.Event(enable_timing=True)
-        s.record()
-        fn()
-        e.record()
-        torch.cuda.synchronize()
-        times.append(s.elapsed_time(e))
-    times.sort()
-    return times[len(times) // 2]
+def store(result, path=None):
+    """Append a result (a dict as returned by the tuners) to the cache."""
+    r = _lib.TuneResult()
+    b, q = result["build"], result["query"]
+    r.build.node_width, r.build.treelet_size, r.build.node_format = (b["node_width"], b["treelet_size"],
+                                                                      b["node_format"])
+    r.query.stack_depth, r.query.waves_per_cu = q["stack_depth"], q["waves_per_cu"]
+    r.best_ms, r.default_ms = result["objective_ms"], result.get("default_ms", 0.0)
+    r.key = result["key"].encode()
+    _lib.check(_lib.load().gpurt_tune_cache_store((path or cache_path()).encode(), ctypes.byref(r)))
 
 
 def _sample(t, n):
@@ -90,85 +89,38 @@ def _sample(t, n):
 
 
 def tune_queries(ctx, verts, tris, queries=None, rays=None, space=None, sample=262144,
-                 builds_per_call=0.0, use_cache=True, path=None):
-    """Pick build and query parameters for one mesh.
+                 builds_per_call=0.0, use_cache=True, path=None, reps=3):
+    """Pick build and query parameters for one mesh (gpurt_tune_queries).
 
     The objective is query time on `sample` of the queries and rays, plus the build time
     times `builds_per_call` (0 = the BVH is built once and queried many times; 1 = a
     rebuild before every query batch)."""
-    import gpurt
-    space = dict(DEFAULT_SPACE, **(space or {}))
-    dev = ctx.device
-    name = torch.cuda.get_device_name(dev)
-    nq = 0 if queries is None else queries.shape[0]
-    nr = 0 if rays is None else rays.shape[0]
-    kind = ("cpq" if nq else "") + ("+" if nq and nr else "") + ("ray" if nr else "")
-    key = signature(name, tris.shape[0], nq, nr, kind, f"b{builds_per_call:g}")
-    if use_cache:
-        hit = load_cache(path).get(key)
-        if hit:
-            return dict(hit, cached=True)
     q = _sample(queries, sample)
     r = _sample(rays, sample)
-    table = []
-    for W, T, NF in itertools.product(space["node_width"], space["treelet_size"], space["node_format"]):
-        torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        bvh = gpurt.Bvh(ctx, verts, tris, node_width=W, treelet_size=T, node_format=NF)
-        ev1.record()
-        torch.cuda.synchronize()
-        build_ms = ev0.elapsed_time(ev1)
-        for S, WV in itertools.product(space["stack_depth"], space["waves_per_cu"]):
-            def run():
-                if q is not None:
-                    bvh.closest_point(q, stack_depth=S, waves_per_cu=WV)
-                if r is not None:
-                    bvh.trace_closest(r, stack_depth=S, waves_per_cu=WV)
-            ms = _time_ms(run)
-            table.append({"node_width": W, "treelet_size": T, "node_format": NF, "stack_depth": S,
-                          "waves_per_cu": WV, "query_ms": ms, "build_ms": build_ms,
-                          "objective_ms": ms + builds_per_call * build_ms})
-        bvh.close()
-    best = min(table, key=lambda e: e["objective_ms"])
-    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"],
-                        "node_format": best["node_format"]},
-              "query": {"stack_depth": best["stack_depth"], "waves_per_cu": best["waves_per_cu"]},
-              "objective_ms": best["objective_ms"], "key": key, "table": table}
-    if use_cache:
-        store_cache(key, {k: v for k, v in result.items() if k != "table"}, path)
-    return dict(result, cached=False)
+    sp, keep = _space(space, (path or cache_path()) if use_cache else None, reps, builds_per_call)
+    out = _lib.TuneResult()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(ctx.device).cuda_stream)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    _lib.check(_lib.load().gpurt_tune_queries(
+        ctx.handle, ptr(verts), verts.shape[1] * 4, verts.shape[0], ptr(tris), tris.shape[0],
+        ptr(q), 0 if q is None else q.shape[0], ptr(r), 0 if r is None else r.shape[0],
+        ctypes.byref(sp), stream, ctypes.byref(out)))
+    del keep
+    return _result(out)
 
 
 def tune_batch(ctx, meshes, space=None, use_cache=True, path=None, reps=3):
-    """Pick parameters for a batch (gpurt.Batch): every candidate is a full replay (all
-    builds and all queries), which is what the batch runs."""
+    """Pick parameters for a batch (gpurt_tune_batch): every candidate is a full replay
+    (all builds and all queries) of these meshes, which is what the batch runs.  `meshes`
+    are the dicts gpurt.Batch takes; their outputs are written by the trial replays."""
     import gpurt
-    space = dict(DEFAULT_SPACE, **(space or {}))
-    name = torch.cuda.get_device_name(ctx.device)
-    n_tris = sum(m["tris"].shape[0] for m in meshes)
-    nq = sum(0 if m.get("queries") is None else m["queries"].shape[0] for m in meshes)
-    nr = sum(0 if m.get("rays") is None else m["rays"].shape[0] for m in meshes)
-    key = signature(name, n_tris, nq, nr, f"batch{_bucket(len(meshes))}", "replay")
-    if use_cache:
-        hit = load_cache(path).get(key)
-        if hit:
-            return dict(hit, cached=True)
-    table = []
-    for W, T, NF, S, WV in itertools.product(space["node_width"], space["treelet_size"],
-                                             space["node_format"], space["stack_depth"],
-                                             space["waves_per_cu"]):
-        b = gpurt.Batch(ctx, meshes, node_width=W, treelet_size=T, stack_depth=S, waves_per_cu=WV,
-                        node_format=NF)
-        ms = _time_ms(b.run, reps)
-        b.close()
-        table.append({"node_width": W, "treelet_size": T, "node_format": NF, "stack_depth": S,
-                      "waves_per_cu": WV, "replay_ms": ms})
-    best = min(table, key=lambda e: e["replay_ms"])
-    result = {"build": {"node_width": best["node_width"], "treelet_size": best["treelet_size"],
-                        "node_format": best["node_format"]},
-              "query": {"stack_depth": best["stack_depth"], "waves_per_cu": best["waves_per_cu"]},
-              "objective_ms": best["replay_ms"], "key": key, "table": table}
-    if use_cache:
-        store_cache(key, {k: v for k, v in result.items() if k != "table"}, path)
-    return dict(result, cached=False)
+    probe = gpurt.Batch(ctx, meshes)  # validates the meshes and lays out the C array
+    sp, keep = _space(space, (path or cache_path()) if use_cache else None, reps)
+    out = _lib.TuneResult()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(ctx.device).cuda_stream)
+    _lib.check(_lib.load().gpurt_tune_batch(ctx.handle, probe._arr, len(meshes), ctypes.byref(sp), stream,
+                                            ctypes.byref(out)))
+    torch.cuda.synchronize(ctx.device)
+    probe.close()
+    del keep
+    return _result(out)

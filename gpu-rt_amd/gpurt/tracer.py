@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from . import Bvh, Tlas, _need, _ptr, _stream
+from . import Bvh, Tlas, _need, _ptr, _stream, _track
 
 INTEGRATORS = {"direct": 0, "mats": 1, "mis": 2}
 BRDFS = {"blinn-phong": 0, "ggx": 1}
@@ -108,6 +108,7 @@ class Scene:
         h = ctypes.c_void_p()
         _lib.check(lib.gpurt_scene_create(self.tlas._h, arr, len(objects), _stream(dev), ctypes.byref(h)))
         self._h = h
+        _track("scene", self)
 
     @property
     def device(self):

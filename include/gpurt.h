@@ -29,6 +29,13 @@
  *     rt.rchit:15).  Ties (equal distance or equal t) go to the LOWEST id.
  *   - Barycentrics of a ray hit are (u,v) with b = (1-u-v, u, v) (rt.rchit:13); the
  *     closest-point query writes all three (b0,b1,b2) of the closest point.
+ *   - Bad input (the reference calls die() on errors, log.h:46-49; here a status or a
+ *     defined answer):  a build or refit over a non-finite position of a vertex that a
+ *     triangle uses (refit: any vertex) returns GPURT_E_INVALID (a refit then leaves the
+ *     tree unchanged).  A ray with a non-finite origin or direction component, a zero
+ *     direction or a NaN tmin / tmax is a miss (prim -1, t +inf, uv 0; not occluded).  A
+ *     closest-point query with a non-finite coordinate finds no triangle: prim -1, dist
+ *     +inf, pt and bary 0.
  */
 #ifndef GPURT_H
 #define GPURT_H
@@ -116,7 +123,22 @@ typedef struct gpurt_bvh_info {
 /* Create a context on HIP device `device`.  Replaces the ray-tracing part of
  * VK::Manager::init / init_rt (src/vk/vulkan.cpp:1136-1160, 1326-1385). */
 GPURT_API int gpurt_ctx_create(int device, gpurt_ctx* out);
+/* Waits for the device, then frees the context (which must outlive its handles). */
 GPURT_API void gpurt_ctx_destroy(gpurt_ctx ctx);
+
+/* Conditions the context's kernels report as they run (sticky bits, set on the device):
+ *   GPURT_STATUS_STACK_OVERFLOW        a traversal stack ran out (that launch's answers
+ *                                      are incomplete); the library sizes every stack for
+ *                                      the deepest tree, so this does not happen
+ *   GPURT_STATUS_TLAS_OUTSIDE_DOMAIN   a ray traced through a TLAS started more than 2^8
+ *                                      scene extents from the origin, outside the domain
+ *                                      where two-level answers are exact (see
+ *                                      gpurt_tlas_build); it was answered anyway
+ * gpurt_ctx_status waits for the device, returns the bits in *flags and clears them when
+ * `clear` is non-zero. */
+#define GPURT_STATUS_STACK_OVERFLOW 1u
+#define GPURT_STATUS_TLAS_OUTSIDE_DOMAIN 2u
+GPURT_API int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear);
 
 /* ---- build ----------------------------------------------------------------------- */
 
@@ -134,11 +156,39 @@ GPURT_API int gpurt_bvh_build(gpurt_ctx ctx, const float* verts, size_t vstride_
  * for optimisation loops that move vertices every step. */
 GPURT_API int gpurt_bvh_refit(gpurt_bvh bvh, const float* verts, size_t vstride_bytes, void* stream);
 
-/* Free a BVH (waits for work queued on its build stream).  Replaces Accel::destroy
- * (vulkan.cpp:768-776). */
+/* Free a BVH.  Replaces Accel::destroy (vulkan.cpp:768-776) and its deferred Drop
+ * (vulkan.h:305-336): the memory is released in stream order after the work of every
+ * call that used the handle (build, refit, queries, backward, TLAS builds and queries
+ * over it), without a host wait. */
 GPURT_API void gpurt_bvh_destroy(gpurt_bvh bvh);
 
 GPURT_API int gpurt_bvh_get_info(gpurt_bvh bvh, gpurt_bvh_info* out);
+
+/* Caller-owned DEVICE buffers for gpurt_bvh_export (each nullable). */
+typedef struct gpurt_bvh_arrays {
+    void* nodes;          /* n_nodes * 32 * W bytes: the FP32 wide nodes                   */
+    void* qnodes;         /* n_nodes * (48, 64, 96 for W = 2, 4, 8) bytes: the 8-bit nodes
+                             (only for node formats 2 and 3)                              */
+    float* tris;          /* n_tris * 12 floats: the packed triangles                      */
+    uint32_t* leaf_order; /* n_tris: packed position -> triangle index in the mesh         */
+} gpurt_bvh_arrays;
+
+/* Copy the built tree into the caller's device buffers (asynchronous on `stream`), for
+ * inspection, invariant checks and serialization.  The reference's acceleration
+ * structure is opaque (vkGetAccelerationStructureBuildSizesKHR, vulkan.cpp:921-928);
+ * this one is laid out as follows.  Node 0 is the root; nodes are stored level by level.
+ *   FP32 node (W children):  float lo_x[W], lo_y[W], lo_z[W], hi_x[W], hi_y[W], hi_z[W];
+ *                            uint32 child[W]; uint32 count[W] (triangles under the child)
+ *   child word: internal node index (< 2^31); leaf: 0x80000000 | (count-1) << 28 | first
+ *               packed triangle (count 1..8); empty slot: 0xFFFFFFFF
+ *   8-bit node: float origin[3]; uint32 exponents (byte a = e_a, scale 2^(e_a - 127));
+ *               uint32 child[W]; uint8 q[6][W] (lo x, y, z, hi x, y, z), zero-padded; child
+ *               j's box is fmaf(q, scale, origin) per plane
+ *   packed triangle: float4 (a.xyz, id bits), float4 (b.xyz, 0), float4 (c.xyz, 0) -- a, b,
+ *               c the mesh's vertices in index order, id the triangle index
+ * Child boxes are inflated by 2^-18 of the largest coordinate magnitude, and every 8-bit
+ * box contains its FP32 box. */
+GPURT_API int gpurt_bvh_export(gpurt_bvh bvh, const gpurt_bvh_arrays* out, void* stream);
 
 /* ---- queries ---------------------------------------------------------------------- */
 
@@ -187,18 +237,19 @@ GPURT_API int gpurt_trace_any(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, ui
  * re-traversal): the closest point p_i of that triangle, its barycentrics b and q_i - p_i
  * are recomputed in FP64 (the computation the forward pass reports from), then
  * g_q[i] = g_dist[i] (q_i - p_i)/|q_i - p_i|; for the three vertices v_j of prim_i,
- * g_verts[v_j] -= b_j g_q[i]  (FP32 atomic scatter-add: g_verts is ACCUMULATED,
- * n_verts x 3, zero it first).  g_q and g_verts are each nullable; so are the saved bary,
- * pt and dist (accepted for the forward's output layout, not read).  A query on the
- * surface (distance 0) contributes zero. */
+ * g_verts[v_j] -= b_j g_q[i]  (g_verts is ACCUMULATED, n_verts x 3: zero it first).
+ * Vertex sums are deterministic: each vertex's terms are added in query order (sorted
+ * by vertex, no atomics), the same bits on every run.  g_q and g_verts are each
+ * nullable; so are the saved bary, pt and dist (accepted for the forward's output
+ * layout, not read).  A query on the surface (distance 0) contributes zero. */
 GPURT_API int gpurt_closest_point_bwd(gpurt_bvh bvh, const float* q, size_t n, const int32_t* prim,
                             const float* bary, const float* pt, const float* dist,
                             const float* g_dist, float* g_q, float* g_verts, void* stream);
 
 /* Gradient of sum_i g_t[i] * t_i for closest hits.  With n = (v1-v0) x (v2-v0):
  * dt/do = -n/(n.d), dt/dd = -t n/(n.d), dt/dv_j = b_j n/(n.d), b = (1-u-v,u,v).
- * g_origin / g_dir are n x 3 (written, nullable); g_verts is ACCUMULATED by atomic
- * scatter-add (nullable).  Misses contribute zero. */
+ * g_origin / g_dir are n x 3 (written, nullable); g_verts is ACCUMULATED (nullable), in
+ * ray order per vertex, deterministically.  Misses contribute zero. */
 GPURT_API int gpurt_trace_bwd(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, const int32_t* prim,
                     const float* t, const float* uv, const float* g_t, float* g_origin,
                     float* g_dir, float* g_verts, void* stream);
@@ -249,15 +300,72 @@ typedef struct gpurt_batch_grad {
 /* Record the batched backward pass (one gpurt_batch_grad per mesh, in the order of
  * gpurt_batch_create) into a second HIP graph: it clears the vertex gradients, then
  * forms every mesh's gradients from the outputs of the last gpurt_batch_run (saved ids,
- * closest-point triangles recomputed in FP64, hit t, (u,v)) without re-traversal,
- * scatter-adding vertex terms with FP32 atomics.  Vertex gradients of hit t need the
- * forward's r_uv.
+ * closest-point triangles recomputed in FP64, hit t, (u,v)) without re-traversal; the
+ * vertex terms of all meshes are sorted by (mesh, vertex) and each vertex's terms added
+ * in a fixed order (no atomics: the same bits on every run).  Vertex gradients of hit t
+ * need the forward's r_uv.  The term buffers (allocated here) take 108 bytes per query
+ * or ray slot (3 terms of 36 bytes; slots are allotted in 64-item chunks per mesh):
+ * about 2 GB for 19 M queries and rays.
  * No counterpart in the reference (it has no gradients); it is the batched form of
  * gpurt_closest_point_bwd + gpurt_trace_bwd.  Calling it again replaces the buffers. */
 GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* grads);
 /* Run the recorded backward graph on `stream` (after gpurt_batch_run on the same stream). */
 GPURT_API int gpurt_batch_run_backward(gpurt_batch batch, void* stream);
 GPURT_API void gpurt_batch_destroy(gpurt_batch batch);
+
+/* ---- autotuner ------------------------------------------------------------------------ */
+
+/* Candidate values per parameter (each list nullable: its default is used). */
+typedef struct gpurt_tune_space {
+    const int32_t* node_width;   int32_t n_node_width;   /* default {2, 4, 8}            */
+    const int32_t* stack_depth;  int32_t n_stack_depth;  /* default {8, 16, 32, 64}      */
+    const int32_t* waves_per_cu; int32_t n_waves_per_cu; /* default {0, 16, 24}; 0 = the
+                                                            library's choice             */
+    const int32_t* treelet_size; int32_t n_treelet_size; /* default {0, 6}              */
+    const int32_t* node_format;  int32_t n_node_format;  /* default {1, 2}              */
+    int32_t reps;              /* timed runs per candidate, median kept (0 = 3)          */
+    float builds_per_run;      /* gpurt_tune_queries: objective = query time + this many
+                                  builds (0: built once, queried many times)             */
+    const char* cache_path;    /* nullable: a text file of tuned results; a stored result
+                                  for the same key is returned without timing anything,
+                                  a new one is appended                                  */
+} gpurt_tune_space;
+
+typedef struct gpurt_tune_result {
+    gpurt_build_opts build;    /* the chosen build options                               */
+    gpurt_query_opts query;    /* the chosen query options (stats null)                  */
+    float best_ms;             /* median time of the chosen candidate (objective)        */
+    float default_ms;          /* the same with every option at its library default      */
+    int32_t candidates;        /* candidates timed (0 when taken from the cache)         */
+    int32_t from_cache;
+    char key[128];             /* cache key: device, workload kind, log2 size buckets    */
+} gpurt_tune_result;
+
+/* Autotuner of the batched form: every candidate of `space` (node width x stack depth x
+ * waves per CU x treelet size x node format) is a full gpurt_batch_create + timed
+ * gpurt_batch_run replays of the caller's own meshes and queries on `stream` (HIP
+ * events), and the fastest is returned, with the time of the library defaults beside
+ * it.  Answers do not depend on any of these parameters (the traversal is exact for any
+ * tree), only speed.  The meshes' output arrays are written by the trial runs.  The
+ * reference's knobs of this kind are host fields (src/vk/rt.h:38-53). */
+GPURT_API int gpurt_tune_batch(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_meshes,
+                               const gpurt_tune_space* space, void* stream, gpurt_tune_result* out);
+
+/* Autotuner of the plain path for one mesh: builds the BVH for every build candidate and
+ * times gpurt_closest_point over q (n_q, nullable) plus gpurt_trace_closest over rays
+ * (n_r, nullable) for every query candidate; objective = query time + builds_per_run x
+ * build time (host wall time of gpurt_bvh_build). */
+GPURT_API int gpurt_tune_queries(gpurt_ctx ctx, const float* verts, size_t vstride_bytes, size_t n_verts,
+                                 const uint32_t* tri_idx, size_t n_tris, const float* q, size_t n_q,
+                                 const gpurt_ray* rays, size_t n_r, const gpurt_tune_space* space,
+                                 void* stream, gpurt_tune_result* out);
+
+/* The tuned-result cache: one line per result, "key<TAB>node_width leaf_size treelet_size
+ * treelet_rounds node_format stack_depth waves_per_cu leaf_trigger refill_trigger best_ms
+ * default_ms"; the last line of a key wins.  Load returns GPURT_E_INVALID when the key is
+ * absent. */
+GPURT_API int gpurt_tune_cache_load(const char* path, const char* key, gpurt_tune_result* out);
+GPURT_API int gpurt_tune_cache_store(const char* path, const gpurt_tune_result* result);
 
 /* ---- two-level instancing (TLAS) ----------------------------------------------------- */
 
@@ -286,8 +394,11 @@ typedef struct gpurt_instance {
  * instance (o' = M^-1 (o - c), d' = M^-1 d with the FP32 rounding of the FP64 inverse of
  * the 3x3 part M; c the translation), so t is the same parameter in both spaces.
  * Results are exact (equal to FP32 brute force over all instances, ties to the lowest
- * instance index, then the lowest primitive id) for ray origins within 2^8 times the
- * scene's extent; transforms must be invertible.  Returns when the TLAS is complete. */
+ * instance index, then the lowest primitive id) for ray origins whose coordinates are at
+ * most 2^8 times the largest |coordinate| of the instances' world boxes (a ray starting
+ * farther out is answered and sets GPURT_STATUS_TLAS_OUTSIDE_DOMAIN, gpurt_ctx_status);
+ * transforms must be finite and invertible (else GPURT_E_INVALID).  Returns when the
+ * TLAS is complete. */
 GPURT_API int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n,
                                const gpurt_build_opts* opts, void* stream, gpurt_tlas* out);
 GPURT_API void gpurt_tlas_destroy(gpurt_tlas tlas);

@@ -234,6 +234,37 @@ public:
     gpurt_bvh accel = nullptr;
 };
 
+// Autotuner: the build and query options that run a workload fastest (gpurt_tune_queries
+// over one mesh's closest-point queries and rays; gpurt_tune_batch over a batch), and the
+// tuned-result cache a program reads at start-up instead of tuning again.  The reference
+// sets these knobs by hand (RTPipe's host fields, src/vk/rt.h:38-53).
+//
+//   gpurt_tune_space sp{};  sp.cache_path = "tuned.txt";
+//   gpurt_tune_result r = gpurt::tune_queries(ctx, mesh, q, nq, rays, nr, &sp);
+//   gpurt::Accel accel(ctx, mesh, &r.build);
+//   accel.trace_closest(rays, nr, prim, t, uv, &r.query);
+inline gpurt_tune_result tune_queries(const Context& ctx, const Mesh& mesh, const float* q, size_t n_q,
+                                      const gpurt_ray* rays, size_t n_r, const gpurt_tune_space* space = nullptr,
+                                      hipStream_t stream = nullptr) {
+    mesh.sync();
+    gpurt_tune_result r;
+    check(gpurt_tune_queries(ctx.handle(), mesh.device_verts(), sizeof(Mesh::Vertex), mesh.verts().size(),
+                             mesh.device_inds(), mesh.inds().size() / 3, q, n_q, rays, n_r, space, stream, &r));
+    return r;
+}
+
+inline gpurt_tune_result tune_batch(const Context& ctx, const std::vector<gpurt_batch_mesh>& meshes,
+                                    const gpurt_tune_space* space = nullptr, hipStream_t stream = nullptr) {
+    gpurt_tune_result r;
+    check(gpurt_tune_batch(ctx.handle(), meshes.data(), meshes.size(), space, stream, &r));
+    return r;
+}
+
+// The cached result for `key` (gpurt_tune_result::key of an earlier tuning run), or false.
+inline bool load_tuned(const std::string& cache_path, const std::string& key, gpurt_tune_result& out) {
+    return gpurt_tune_cache_load(cache_path.c_str(), key.c_str(), &out) == GPURT_OK;
+}
+
 // Mirrors the top-level VK::Accel(const std::vector<Drop<Accel>>&, const std::vector<Mat4>&)
 // (vulkan.h:259, vulkan.cpp:777-856): instance i places blas[i] by the row-major 3x4
 // object-to-world transform T[i] with custom index i and mask 0xFF.  The BLASes must

@@ -323,6 +323,17 @@ ORC_API int orc_mt_f32(const float* o, const float* d, const float* pa, const fl
     return 1;
 }
 
+/* Inputs the library defines as unanswerable (include/gpurt.h): a ray with a non-finite
+ * origin or direction component, a zero direction or a NaN bound is a miss; a query point
+ * with a non-finite coordinate finds no triangle (prim -1, distance +inf, point and
+ * barycentrics 0).  The reference leaves such rays undefined (traceRayEXT requires finite
+ * rays); the library and this restatement both answer them this way. */
+ORC_API int orc_ray_ok(const float* r) {
+    return isfinite(r[0]) && isfinite(r[1]) && isfinite(r[2]) && isfinite(r[4]) && isfinite(r[5]) &&
+           isfinite(r[6]) && (r[4] != 0.0f || r[5] != 0.0f || r[6] != 0.0f) && !isnan(r[3]) && !isnan(r[7]);
+}
+ORC_API int orc_point_ok(const float* q) { return isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]); }
+
 /* ------------------------------------------------------------------------------------ */
 /* FP32 brute force                                                                     */
 /* ------------------------------------------------------------------------------------ */
@@ -344,7 +355,7 @@ static void cpq_bf32_range(void* vctx, int64_t lo, int64_t hi) {
         const float* q = c->Q + 3 * i;
         float best = INFINITY, bcp[3] = {0, 0, 0}, bb[3] = {0, 0, 0}, bd = INFINITY;
         int64_t bi = -1;
-        for (int64_t t = 0; t < c->nt; t++) {
+        for (int64_t t = 0; t < c->nt && orc_point_ok(q); t++) {
             const uint32_t* f = c->F + 3 * t;
             float cp[3], br[3];
             float d2 = orc_cp_tri_f32(q, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], cp, br);
@@ -386,7 +397,7 @@ static void ray_closest_bf32_range(void* vctx, int64_t lo, int64_t hi) {
         const float* r = c->R + 8 * i;
         float tmin = r[3], best = r[7], bu = 0, bv = 0;
         int64_t bi = -1;
-        for (int64_t k = 0; k < c->nt; k++) {
+        for (int64_t k = 0; k < c->nt && orc_ray_ok(r); k++) {
             const uint32_t* f = c->F + 3 * k;
             float t, u, v;
             if (!orc_mt_f32(r, r + 4, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], &t, &u, &v))
@@ -405,7 +416,7 @@ static void ray_any_bf32_range(void* vctx, int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; i++) {
         const float* r = c->R + 8 * i;
         uint8_t hit = 0;
-        for (int64_t k = 0; k < c->nt && !hit; k++) {
+        for (int64_t k = 0; k < c->nt && !hit && orc_ray_ok(r); k++) {
             const uint32_t* f = c->F + 3 * k;
             float t, u, v;
             if (!orc_mt_f32(r, r + 4, c->V + 3 * f[0], c->V + 3 * f[1], c->V + 3 * f[2], &t, &u, &v))
@@ -494,7 +505,7 @@ static void inst_ray_range(void* vctx, int64_t lo, int64_t hi) {
         float tmin = r[3], best = r[7], bu = 0, bv = 0;
         int64_t bi = -1, bk = -1;
         uint8_t hit = 0;
-        for (int64_t n = 0; n < c->n_inst && !hit; n++) {
+        for (int64_t n = 0; n < c->n_inst && !hit && orc_ray_ok(r); n++) {
             if ((c->mask[n] & 0xFFu) == 0) continue;
             float oo[3], od[3];
             to_object_f32(c->minv + 9 * n, c->xf + 12 * n, r, r + 4, oo, od);
@@ -1076,7 +1087,7 @@ static void lbvh_cpq_range(void* vctx, int64_t lo, int64_t hi) {
         const float* btr = NULL;
         int32_t bi = -1;
         int sp = 0;
-        st_n[sp] = 0; st_d[sp] = 0.0f; sp++;
+        if (orc_point_ok(q)) { st_n[sp] = 0; st_d[sp] = 0.0f; sp++; }
         while (sp > 0) {
             sp--;
             int32_t ni = st_n[sp];
@@ -1166,7 +1177,7 @@ static void lbvh_ray_range(void* vctx, int64_t lo, int64_t hi) {
         ray_inv(d, inv);
         int sp = 0;
         float t0 = box_ray(&b->nodes[0], o, inv, tmin, best);
-        if (t0 < INFINITY) { st_n[sp] = 0; st_t[sp] = t0; sp++; }
+        if (t0 < INFINITY && orc_ray_ok(r)) { st_n[sp] = 0; st_t[sp] = t0; sp++; }
         while (sp > 0 && !hit) {
             sp--;
             int32_t ni = st_n[sp];

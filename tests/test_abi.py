@@ -45,6 +45,7 @@ def _c_layout(struct, fields):
 @pytest.mark.parametrize("cls,struct", [(_lib.BuildOpts, "gpurt_build_opts"),
                                         (_lib.QueryOpts, "gpurt_query_opts"),
                                         (_lib.BvhInfo, "gpurt_bvh_info"),
+                                        (_lib.BvhArrays, "gpurt_bvh_arrays"),
                                         (_lib.BatchMesh, "gpurt_batch_mesh"),
                                         (_lib.BatchGrad, "gpurt_batch_grad"),
                                         (_lib.Instance, "gpurt_instance"),
@@ -69,8 +70,12 @@ def test_null_arguments_return_invalid_without_device_work():
     assert lib.gpurt_closest_point_bwd(None, None, 1, None, None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_trace_bwd(None, None, 1, None, None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_bvh_get_info(None, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_bvh_export(None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_trace_closest_noalloc(None, None, 1, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_ctx_create(0, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_ctx_status(None, None, 0) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_tune_cache_load(None, None, None) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_tune_batch(None, None, 0, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_scene_create(None, None, 1, None, ctypes.byref(out)) == _lib.GPURT_E_INVALID
     assert lib.gpurt_render(None, None, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_scene_light_pdf(None, None, None, 1, None, None) == _lib.GPURT_E_INVALID

@@ -1,5 +1,5 @@
-"""Autotuner: cache keys and storage (CPU), and a tuned configuration that stays exact
-(GPU)."""
+"""Autotuner: the tuned-result cache (CPU: the library's text cache, also read by C++
+callers), and tuned configurations that stay exact (GPU)."""
 import numpy as np
 import pytest
 import torch
@@ -9,20 +9,32 @@ from gpurt import workloads as W
 from oracle import oracle as O
 
 
-def test_signature_buckets_sizes():
-    a = autotune.signature("dev", 1_000_000, 1 << 20, 0, "cpq", "b0")
-    b = autotune.signature("dev", 1_040_000, (1 << 20) + 5, 0, "cpq", "b0")
-    c = autotune.signature("dev", 4_000_000, 1 << 20, 0, "cpq", "b0")
-    assert a == b and a != c
+def _res(key, w, s, ms):
+    return {"build": {"node_width": w, "treelet_size": 0, "node_format": 2},
+            "query": {"stack_depth": s, "waves_per_cu": 16}, "objective_ms": ms, "default_ms": 2 * ms,
+            "key": key}
 
 
 def test_cache_round_trip(tmp_path):
-    path = str(tmp_path / "tune.json")
-    assert autotune.load_cache(path) == {}
-    autotune.store_cache("k1", {"build": {"node_width": 4}}, path)
-    autotune.store_cache("k2", {"build": {"node_width": 8}}, path)
-    c = autotune.load_cache(path)
-    assert c["k1"]["build"]["node_width"] == 4 and c["k2"]["build"]["node_width"] == 8
+    path = str(tmp_path / "tune.txt")
+    assert autotune.load("k1", path) is None
+    autotune.store(_res("gfx950x256|cpq|t20|q20|r-1|b0", 4, 8, 1.5), path)
+    autotune.store(_res("k2", 8, 16, 2.5), path)
+    autotune.store(_res("gfx950x256|cpq|t20|q20|r-1|b0", 2, 32, 1.25), path)  # a later line wins
+    a = autotune.load("gfx950x256|cpq|t20|q20|r-1|b0", path)
+    assert a["build"]["node_width"] == 2 and a["query"]["stack_depth"] == 32 and a["cached"]
+    assert a["objective_ms"] == 1.25 and a["default_ms"] == 2.5
+    b = autotune.load("k2", path)
+    assert b["build"] == {"node_width": 8, "treelet_size": 0, "node_format": 2}
+    assert autotune.load("k", path) is None  # prefixes do not match
+    with open(path) as f:
+        assert len(f.read().splitlines()) == 3
+
+
+def test_cache_rejects_bad_keys(tmp_path):
+    from gpurt import _lib
+    with pytest.raises(_lib.GpurtError):
+        autotune.store(_res("a\tb", 4, 8, 1.0), str(tmp_path / "t.txt"))
 
 
 @pytest.mark.gpu
@@ -38,9 +50,10 @@ def test_tuned_queries_stay_exact(ctx, dev, tmp_path):
     space = {"node_width": (2, 4), "stack_depth": (8, 16), "waves_per_cu": (0,), "treelet_size": (0, 5),
              "node_format": (1, 2)}
     best = autotune.tune_queries(ctx, tv, tf, queries=tq, rays=tr, space=space, path=path)
-    assert not best["cached"] and len(best["table"]) == 16
+    assert not best["cached"] and best["candidates"] == 16
+    assert 0 < best["objective_ms"] and 0 < best["default_ms"]
     again = autotune.tune_queries(ctx, tv, tf, queries=tq, rays=tr, space=space, path=path)
-    assert again["cached"] and again["build"] == best["build"]
+    assert again["cached"] and again["build"] == best["build"] and again["query"] == best["query"]
     bvh = gpurt.Bvh(ctx, tv, tf, **best["build"])
     prim, _, dist, _ = bvh.closest_point(tq, **best["query"])
     torch.cuda.synchronize()
@@ -60,7 +73,8 @@ def test_tuned_batch(ctx, dev, tmp_path):
                        "queries": torch.from_numpy(q).to(dev)})
     space = {"node_width": (4, 8), "stack_depth": (8,), "waves_per_cu": (0,), "treelet_size": (0,),
              "node_format": (2,)}
-    best = autotune.tune_batch(ctx, meshes, space=space, path=str(tmp_path / "t.json"))
+    best = autotune.tune_batch(ctx, meshes, space=space, path=str(tmp_path / "t.txt"))
+    assert best["candidates"] == 2 and best["build"]["node_width"] in (4, 8)
     res = gpurt.Batch(ctx, meshes, **best["build"], **best["query"]).run()
     torch.cuda.synchronize()
     for m, r in zip(meshes, res):

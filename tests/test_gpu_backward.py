@@ -2,9 +2,10 @@
 Functions) against the oracle: FP64 closed forms evaluated at the GPU's argmin, FP64
 closed forms at the FP64 argmin (brute force), and FP64 central differences.
 
-Vertex gradients are summed with FP32 atomics in an unspecified order, so they are
-compared with tolerances (stated per test); query/origin gradients are per-query and
-deterministic."""
+Vertex gradients are FP32 sums of per-query terms, compared with tolerances (stated per
+test) against the FP64 forms; the plain path adds each vertex's terms in query order
+(sorted by vertex, no atomics), so its gradients are the same bits on every run, and the
+batched path's atomic sums agree with them within rounding."""
 import numpy as np
 import pytest
 import torch
@@ -176,3 +177,36 @@ def test_misses_and_zero_distance_give_zero_gradient(ctx, dev):
     assert torch.count_nonzero(dist) == 0
     dist.sum().backward()
     assert torch.count_nonzero(q.grad) == 0
+
+
+def test_plain_vertex_gradients_are_deterministic(ctx, dev):
+    """The same backward, run five times on the differentiable workload's kind of input
+    (many queries per vertex), gives identical bits; and the sum over every term, checked
+    against the FP64 closed form, is right."""
+    V, F = _sphere(6, 3)
+    Q = W.surface_samples(V.astype(np.float64), F, np.random.default_rng(4), 200000, 0.005).astype(np.float32)
+    rng = np.random.default_rng(5)
+    o = W.random_dirs(rng, 50000) * 3.0
+    d = -o / np.linalg.norm(o, axis=1, keepdims=True)
+    R = W.pack_rays(o, d, 1e-5, 1e7)
+    tv, tq, tr = _t(V, dev), _t(Q, dev), _t(R, dev)
+    bvh = gpurt.Bvh(ctx, tv, _t(F, dev))
+    prim, pt, dist, bary = bvh.closest_point(tq)
+    rp, rt, ruv = bvh.trace_closest(tr)
+    g1 = torch.ones(len(Q), device=dev)
+    g2 = torch.ones(len(R), device=dev)
+    runs = []
+    for _ in range(5):
+        gv = torch.zeros((V.shape[0], 3), device=dev)
+        bvh.closest_point_bwd(tq, prim, bary, pt, dist, g1, None, gv)
+        bvh.trace_bwd(tr, rp, rt, ruv, g2, None, None, gv)
+        torch.cuda.synchronize()
+        runs.append(gv.cpu().numpy())
+    for r in runs[1:]:
+        assert np.array_equal(r, runs[0])
+    p = prim.cpu().numpy()
+    _, ev = O.cpq_grad_f64(V.astype(np.float64), F, Q.astype(np.float64), p)
+    _, _, erv = O.ray_grad_f64(V.astype(np.float64), F, R.astype(np.float64), rp.cpu().numpy(),
+                            rt.cpu().numpy().astype(np.float64), ruv.cpu().numpy().astype(np.float64))
+    scale = np.abs(ev + erv).max()
+    np.testing.assert_allclose(runs[0], ev + erv, atol=1e-5 * scale, rtol=1e-4)

@@ -125,3 +125,20 @@ def test_tlas_rejects_bad_instances(ctx, dev):
     singular[2, 2] = 0.0
     with pytest.raises(gpurt.GpurtError):
         gpurt.Tlas(ctx, [(b4, singular)])
+
+
+def test_rays_outside_the_exact_domain_are_reported(ctx, dev):
+    """gpurt_ctx_status reports a TLAS ray whose origin lies beyond 2^8 x the scene's
+    largest coordinate (answered, but outside the domain where exactness is argued);
+    rays inside it leave the status clear."""
+    sc = W.instanced_scene(n_rays=512, grid=41, n_spheres=2)
+    blas = [gpurt.Bvh(ctx, torch.from_numpy(V).to(dev), torch.from_numpy(F).to(dev)) for V, F in sc["meshes"]]
+    tlas = gpurt.Tlas(ctx, [(blas[m], sc["xf"][i].reshape(3, 4)) for i, m in enumerate(sc["mesh_of"])])
+    ctx.status(clear=True)
+    tlas.trace_closest(torch.from_numpy(sc["rays"]).to(dev))
+    assert ctx.status() == set()
+    far = sc["rays"][:4].copy()
+    far[:, 0] = 1e9  # far beyond 2^8 scene extents
+    tlas.trace_any(torch.from_numpy(far).to(dev))
+    assert ctx.status(clear=True) == {"tlas_outside_domain"}
+    assert ctx.status() == set()

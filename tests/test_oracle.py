@@ -196,6 +196,46 @@ def test_lbvh_matches_brute_force_rays():
     assert np.array_equal(bvh.trace_any(w["rays_any"]), O.ray_any_brute_f32(V, F, w["rays_any"]))
 
 
+def test_inputs_that_cannot_be_answered():
+    """Known answers of the bad-input rule: a ray straight down onto the unit right
+    triangle hits it; the same ray with a NaN or infinite origin / direction component,
+    a zero direction or a NaN bound is a miss in every ray routine (brute force, LBVH,
+    instanced), while an infinite tmax is an ordinary ray.  A query point with a
+    non-finite coordinate finds no triangle (-1, +inf, zeros), in brute force and LBVH."""
+    V = np.array(TRI, np.float32)
+    F = np.array([[0, 1, 2]], np.uint32)
+    good = np.array([0.25, 0.25, 1.0, 0.0, 0.0, 0.0, -1.0, 10.0], np.float32)
+    rays = [good]
+    for k, v in ((0, np.nan), (1, np.inf), (2, -np.inf), (4, np.nan), (6, -np.inf), (3, np.nan), (7, np.nan)):
+        r = good.copy()
+        r[k] = v
+        rays.append(r)
+    z = good.copy()
+    z[4:7] = 0.0
+    rays.append(z)
+    f = good.copy()
+    f[7] = np.inf
+    rays.append(f)
+    R = np.array(rays, np.float32)
+    miss = np.array([False] + [True] * 8 + [False])
+    prim, t, uv = O.ray_closest_brute_f32(V, F, R)
+    assert (prim[miss] == -1).all() and (t[miss] == np.inf).all() and (uv[miss] == 0).all()
+    assert (prim[~miss] == 0).all() and (t[~miss] == 1.0).all()
+    assert np.array_equal(O.ray_any_brute_f32(V, F, R), (~miss).astype(np.uint8))
+    bvh = O.CpuBvh(V, F, 4)
+    for g, e in zip(bvh.trace_closest(R), (prim, t, uv)):
+        assert np.array_equal(g, e)
+    assert np.array_equal(bvh.trace_any(R), (~miss).astype(np.uint8))
+    obj, ip, it, iuv = O.inst_ray_brute_f32(np.eye(3, 4), [0], [(V, F)], R)
+    assert np.array_equal(ip, prim) and np.array_equal(it, t) and np.array_equal(obj[miss], prim[miss])
+    assert np.array_equal(O.inst_ray_brute_f32(np.eye(3, 4), [0], [(V, F)], R, any_hit=True),
+                          (~miss).astype(np.uint8))
+    Q = np.array([[0.25, 0.25, 1.0], [np.nan, 0, 0], [0, np.inf, 0], [0, 0, -np.inf]], np.float32)
+    for p, pt, d, b in (O.cpq_brute_f32(V, F, Q), bvh.closest_point(Q)):
+        assert p.tolist() == [0, -1, -1, -1] and d[0] == 1.0 and (d[1:] == np.inf).all()
+        assert (pt[1:] == 0).all() and (b[1:] == 0).all()
+
+
 def test_ray_brute_f32_vs_f64():
     w = W.rays_scene(n_rays=2048, grid=41, n_spheres=3)
     V, F, R = w["verts"], w["tris"], w["rays"]
