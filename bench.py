@@ -338,13 +338,24 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     gv = torch.zeros((tv.shape[0], 3), device=dev)
     ms_cb = timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2)
     ms_rb = timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)
+    # vertex terms are FP32 atomic adds: 9 per query off the surface and per hitting ray
+    n_cq_atomics = 9 * int((dist > 0).sum())
+    n_r_atomics = 9 * int((rp >= 0).sum())
+    # the fixed-order (deterministic) vertex sums: sort by vertex, two-pass segmented sums
+    ctx.deterministic = True
+    ms_cb_det = timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2)
+    ms_rb_det = timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)
+    ctx.deterministic = False
     del bvh
     return {"workload": "diff", "n_tris": int(F.shape[0]), "n_queries": nq, "n_rays": nr,
             "fwd_units_per_s": (nq + nr) / (ms_f * 1e-3), "fwd_ms": ms_f,
             "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb,
             "fwd_bwd_over_fwd": ms_fb / ms_f,
             "cpq_bwd_roofline": roof(nq * CPQ_BWD_BYTES, ms_cb, "k_cpq_bwd"),
-            "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "k_ray_bwd")}
+            "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "k_ray_bwd"),
+            "cpq_bwd_atomics_per_s": n_cq_atomics / (ms_cb * 1e-3),
+            "ray_bwd_atomics_per_s": n_r_atomics / (ms_rb * 1e-3),
+            "cpq_bwd_deterministic_ms": ms_cb_det, "ray_bwd_deterministic_ms": ms_rb_det}
 
 
 def batched_bench(args, torch, gpurt, W, dev, ctx):
@@ -547,12 +558,13 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     plain streaming kernel that reads 256 MiB and writes 256 MiB, whose
     FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
     kind = args.profile_workload
-    w = W.rays_scene() if kind == "rays_scene" else W.diff() if kind == "diff_bwd" else getattr(W, kind)()
+    w = W.rays_scene() if kind == "rays_scene" else W.diff() if kind.startswith("diff_bwd") else getattr(W, kind)()
+    ctx.deterministic = kind == "diff_bwd_fixed"
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
                     node_format=args.format, leaf_size=args.leaf)
-    if kind == "diff_bwd":
+    if kind.startswith("diff_bwd"):
         q = torch.from_numpy(w["queries"]).to(dev)
         rays = gpurt.make_rays(torch.from_numpy(w["ray_o"]).to(dev), torch.from_numpy(w["ray_d"]).to(dev))
         n = q.shape[0] + rays.shape[0]
@@ -629,9 +641,10 @@ def main():
                     help="only run closest-hit launches + calibration copies (for rocprofv3)")
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--profile-workload", default="rays_scene",
-                    choices=["rays_scene", "cpq_small", "cpq_large", "diff_bwd"],
+                    choices=["rays_scene", "cpq_small", "cpq_large", "diff_bwd", "diff_bwd_fixed"],
                     help="kernel of the --profile run: closest-hit rays, closest-point queries, or the "
-                         "backward passes of the differentiable workload")
+                         "backward passes of the differentiable workload (atomic or fixed-order "
+                         "vertex sums)")
     ap.add_argument("--no-tune", action="store_true", help="batched leg: skip the autotuner sweep")
     ap.add_argument("--only", choices=["diff", "cpq_large", "batched", "tracer"],
                     help="print only this leg (for profiling it)")

@@ -237,6 +237,12 @@ int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear) {
     return GPURT_OK;
 }
 
+int gpurt_ctx_set_deterministic(gpurt_ctx ctx, int on) {
+    if (!ctx) return set_error(GPURT_E_INVALID, "null context");
+    ctx->deterministic = on != 0;
+    return GPURT_OK;
+}
+
 static int check_mesh(const float* verts, size_t vstride_bytes, size_t n_verts,
                       const uint32_t* tri_idx, size_t n_tris) {
     if (!verts || !tri_idx) return set_error(GPURT_E_INVALID, "null vertex or index pointer");

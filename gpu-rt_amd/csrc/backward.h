@@ -9,9 +9,9 @@
 //             dt/do = -n/(n.d),  dt/dd = -t n/(n.d),  dt/dv_j = b_j n/(n.d)
 //
 // Vertex positions are read with a stride of `vs` floats; vertex gradients are n x 3.
-// The terms of one query or ray are formed here; the plain (backward.hip) and batch
-// (batch.hip) kernels sort them by vertex and add each vertex's terms in a fixed order,
-// without atomics, so vertex gradients are the same bits on every run.
+// The terms of one query or ray are formed here.  By default they are scatter-added with
+// hardware FP32 atomics (order, and so the last bits, vary between runs); a context set
+// to deterministic sums sorts them by vertex and adds them in a fixed order instead.
 #pragma once
 
 #include "gpurt_internal.h"
@@ -140,6 +140,80 @@ __device__ __forceinline__ CornerTerms ray_bwd_terms(size_t i, const float4* __r
         ct.c[j][2] = nrm.z * w;
     }
     return ct;
+}
+
+// hardware FP32 atomic scatter of the terms (the default, non-deterministic sums)
+__device__ __forceinline__ void scatter_terms(const CornerTerms& t, float* g_verts) {
+    if (!g_verts) return;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (t.v[j] == kNoVertex) continue;
+        const size_t v = t.v[j];
+        unsafeAtomicAdd(&g_verts[3 * v], t.c[j][0]);
+        unsafeAtomicAdd(&g_verts[3 * v + 1], t.c[j][1]);
+        unsafeAtomicAdd(&g_verts[3 * v + 2], t.c[j][2]);
+    }
+}
+
+// ---- deterministic vertex sums -------------------------------------------------------
+// The terms, sorted by key (a vertex, or (mesh, vertex) in a batch; `none` for terms that
+// add nothing), are summed per key in two passes with a fixed order (the same bits on
+// every run), and without a serial loop over a long run:
+//   k_wave_segsum  one thread per sorted term: within each 64-term wave, a segmented
+//                  suffix scan over shuffles; the first term of each key in the wave keeps
+//                  the sum of that key's terms in the wave (partial[p])
+//   k_run_sum      the first term of each key's run adds the partials of the waves the
+//                  run spans (one per 64 terms) and hands the total to the sink
+template <int BS>
+__global__ __launch_bounds__(BS) void k_wave_segsum(uint32_t m, const uint64_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ vals, const float* __restrict__ terms,
+                                                    float* __restrict__ partial) {
+    const uint32_t p = blockIdx.x * BS + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool in = p < m;
+    const uint64_t key = in ? keys[p] : ~0ull;
+    const bool head = in && (lane == 0 || keys[p - 1] != key);
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (in) {
+        const size_t k = vals[p];
+        x = terms[3 * k];
+        y = terms[3 * k + 1];
+        z = terms[3 * k + 2];
+    }
+    // lanes of this key in this wave: [lane, next head)
+    const uint64_t heads = __ballot(head);
+    const uint64_t after = lane == 63 ? 0ull : heads >> (lane + 1);
+    const uint32_t end = after ? lane + 1 + (uint32_t)__builtin_ctzll(after) : 64u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float ox = __shfl_down(x, d), oy = __shfl_down(y, d), oz = __shfl_down(z, d);
+        if (lane + d < end) {
+            x += ox;
+            y += oy;
+            z += oz;
+        }
+    }
+    if (head) {
+        partial[3 * (size_t)p] = x;
+        partial[3 * (size_t)p + 1] = y;
+        partial[3 * (size_t)p + 2] = z;
+    }
+}
+
+template <int BS, typename Sink>
+__global__ __launch_bounds__(BS) void k_run_sum(uint32_t m, const uint64_t* __restrict__ keys,
+                                                const float* __restrict__ partial, uint64_t none, Sink sink) {
+    const uint32_t p = blockIdx.x * BS + threadIdx.x;
+    if (p >= m) return;
+    const uint64_t key = keys[p];
+    if (key >= none || (p > 0 && keys[p - 1] == key)) return;
+    float x = partial[3 * (size_t)p], y = partial[3 * (size_t)p + 1], z = partial[3 * (size_t)p + 2];
+    for (uint32_t q = (p | 63u) + 1u; q < m && keys[q] == key; q += 64u) {
+        x += partial[3 * (size_t)q];
+        y += partial[3 * (size_t)q + 1];
+        z += partial[3 * (size_t)q + 2];
+    }
+    sink(key, x, y, z);
 }
 
 }  // namespace gpurt_impl

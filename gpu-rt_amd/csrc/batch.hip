@@ -237,18 +237,19 @@ __global__ __launch_bounds__(256) void k_batch_cpq_finish(const BatchMeshDev* __
                    md.q_bary);
 }
 
-// Vertex terms of the batched backward, one wave per 64-item chunk of the forward chunk
-// list (closest-point chunks, then ray chunks): item (chunk c, lane l) owns term slots
-// 3 (64 c + l) .. + 2, keyed by (mesh << vbits | vertex), or `none` when it adds nothing
-// (no such item, no vertex gradient requested, a zero weight).  Query and ray gradients
-// are written here.
+// Batched backward, one wave per 64-item chunk of the forward chunk list (closest-point
+// chunks, then ray chunks).  Query and ray gradients are written here.  Vertex terms are
+// scatter-added with FP32 atomics when `keys` is null; otherwise item (chunk c, lane l)
+// writes them to term slots 3 (64 c + l) .. + 2, keyed by (mesh << vbits | vertex), or
+// `none` when it adds nothing (no such item, no vertex gradient requested), for the
+// fixed-order sum.
 template <bool RAYS>
-__global__ __launch_bounds__(256) void k_batch_bwd_terms(const BatchMeshDev* __restrict__ meshes,
-                                                         const BatchGradDev* __restrict__ grads,
-                                                         const uint2* __restrict__ chunks, uint32_t n_chunks,
-                                                         uint32_t first_chunk, int vbits, uint64_t none,
-                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                         float* __restrict__ terms) {
+__global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restrict__ meshes,
+                                                   const BatchGradDev* __restrict__ grads,
+                                                   const uint2* __restrict__ chunks, uint32_t n_chunks,
+                                                   uint32_t first_chunk, int vbits, uint64_t none,
+                                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                   float* __restrict__ terms) {
     const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (c >= n_chunks) return;
     const uint2 ch = chunks[c];
@@ -264,6 +265,10 @@ __global__ __launch_bounds__(256) void k_batch_bwd_terms(const BatchMeshDev* __r
         if (i < md.n_q && gd.g_dist)
             t = cpq_bwd_terms(i, md.q, md.q_prim, gd.g_dist, gd.tri_idx, gd.n_tris, gd.verts, gd.vstride, gd.g_q);
     }
+    if (!keys) {
+        scatter_terms(t, gd.g_verts);
+        return;
+    }
     const size_t k0 = 3 * (64 * ((size_t)first_chunk + c) + (threadIdx.x & 63u));
 #pragma unroll
     for (int j = 0; j < 3; j++) {
@@ -276,29 +281,19 @@ __global__ __launch_bounds__(256) void k_batch_bwd_terms(const BatchMeshDev* __r
     }
 }
 
-// One thread per sorted term; the first term of a (mesh, vertex) run adds the run, in
-// term order, to that mesh's vertex gradient (cleared by k_batch_zero_vgrads).
-__global__ __launch_bounds__(kBlock) void k_batch_vertex_runs(uint32_t m, const uint64_t* __restrict__ keys,
-                                                              const uint32_t* __restrict__ vals,
-                                                              const float* __restrict__ terms, int vbits,
-                                                              uint64_t none, const BatchGradDev* __restrict__ grads) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= m) return;
-    const uint64_t key = keys[p];
-    if (key >= none || (p > 0 && keys[p - 1] == key)) return;
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (uint32_t r = p; r < m && keys[r] == key; r++) {
-        const size_t k = vals[r];
-        sx += terms[3 * k];
-        sy += terms[3 * k + 1];
-        sz += terms[3 * k + 2];
+// adds a (mesh << vbits | vertex) total to that mesh's vertex gradient (cleared by
+// k_batch_zero_vgrads; each key has one run)
+struct MeshVertexSink {
+    const BatchGradDev* grads;
+    int vbits;
+    __device__ void operator()(uint64_t key, float x, float y, float z) const {
+        float* g = grads[key >> vbits].g_verts;
+        const size_t v = key & ((1ull << vbits) - 1);
+        g[3 * v] += x;
+        g[3 * v + 1] += y;
+        g[3 * v + 2] += z;
     }
-    float* g = grads[key >> vbits].g_verts;
-    const size_t v = key & ((1ull << vbits) - 1);
-    g[3 * v] += sx;
-    g[3 * v + 1] += sy;
-    g[3 * v + 2] += sz;
-}
+};
 
 }  // namespace gpurt_impl
 
@@ -596,8 +591,10 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
             for (uint64_t f = 0; f < 3 * (uint64_t)hm.n_verts; f += kZeroSeg) segs.push_back(make_uint2((uint32_t)m, (uint32_t)f));
     }
     // vertex terms: 3 per item slot of every chunk, keyed by (mesh, vertex)
+    // (fixed-order sums only; the default adds them with atomics)
+    const bool fixed = b->ctx->deterministic && !segs.empty();
     const uint32_t n_chunks = b->n_cpq_chunks + b->n_ray_chunks;
-    const uint64_t m64 = 3ull * 64 * n_chunks;
+    const uint64_t m64 = fixed ? 3ull * 64 * n_chunks : 0;
     if (m64 > 0xFFFFFFFFull - kBlock) return set_error(GPURT_E_INVALID, "too many queries for a batched backward");
     const uint32_t m = (uint32_t)m64;
     uint32_t max_verts = 1;
@@ -608,7 +605,7 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
     if (vbits + mbits > 63) return set_error(GPURT_E_INVALID, "batch too large for a batched backward");
     const uint64_t none = (uint64_t)n_meshes << vbits;
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t tbytes = 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) + up(12 * (size_t)m) +
+    const size_t tbytes = 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) + 2 * up(12 * (size_t)m) +
                           up(4 * radix_sort_scratch_words(std::max<uint32_t>(m, 1)));
     DevGuard dg(b->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
@@ -620,14 +617,20 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         b->drop_backward();
         return set_error(GPURT_E_OOM, "batch gradient allocation failed");
     }
-    char* tp = static_cast<char*>(b->terms_mem);
-    uint64_t* k0 = reinterpret_cast<uint64_t*>(tp);
-    uint64_t* k1 = reinterpret_cast<uint64_t*>(tp + up(8 * (size_t)m));
-    uint32_t* v0 = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m));
-    uint32_t* v1 = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m) + up(4 * (size_t)m));
-    float* terms = reinterpret_cast<float*>(tp + 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m));
-    uint32_t* sort_scratch = reinterpret_cast<uint32_t*>(tp + 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) +
-                                                         up(12 * (size_t)m));
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    uint32_t *v0 = nullptr, *v1 = nullptr, *sort_scratch = nullptr;
+    float *terms = nullptr, *partial = nullptr;
+    if (m) {
+        char* tp = static_cast<char*>(b->terms_mem);
+        const size_t m8 = up(8 * (size_t)m), m4 = up(4 * (size_t)m), m12 = up(12 * (size_t)m);
+        k0 = reinterpret_cast<uint64_t*>(tp);
+        k1 = reinterpret_cast<uint64_t*>(tp + m8);
+        v0 = reinterpret_cast<uint32_t*>(tp + 2 * m8);
+        v1 = reinterpret_cast<uint32_t*>(tp + 2 * m8 + m4);
+        terms = reinterpret_cast<float*>(tp + 2 * m8 + 2 * m4);
+        partial = reinterpret_cast<float*>(tp + 2 * m8 + 2 * m4 + m12);
+        sort_scratch = reinterpret_cast<uint32_t*>(tp + 2 * m8 + 2 * m4 + 2 * m12);
+    }
     if (hipMemcpy(b->grads_d, gd.data(), sizeof(BatchGradDev) * n_meshes, hipMemcpyHostToDevice) != hipSuccess ||
         (!segs.empty() &&
          hipMemcpy(b->zero_segs_d, segs.data(), sizeof(uint2) * segs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
@@ -644,17 +647,19 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         hipLaunchKernelGGL(k_batch_zero_vgrads, dim3((unsigned)segs.size()), dim3(256), 0, b->cap, b->grads_d,
                            b->zero_segs_d);
     if (b->n_cpq_chunks)
-        hipLaunchKernelGGL(k_batch_bwd_terms<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap,
-                           b->meshes_d, b->grads_d, b->chunks_d, b->n_cpq_chunks, 0u, vbits, none, k0, v0, terms);
+        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
+                           b->grads_d, b->chunks_d, b->n_cpq_chunks, 0u, vbits, none, k0, v0, terms);
     if (b->n_ray_chunks)
-        hipLaunchKernelGGL(k_batch_bwd_terms<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap,
-                           b->meshes_d, b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks, b->n_cpq_chunks,
-                           vbits, none, k0, v0, terms);
+        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
+                           b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks, b->n_cpq_chunks, vbits, none,
+                           k0, v0, terms);
     if (m) {
-        // each (mesh, vertex)'s terms in chunk order, then summed without atomics
+        // each (mesh, vertex)'s terms in chunk order, summed in a fixed order
         const bool in1 = radix_sort_pairs(k0, v0, k1, v1, m, vbits + mbits, sort_scratch, b->cap);
-        hipLaunchKernelGGL(k_batch_vertex_runs, dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m, in1 ? k1 : k0,
-                           in1 ? v1 : v0, terms, vbits, none, b->grads_d);
+        hipLaunchKernelGGL(k_wave_segsum<kBlock>, dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m, in1 ? k1 : k0,
+                           in1 ? v1 : v0, terms, partial);
+        hipLaunchKernelGGL((k_run_sum<kBlock, MeshVertexSink>), dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m,
+                           in1 ? k1 : k0, partial, none, MeshVertexSink{b->grads_d, vbits});
     }
     hipError_t el = hipGetLastError();
     hipGraph_t g = nullptr;

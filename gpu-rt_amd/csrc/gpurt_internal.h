@@ -125,6 +125,8 @@ struct gpurt_ctx_s {
     int n_cu = 256;
     int lds_per_cu = 160 * 1024;
     uint32_t* err = nullptr;  // sticky device error bits (bit 0: traversal stack overflow)
+    // vertex-gradient sums in a fixed order (sort by vertex) instead of FP32 atomics
+    bool deterministic = false;
     // stream on which destroyed handles release their memory, after waiting for the
     // recorded uses of the handle (stream-ordered, no host wait)
     hipStream_t release = nullptr;

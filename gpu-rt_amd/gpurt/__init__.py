@@ -84,7 +84,7 @@ def _check_out(n, device, **arrays):
 class Context:
     """Library context on one HIP device (gpurt_ctx_create)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, deterministic=False):
         lib = _lib.load()
         if isinstance(device, torch.device):
             device = device.index or 0
@@ -93,6 +93,18 @@ class Context:
         _lib.check(lib.gpurt_ctx_create(int(device), ctypes.byref(h)))
         self._h = h
         _track("ctx", self)
+        self.deterministic = deterministic
+
+    @property
+    def deterministic(self):
+        """Vertex-gradient sums in a fixed order (the same bits every run) instead of
+        FP32 atomics (gpurt_ctx_set_deterministic)."""
+        return self._det
+
+    @deterministic.setter
+    def deterministic(self, on):
+        _lib.check(_lib.load().gpurt_ctx_set_deterministic(self._h, int(bool(on))))
+        self._det = bool(on)
 
     @property
     def handle(self):

@@ -124,12 +124,13 @@ def test_batch_with_treelets_matches_oracle(ctx, dev):
         assert np.array_equal(r["r_t"].cpu().numpy(), rt)
 
 
-@pytest.mark.parametrize("fmt", [1, 2])
-def test_batch_backward_matches_plain_backward(ctx, dev, fmt):
+@pytest.mark.parametrize("fmt,det", [(1, False), (2, False), (2, True)])
+def test_batch_backward_matches_plain_backward(dev, fmt, det):
     """The recorded batched backward gives the plain per-mesh backward's gradients:
     query/origin/direction gradients bit for bit (same per-query arithmetic), vertex
-    gradients up to the order of the FP32 atomic sums; a second replay clears and
-    recomputes the vertex gradients."""
+    gradients up to the order of the FP32 sums; a second replay clears and recomputes the
+    vertex gradients, with the same bits on a deterministic context."""
+    ctx = gpurt.Context(0, deterministic=det)
     host = _small_meshes(seed=11)
     dm = _dev(host, dev)
     batch = gpurt.Batch(ctx, dm, node_format=fmt)
@@ -139,10 +140,16 @@ def test_batch_backward_matches_plain_backward(ctx, dev, fmt):
     for g in grads:
         g["g_dist"].uniform_(0.5, 2.0, generator=gen)
         g["g_t"].uniform_(0.5, 2.0, generator=gen)
+    first = None
     for rep in range(2):
         batch.run()
         batch.backward()
         torch.cuda.synchronize()
+        if det:
+            now = [g["g_verts"].clone() for g in grads]
+            if first is not None:
+                assert all(torch.equal(a, b) for a, b in zip(first, now))
+            first = now
         for i, (m, r, g) in enumerate(zip(dm, res, grads)):
             bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_format=fmt)
             prim, pt, dist, bary = bvh.closest_point(m["queries"])
@@ -168,6 +175,9 @@ def test_batch_backward_matches_plain_backward(ctx, dev, fmt):
                                    g["g_dist"].cpu().numpy().astype(np.float64))
             ok = r["q_dist"].cpu().numpy() > 1e-3
             np.testing.assert_allclose(g["g_q"].cpu().numpy()[ok], eq[ok], atol=1e-3, rtol=1e-3)
+            bvh.close()
+    batch.close()
+    ctx.close()
 
 
 def test_batch_backward_saved_outputs(ctx, dev):
