@@ -390,25 +390,6 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
 
     ms_p = timed(plain, 2, 1)
     torch.cuda.synchronize()
-    # the autotuner over the space the batched workload names (node width {2,4,8}, stack
-    # depth {16,32,64}, waves per CU), untimed; then the tuned batch is timed like the
-    # default one
-    tuned = None
-    if not args.no_tune:
-        from gpurt import autotune
-        t0 = time.perf_counter()
-        tuned = autotune.tune_batch(ctx, meshes, use_cache=False, reps=3,
-                                    space={"node_width": (2, 4, 8), "stack_depth": (16, 32, 64),
-                                           "waves_per_cu": (8, 16, 24), "treelet_size": (args.batch_treelet,),
-                                           "node_format": (args.format,)})
-        tune_s = time.perf_counter() - t0
-        tbatch = gpurt.Batch(ctx, meshes, leaf_size=args.leaf, **tuned["build"], **tuned["query"])
-        ms_t = timed(tbatch.run, max(3, args.steps // 4), 1)
-        tres = tbatch.run()
-        torch.cuda.synchronize()
-        parity_tuned = all(torch.equal(a["q_prim"], b["q_prim"]) and torch.equal(a["r_t"], b["r_t"])
-                           for a, b in zip(tres, res))
-        tbatch.close()
     parity = all(torch.equal(plain_out[i][0], r["q_prim"]) and torch.equal(plain_out[i][1], r["q_dist"])
                  and torch.equal(plain_out[i][2], r["r_prim"]) and torch.equal(plain_out[i][3], r["r_t"])
                  for i, r in enumerate(res))
@@ -446,6 +427,36 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
                      and float((plain_g[i][2] - g["g_verts"]).abs().max()) <=
                      1e-5 * (float(plain_g[i][2].abs().max()) + 1e-6)
                      for i, g in enumerate(grads))
+    # the batched backward with fixed-order (deterministic) vertex sums
+    ctx.deterministic = True
+    dgrads = batch.enable_backward()
+    ctx.deterministic = False
+    ms_bfb_det = timed(batched_fb, max(3, args.steps // 4), 1)
+    torch.cuda.synchronize()
+    parity_det = all(torch.equal(a["g_q"], b["g_q"]) and
+                     float((a["g_verts"] - b["g_verts"]).abs().max()) <= 1e-5 * (float(a["g_verts"].abs().max()) + 1e-6)
+                     for a, b in zip(grads, dgrads))
+    # the autotuner over the space the batched workload names (node width {2,4,8}, stack
+    # depth {16,32,64}, waves per CU), untimed; then the tuned batch is timed like the
+    # default one.  It runs after the plain path's timings: the tuner's many batch
+    # creations and destructions leave the HIP runtime slow to service host calls for a
+    # few hundred milliseconds, which would inflate the plain path's host-bound time.
+    tuned = None
+    if not args.no_tune:
+        from gpurt import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune_batch(ctx, meshes, use_cache=False, reps=3,
+                                    space={"node_width": (2, 4, 8), "stack_depth": (16, 32, 64),
+                                           "waves_per_cu": (8, 16, 24), "treelet_size": (args.batch_treelet,),
+                                           "node_format": (args.format,)})
+        tune_s = time.perf_counter() - t0
+        tbatch = gpurt.Batch(ctx, meshes, leaf_size=args.leaf, **tuned["build"], **tuned["query"])
+        ms_t = timed(tbatch.run, max(3, args.steps // 4), 1)
+        tres = tbatch.run()
+        torch.cuda.synchronize()
+        parity_tuned = all(torch.equal(a["q_prim"], b["q_prim"]) and torch.equal(a["r_t"], b["r_t"])
+                           for a, b in zip(tres, res))
+        tbatch.close()
     nq = sum(m["queries"].shape[0] for m in meshes)
     nr = sum(m["rays"].shape[0] for m in meshes)
     batch.close()
@@ -457,6 +468,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "fwd_bwd_batched_units_per_s": (nq + nr) / (ms_bfb * 1e-3), "fwd_bwd_batched_ms": ms_bfb,
             "fwd_bwd_plain_units_per_s": (nq + nr) / (ms_pfb * 1e-3), "fwd_bwd_plain_ms": ms_pfb,
             "fwd_bwd_batched_over_plain": ms_pfb / ms_bfb, "batch_create_s": create_s,
+            "fwd_bwd_batched_deterministic_ms": ms_bfb_det, "parity_bwd_deterministic_vs_atomic": bool(parity_det),
             "autotuned": None if tuned is None else {
                 "build": tuned["build"], "query": tuned["query"], "candidates": tuned["candidates"],
                 "tuner_s": tune_s, "tuner_default_ms": tuned["default_ms"], "tuner_best_ms": tuned["objective_ms"],
