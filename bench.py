@@ -158,11 +158,11 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     # keeps both ray kinds in flight) and joins before the next step; the end of one
     # launch, where few lanes are busy, then overlaps the other's work.
     # Each concurrent pass launches 16 waves per CU (half of a CU's 32 wave slots) so that
-    # both are resident together, and refills at 8 idle lanes; alone, a pass keeps the
+    # both are resident together, and refills at 16 idle lanes; alone, a pass keeps the
     # library defaults.
     s_any = torch.cuda.Stream(dev)
     waves_overlap = args.waves or 16
-    refill_overlap = args.refill_trigger or 8
+    refill_overlap = args.refill_trigger or 16
 
     def step_overlapped():
         fork = torch.cuda.Event()
@@ -336,26 +336,28 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     gq = torch.empty_like(q0)
     go = torch.empty((nr, 3), device=dev)
     gv = torch.zeros((tv.shape[0], 3), device=dev)
-    ms_cb = timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2)
-    ms_rb = timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)
-    # vertex terms are FP32 atomic adds: 9 per query off the surface and per hitting ray
+    # the backward launches in each vertex-sum mode: FP32 atomics (9 adds per query off
+    # the surface and per hitting ray), the fixed order (sort by vertex, segmented sums),
+    # and the default, which picks the faster for the call's size
+    bwd = {}
+    for mode in ("atomic", "fixed", "auto"):
+        ctx.vertex_sums = mode
+        bwd[mode] = {"cpq_ms": timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2),
+                     "ray_ms": timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)}
+    ctx.vertex_sums = "auto"
+    ms_cb, ms_rb = bwd["auto"]["cpq_ms"], bwd["auto"]["ray_ms"]
     n_cq_atomics = 9 * int((dist > 0).sum())
     n_r_atomics = 9 * int((rp >= 0).sum())
-    # the fixed-order (deterministic) vertex sums: sort by vertex, two-pass segmented sums
-    ctx.deterministic = True
-    ms_cb_det = timed(lambda: bvh.closest_point_bwd(q0, prim, bary, pt, dist, g1, gq, gv), args.steps, 2)
-    ms_rb_det = timed(lambda: bvh.trace_bwd(rays, rp, rt, ruv, g2, go, None, gv), args.steps, 2)
-    ctx.deterministic = False
     del bvh
     return {"workload": "diff", "n_tris": int(F.shape[0]), "n_queries": nq, "n_rays": nr,
             "fwd_units_per_s": (nq + nr) / (ms_f * 1e-3), "fwd_ms": ms_f,
             "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb,
             "fwd_bwd_over_fwd": ms_fb / ms_f,
-            "cpq_bwd_roofline": roof(nq * CPQ_BWD_BYTES, ms_cb, "k_cpq_bwd"),
-            "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "k_ray_bwd"),
-            "cpq_bwd_atomics_per_s": n_cq_atomics / (ms_cb * 1e-3),
-            "ray_bwd_atomics_per_s": n_r_atomics / (ms_rb * 1e-3),
-            "cpq_bwd_deterministic_ms": ms_cb_det, "ray_bwd_deterministic_ms": ms_rb_det}
+            "cpq_bwd_roofline": roof(nq * CPQ_BWD_BYTES, ms_cb, "closest-point backward (default vertex sums)"),
+            "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "hit-distance backward (default vertex sums)"),
+            "bwd_ms_by_vertex_sums": bwd,
+            "cpq_bwd_atomic_rate_per_s": n_cq_atomics / (bwd["atomic"]["cpq_ms"] * 1e-3),
+            "ray_bwd_atomic_rate_per_s": n_r_atomics / (bwd["atomic"]["ray_ms"] * 1e-3)}
 
 
 def batched_bench(args, torch, gpurt, W, dev, ctx):
@@ -427,11 +429,12 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
                      and float((plain_g[i][2] - g["g_verts"]).abs().max()) <=
                      1e-5 * (float(plain_g[i][2].abs().max()) + 1e-6)
                      for i, g in enumerate(grads))
-    # the batched backward with fixed-order (deterministic) vertex sums
-    ctx.deterministic = True
+    # the batched backward with FP32 atomic vertex sums (the default picks the fixed order
+    # for a batch this size)
+    ctx.vertex_sums = "atomic"
     dgrads = batch.enable_backward()
-    ctx.deterministic = False
-    ms_bfb_det = timed(batched_fb, max(3, args.steps // 4), 1)
+    ctx.vertex_sums = "auto"
+    ms_bfb_atomic = timed(batched_fb, max(3, args.steps // 4), 1)
     torch.cuda.synchronize()
     parity_det = all(torch.equal(a["g_q"], b["g_q"]) and
                      float((a["g_verts"] - b["g_verts"]).abs().max()) <= 1e-5 * (float(a["g_verts"].abs().max()) + 1e-6)
@@ -468,7 +471,7 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "fwd_bwd_batched_units_per_s": (nq + nr) / (ms_bfb * 1e-3), "fwd_bwd_batched_ms": ms_bfb,
             "fwd_bwd_plain_units_per_s": (nq + nr) / (ms_pfb * 1e-3), "fwd_bwd_plain_ms": ms_pfb,
             "fwd_bwd_batched_over_plain": ms_pfb / ms_bfb, "batch_create_s": create_s,
-            "fwd_bwd_batched_deterministic_ms": ms_bfb_det, "parity_bwd_deterministic_vs_atomic": bool(parity_det),
+            "fwd_bwd_batched_atomic_sums_ms": ms_bfb_atomic, "parity_bwd_atomic_vs_fixed_sums": bool(parity_det),
             "autotuned": None if tuned is None else {
                 "build": tuned["build"], "query": tuned["query"], "candidates": tuned["candidates"],
                 "tuner_s": tune_s, "tuner_default_ms": tuned["default_ms"], "tuner_best_ms": tuned["objective_ms"],
@@ -571,7 +574,8 @@ def profile_mode(args, torch, gpurt, W, dev, ctx):
     FETCH_SIZE/WRITE_SIZE calibrate the counters)."""
     kind = args.profile_workload
     w = W.rays_scene() if kind == "rays_scene" else W.diff() if kind.startswith("diff_bwd") else getattr(W, kind)()
-    ctx.deterministic = kind == "diff_bwd_fixed"
+    if kind.startswith("diff_bwd"):
+        ctx.vertex_sums = "fixed" if kind == "diff_bwd_fixed" else "atomic"
     tv = torch.from_numpy(w["verts"]).to(dev)
     tf = torch.from_numpy(w["tris"]).to(dev)
     bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
@@ -655,8 +659,8 @@ def main():
     ap.add_argument("--profile-workload", default="rays_scene",
                     choices=["rays_scene", "cpq_small", "cpq_large", "diff_bwd", "diff_bwd_fixed"],
                     help="kernel of the --profile run: closest-hit rays, closest-point queries, or the "
-                         "backward passes of the differentiable workload (atomic or fixed-order "
-                         "vertex sums)")
+                         "backward passes of the differentiable workload (FP32 atomic or "
+                         "fixed-order vertex sums)")
     ap.add_argument("--no-tune", action="store_true", help="batched leg: skip the autotuner sweep")
     ap.add_argument("--only", choices=["diff", "cpq_large", "batched", "tracer"],
                     help="print only this leg (for profiling it)")

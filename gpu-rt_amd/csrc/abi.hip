@@ -237,9 +237,11 @@ int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear) {
     return GPURT_OK;
 }
 
-int gpurt_ctx_set_deterministic(gpurt_ctx ctx, int on) {
+int gpurt_ctx_set_vertex_sums(gpurt_ctx ctx, int mode) {
     if (!ctx) return set_error(GPURT_E_INVALID, "null context");
-    ctx->deterministic = on != 0;
+    if (mode != GPURT_SUMS_AUTO && mode != GPURT_SUMS_FIXED_ORDER && mode != GPURT_SUMS_ATOMIC)
+        return set_error(GPURT_E_INVALID, "unknown vertex-sum mode");
+    ctx->vertex_sums = mode;
     return GPURT_OK;
 }
 

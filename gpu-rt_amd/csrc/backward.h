@@ -9,9 +9,9 @@
 //             dt/do = -n/(n.d),  dt/dd = -t n/(n.d),  dt/dv_j = b_j n/(n.d)
 //
 // Vertex positions are read with a stride of `vs` floats; vertex gradients are n x 3.
-// The terms of one query or ray are formed here.  By default they are scatter-added with
-// hardware FP32 atomics (order, and so the last bits, vary between runs); a context set
-// to deterministic sums sorts them by vertex and adds them in a fixed order instead.
+// The terms of one query or ray are formed here.  They are either scatter-added with
+// hardware FP32 atomics (the order, and so the last bits, vary between runs) or sorted by
+// vertex and added in a fixed order (below); the context's vertex-sum mode picks one.
 #pragma once
 
 #include "gpurt_internal.h"
@@ -142,7 +142,7 @@ __device__ __forceinline__ CornerTerms ray_bwd_terms(size_t i, const float4* __r
     return ct;
 }
 
-// hardware FP32 atomic scatter of the terms (the default, non-deterministic sums)
+// hardware FP32 atomic scatter of the terms (the order of the adds varies between runs)
 __device__ __forceinline__ void scatter_terms(const CornerTerms& t, float* g_verts) {
     if (!g_verts) return;
 #pragma unroll
@@ -155,7 +155,7 @@ __device__ __forceinline__ void scatter_terms(const CornerTerms& t, float* g_ver
     }
 }
 
-// ---- deterministic vertex sums -------------------------------------------------------
+// ---- fixed-order vertex sums ---------------------------------------------------------
 // The terms, sorted by key (a vertex, or (mesh, vertex) in a batch; `none` for terms that
 // add nothing), are summed per key in two passes with a fixed order (the same bits on
 // every run), and without a serial loop over a long run:

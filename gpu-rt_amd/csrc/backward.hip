@@ -3,13 +3,14 @@
 // Both passes reuse the forward pass's saved primitive ids and barycentrics (no
 // re-traversal; the per-query math is in backward.h).  Query / ray gradients are written
 // directly.  Vertex gradients are sums over every query whose triangle uses the vertex:
-//   default        k_cpq_bwd / k_ray_bwd add each query's three vertex terms with FP32
-//                  atomics: one launch, bound by the atomics (~9 random device-scope FP32
-//                  atomics per query; the summation order, and so the last bits, vary)
-//   deterministic  (gpurt_ctx_set_deterministic) k_*_bwd_terms write the terms with the
-//                  vertex they belong to, a stable radix sort orders them by vertex, and
-//                  k_wave_segsum + k_run_sum add each vertex's terms in a fixed order: the
-//                  same bits on every run
+//   atomic       k_cpq_bwd / k_ray_bwd add each query's three vertex terms with FP32
+//                atomics: one launch, bound by the atomics (~9 random device-scope FP32
+//                atomics per query; the summation order, and so the last bits, vary)
+//   fixed order  k_*_bwd_terms write the terms with the vertex they belong to, a stable
+//                radix sort orders them by vertex, and k_wave_segsum + k_run_sum add each
+//                vertex's terms in a fixed order: the same bits on every run
+// The context's mode picks one (gpurt_ctx_set_vertex_sums); by default the fixed order
+// runs for calls of at least kFixedOrderMinTerms terms, where it is also the faster.
 // Algorithmic bytes: a closest-point query reads q, g, prim and its triangle (12 + 4 + 4
 // + 12 + 36) and writes g_q (12) and adds to three vertices (3 x 2 x 12); a ray reads its
 // direction, t, uv, g and its triangle.
@@ -154,7 +155,7 @@ void sum_terms(TermScratch& t, size_t n, uint32_t n_verts, float* g_verts, hipSt
 int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim, const float* g,
                    float* g_q, float* g_verts, hipStream_t s) {
     if (n == 0) return GPURT_OK;
-    if (!b->ctx->deterministic || !g_verts) {
+    if (!g_verts || !fixed_order_sums(b->ctx, 3 * n)) {
         hipLaunchKernelGGL(k_cpq_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim, g,
                            b->tri_idx, (uint32_t)b->n_tris, b->verts, g_q, g_verts);
         GPURT_HIP(hipGetLastError());
@@ -177,7 +178,7 @@ int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_
                    float* g_verts, hipStream_t s) {
     if (n == 0) return GPURT_OK;
     const bool vgrad = g_verts && uv;
-    if (!b->ctx->deterministic || !vgrad) {
+    if (!vgrad || !fixed_order_sums(b->ctx, 3 * n)) {
         hipLaunchKernelGGL(k_ray_bwd, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n,
                            reinterpret_cast<const float4*>(rays), prim, t, uv, g, b->tri_idx, (uint32_t)b->n_tris,
                            b->verts, g_o, g_d, g_verts);

@@ -591,9 +591,9 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
             for (uint64_t f = 0; f < 3 * (uint64_t)hm.n_verts; f += kZeroSeg) segs.push_back(make_uint2((uint32_t)m, (uint32_t)f));
     }
     // vertex terms: 3 per item slot of every chunk, keyed by (mesh, vertex)
-    // (fixed-order sums only; the default adds them with atomics)
-    const bool fixed = b->ctx->deterministic && !segs.empty();
+    // (for the fixed-order sums; otherwise they are added with atomics)
     const uint32_t n_chunks = b->n_cpq_chunks + b->n_ray_chunks;
+    const bool fixed = !segs.empty() && fixed_order_sums(b->ctx, 3ull * 64 * n_chunks);
     const uint64_t m64 = fixed ? 3ull * 64 * n_chunks : 0;
     if (m64 > 0xFFFFFFFFull - kBlock) return set_error(GPURT_E_INVALID, "too many queries for a batched backward");
     const uint32_t m = (uint32_t)m64;

@@ -125,8 +125,8 @@ struct gpurt_ctx_s {
     int n_cu = 256;
     int lds_per_cu = 160 * 1024;
     uint32_t* err = nullptr;  // sticky device error bits (bit 0: traversal stack overflow)
-    // vertex-gradient sums in a fixed order (sort by vertex) instead of FP32 atomics
-    bool deterministic = false;
+    // GPURT_SUMS_* mode of the backward passes' vertex-gradient sums
+    int vertex_sums = 0;
     // stream on which destroyed handles release their memory, after waiting for the
     // recorded uses of the handle (stream-ordered, no host wait)
     hipStream_t release = nullptr;
@@ -136,6 +136,21 @@ struct gpurt_ctx_s {
     std::set<gpurt_bvh_s*> bvhs;
     std::set<gpurt_tlas_s*> tlases;
 };
+
+namespace gpurt_impl {
+// Vertex-gradient sums of the backward passes: the fixed-order sum (sort by vertex, then
+// segmented sums) when the context asks for it, FP32 atomics when it asks for them, and
+// otherwise whichever is faster -- the fixed order from about 2 M vertex terms, where
+// the atomics' device-wide rate (~20 G/s) costs more than the sort's passes (1 M
+// closest-point queries = 3.1 M terms: 0.38 vs 0.48 ms; a 256-mesh batch of 19 M queries
+// and rays: forward + backward 32.3 vs 36.1 ms; 262 K rays = 0.8 M terms: 0.17 vs 0.13
+// ms).
+constexpr size_t kFixedOrderMinTerms = size_t(1) << 21;
+inline bool fixed_order_sums(const gpurt_ctx_s* c, size_t terms) {
+    return c->vertex_sums == GPURT_SUMS_FIXED_ORDER ||
+           (c->vertex_sums == GPURT_SUMS_AUTO && terms >= kFixedOrderMinTerms);
+}
+}  // namespace gpurt_impl
 
 struct gpurt_bvh_s {
     gpurt_ctx ctx = nullptr;

@@ -129,9 +129,13 @@ static int dispatch_stack(int S, QKind kind, const TravArgs& a, unsigned grid, h
 // 1M rays: closest 3.29 -> 4.05, any hit 4.51 -> 5.70 G rays/s against a refill on every
 // finished lane at 8 waves/SIMD; closest-point queries near a 1.3M-triangle surface: 0.88
 // -> 1.06 G queries/s at 4 waves/SIMD.
+// Rays re-swept on rays_scene after the two-visit node step: closest hit at 28 waves per
+// CU and a refill at 24 idle lanes 218 us (24 waves, 16 lanes: 230 us), any hit at 28
+// waves and 16 lanes 148-151 us (158 us); 32 waves crowd the L2 (+5%).  (The TLAS
+// passes its own launch options.)
 static uint32_t default_leaf_trigger(QKind) { return 32u; }
-static uint32_t default_refill_trigger(QKind) { return 16u; }
-static int default_waves_per_cu(QKind kind) { return kind == QKind::Cpq ? 16 : 24; }
+static uint32_t default_refill_trigger(QKind kind) { return kind == QKind::Closest ? 24u : 16u; }
+static int default_waves_per_cu(QKind kind) { return kind == QKind::Cpq ? 16 : 28; }
 
 static int block_for_stack(int S) { return S <= 16 ? 256 : (S == 32 ? 128 : 64); }
 

@@ -84,7 +84,7 @@ def _check_out(n, device, **arrays):
 class Context:
     """Library context on one HIP device (gpurt_ctx_create)."""
 
-    def __init__(self, device=0, deterministic=False):
+    def __init__(self, device=0, vertex_sums="auto"):
         lib = _lib.load()
         if isinstance(device, torch.device):
             device = device.index or 0
@@ -93,18 +93,23 @@ class Context:
         _lib.check(lib.gpurt_ctx_create(int(device), ctypes.byref(h)))
         self._h = h
         _track("ctx", self)
-        self.deterministic = deterministic
+        self.vertex_sums = vertex_sums
+
+    _SUMS = {"auto": _lib.SUMS_AUTO, "fixed": _lib.SUMS_FIXED_ORDER, "atomic": _lib.SUMS_ATOMIC}
 
     @property
-    def deterministic(self):
-        """Vertex-gradient sums in a fixed order (the same bits every run) instead of
-        FP32 atomics (gpurt_ctx_set_deterministic)."""
-        return self._det
+    def vertex_sums(self):
+        """How backward passes add vertex-gradient terms (gpurt_ctx_set_vertex_sums):
+        "fixed" (sorted by vertex, a fixed order: the same bits every run), "atomic"
+        (FP32 atomics) or "auto" (the faster of the two for each call's size)."""
+        return self._sums
 
-    @deterministic.setter
-    def deterministic(self, on):
-        _lib.check(_lib.load().gpurt_ctx_set_deterministic(self._h, int(bool(on))))
-        self._det = bool(on)
+    @vertex_sums.setter
+    def vertex_sums(self, mode):
+        if mode not in self._SUMS:
+            raise ValueError(f"vertex_sums must be one of {sorted(self._SUMS)}, got {mode!r}")
+        _lib.check(_lib.load().gpurt_ctx_set_vertex_sums(self._h, self._SUMS[mode]))
+        self._sums = mode
 
     @property
     def handle(self):

@@ -17,6 +17,7 @@ GPURT_E_DEVICE = -4
 GPURT_E_UNSUPPORTED = -5
 STATUS_STACK_OVERFLOW = 1
 STATUS_TLAS_OUTSIDE_DOMAIN = 2
+SUMS_AUTO, SUMS_FIXED_ORDER, SUMS_ATOMIC = 0, 1, 2
 
 _ERR_NAMES = {
     GPURT_E_INVALID: "GPURT_E_INVALID",
@@ -28,7 +29,7 @@ _ERR_NAMES = {
 
 # Every symbol include/gpurt.h declares.
 EXPORTS = (
-    "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_ctx_status", "gpurt_ctx_set_deterministic", "gpurt_bvh_build", "gpurt_bvh_refit",
+    "gpurt_ctx_create", "gpurt_ctx_destroy", "gpurt_ctx_status", "gpurt_ctx_set_vertex_sums", "gpurt_bvh_build", "gpurt_bvh_refit",
     "gpurt_bvh_destroy", "gpurt_bvh_get_info", "gpurt_bvh_export", "gpurt_closest_point",
     "gpurt_trace_closest",
     "gpurt_trace_closest_noalloc", "gpurt_trace_any", "gpurt_closest_point_bwd", "gpurt_trace_bwd",
@@ -144,7 +145,7 @@ def load():
         "gpurt_ctx_create": (I, [I, ctypes.POINTER(P)]),
         "gpurt_ctx_destroy": (V, [P]),
         "gpurt_ctx_status": (I, [P, ctypes.POINTER(ctypes.c_uint32), I]),
-        "gpurt_ctx_set_deterministic": (I, [P, I]),
+        "gpurt_ctx_set_vertex_sums": (I, [P, I]),
         "gpurt_bvh_build": (I, [P, P, S, S, P, S, ctypes.POINTER(BuildOpts), P, ctypes.POINTER(P)]),
         "gpurt_bvh_refit": (I, [P, P, S, P]),
         "gpurt_bvh_destroy": (V, [P]),

@@ -140,15 +140,23 @@ GPURT_API void gpurt_ctx_destroy(gpurt_ctx ctx);
 #define GPURT_STATUS_TLAS_OUTSIDE_DOMAIN 2u
 GPURT_API int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear);
 
-/* Vertex-gradient sums of the backward passes (gpurt_closest_point_bwd, gpurt_trace_bwd,
- * gpurt_batch_set_grads).  Off (the default): every query's three vertex terms are added
- * with FP32 atomics, one pass, and the summation order, so the last bits, can vary from
- * run to run.  On: the terms are sorted by vertex (a stable radix sort) and each vertex's
- * terms added in a fixed order (a tree over query order), the same bits on every run, for 144 bytes of stream-ordered
- * scratch per query and about twice the time.  Applies to calls made after it; a batch
- * reads it in gpurt_batch_set_grads.  No counterpart in the reference (it has no
- * gradients). */
-GPURT_API int gpurt_ctx_set_deterministic(gpurt_ctx ctx, int on);
+/* How the backward passes (gpurt_closest_point_bwd, gpurt_trace_bwd, gpurt_batch_set_grads)
+ * add the per-query vertex terms (three per query or ray) into the vertex gradients:
+ *   GPURT_SUMS_FIXED_ORDER  the terms are sorted by vertex (a stable radix sort) and each
+ *                           vertex's terms added in a fixed order (a tree over query
+ *                           order): the same bits on every run; 144 bytes of
+ *                           stream-ordered scratch per query
+ *   GPURT_SUMS_ATOMIC       FP32 atomic adds, one pass; the summation order, and so the
+ *                           last bits, can vary from run to run
+ *   GPURT_SUMS_AUTO         (the default) whichever is faster for the call: the fixed
+ *                           order from 2^21 terms (~700 K queries; the atomics' device-wide
+ *                           rate then costs more than the sort), atomics below
+ * Applies to calls made after it; a batch reads it in gpurt_batch_set_grads.  An unknown
+ * mode: GPURT_E_INVALID.  No counterpart in the reference (it has no gradients). */
+#define GPURT_SUMS_AUTO 0
+#define GPURT_SUMS_FIXED_ORDER 1
+#define GPURT_SUMS_ATOMIC 2
+GPURT_API int gpurt_ctx_set_vertex_sums(gpurt_ctx ctx, int mode);
 
 /* ---- build ----------------------------------------------------------------------- */
 
@@ -248,8 +256,8 @@ GPURT_API int gpurt_trace_any(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, ui
  * are recomputed in FP64 (the computation the forward pass reports from), then
  * g_q[i] = g_dist[i] (q_i - p_i)/|q_i - p_i|; for the three vertices v_j of prim_i,
  * g_verts[v_j] -= b_j g_q[i]  (g_verts is ACCUMULATED, n_verts x 3: zero it first).
- * Vertex sums use FP32 atomics, or a fixed order on a context set with
- * gpurt_ctx_set_deterministic.  g_q and g_verts are each
+ * Vertex sums use FP32 atomics or a fixed order (gpurt_ctx_set_vertex_sums).  g_q and
+ * g_verts are each
  * nullable; so are the saved bary, pt and dist (accepted for the forward's output
  * layout, not read).  A query on the surface (distance 0) contributes zero. */
 GPURT_API int gpurt_closest_point_bwd(gpurt_bvh bvh, const float* q, size_t n, const int32_t* prim,
@@ -259,7 +267,7 @@ GPURT_API int gpurt_closest_point_bwd(gpurt_bvh bvh, const float* q, size_t n, c
 /* Gradient of sum_i g_t[i] * t_i for closest hits.  With n = (v1-v0) x (v2-v0):
  * dt/do = -n/(n.d), dt/dd = -t n/(n.d), dt/dv_j = b_j n/(n.d), b = (1-u-v,u,v).
  * g_origin / g_dir are n x 3 (written, nullable); g_verts is ACCUMULATED (nullable), with
- * atomics or in a fixed order (gpurt_ctx_set_deterministic).  Misses contribute
+ * atomics or in a fixed order (gpurt_ctx_set_vertex_sums).  Misses contribute
  * zero. */
 GPURT_API int gpurt_trace_bwd(gpurt_bvh bvh, const gpurt_ray* rays, size_t n, const int32_t* prim,
                     const float* t, const float* uv, const float* g_t, float* g_origin,
@@ -312,12 +320,13 @@ typedef struct gpurt_batch_grad {
  * gpurt_batch_create) into a second HIP graph: it clears the vertex gradients, then
  * forms every mesh's gradients from the outputs of the last gpurt_batch_run (saved ids,
  * closest-point triangles recomputed in FP64, hit t, (u,v)) without re-traversal; the
- * vertex terms are added with FP32 atomics, or, on a context set with
- * gpurt_ctx_set_deterministic when this is called, sorted by (mesh, vertex) and each
- * vertex's terms added in a fixed order (the same bits on every run).  Vertex gradients
- * of hit t need the forward's r_uv.  The fixed-order sum's term buffers (allocated
- * here) take 144 bytes per query or ray slot (3 terms of 48 bytes; slots are allotted in
- * 64-item chunks per mesh): about 2.7 GB for 19 M queries and rays.
+ * vertex terms of all meshes are added with FP32 atomics or, by the context's
+ * gpurt_ctx_set_vertex_sums mode when this is called (the default picks it for batches
+ * of 2^21 terms and more), sorted by (mesh, vertex) and each vertex's terms added in a
+ * fixed order (the same bits on every run).  Vertex gradients of hit t need the
+ * forward's r_uv.  The fixed-order sum's term buffers (allocated here) take 144 bytes
+ * per query or ray slot (3 terms of 48 bytes; slots are allotted in 64-item chunks per
+ * mesh): about 2.7 GB for 19 M queries and rays.
  * No counterpart in the reference (it has no gradients); it is the batched form of
  * gpurt_closest_point_bwd + gpurt_trace_bwd.  Calling it again replaces the buffers. */
 GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* grads);

@@ -74,7 +74,7 @@ def test_null_arguments_return_invalid_without_device_work():
     assert lib.gpurt_trace_closest_noalloc(None, None, 1, None, None, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_ctx_create(0, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_ctx_status(None, None, 0) == _lib.GPURT_E_INVALID
-    assert lib.gpurt_ctx_set_deterministic(None, 1) == _lib.GPURT_E_INVALID
+    assert lib.gpurt_ctx_set_vertex_sums(None, 1) == _lib.GPURT_E_INVALID
     assert lib.gpurt_tune_cache_load(None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_tune_batch(None, None, 0, None, None, None) == _lib.GPURT_E_INVALID
     assert lib.gpurt_scene_create(None, None, 1, None, ctypes.byref(out)) == _lib.GPURT_E_INVALID

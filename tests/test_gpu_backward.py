@@ -3,9 +3,9 @@ Functions) against the oracle: FP64 closed forms evaluated at the GPU's argmin, 
 closed forms at the FP64 argmin (brute force), and FP64 central differences.
 
 Vertex gradients are FP32 sums of per-query terms, compared with tolerances (stated per
-test) against the FP64 forms; by default they are added with FP32 atomics (the order, and
-so the last bits, vary), and on a context set deterministic each vertex's terms are added
-in a fixed order, the same bits on every run."""
+test) against the FP64 forms; they are added with FP32 atomics (the order, and so the
+last bits, vary) or sorted by vertex and added in a fixed order, the same bits on every
+run (the context's vertex_sums mode; "auto" picks by size)."""
 import numpy as np
 import pytest
 import torch
@@ -179,12 +179,13 @@ def test_misses_and_zero_distance_give_zero_gradient(ctx, dev):
     assert torch.count_nonzero(q.grad) == 0
 
 
-@pytest.mark.parametrize("det", [False, True])
-def test_plain_vertex_gradient_sums(dev, det):
+@pytest.mark.parametrize("mode", ["atomic", "fixed", "auto"])
+def test_plain_vertex_gradient_sums(dev, mode):
     """The same backward, run five times on the differentiable workload's kind of input
     (many queries per vertex): the sum over every term matches the FP64 closed form in
-    both modes, and a deterministic context gives identical bits every run."""
-    ctx = gpurt.Context(0, deterministic=det)
+    every mode, and fixed-order sums give identical bits every run."""
+    det = mode == "fixed"
+    ctx = gpurt.Context(0, vertex_sums=mode)
     V, F = _sphere(6, 3)
     Q = W.surface_samples(V.astype(np.float64), F, np.random.default_rng(4), 200000, 0.005).astype(np.float32)
     rng = np.random.default_rng(5)
@@ -215,4 +216,14 @@ def test_plain_vertex_gradient_sums(dev, det):
     for r in runs:
         np.testing.assert_allclose(r, ev + erv, atol=1e-5 * scale, rtol=1e-4)
     bvh.close()
+    ctx.close()
+
+
+def test_vertex_sum_modes_are_checked(dev):
+    ctx = gpurt.Context(0)
+    assert ctx.vertex_sums == "auto"
+    with pytest.raises(ValueError):
+        ctx.vertex_sums = "sorted"
+    from gpurt import _lib
+    assert _lib.load().gpurt_ctx_set_vertex_sums(ctx.handle, 7) == _lib.GPURT_E_INVALID
     ctx.close()

@@ -129,8 +129,8 @@ def test_batch_backward_matches_plain_backward(dev, fmt, det):
     """The recorded batched backward gives the plain per-mesh backward's gradients:
     query/origin/direction gradients bit for bit (same per-query arithmetic), vertex
     gradients up to the order of the FP32 sums; a second replay clears and recomputes the
-    vertex gradients, with the same bits on a deterministic context."""
-    ctx = gpurt.Context(0, deterministic=det)
+    vertex gradients, with the same bits under fixed-order sums."""
+    ctx = gpurt.Context(0, vertex_sums="fixed" if det else "atomic")
     host = _small_meshes(seed=11)
     dm = _dev(host, dev)
     batch = gpurt.Batch(ctx, dm, node_format=fmt)
