@@ -22,7 +22,9 @@ The same JSON line carries:
   extra         the other workloads, each with the same roofline fields: closest-point
                 queries/s (5,120 and 5,242,880 triangles), any hit, forward and backward
                 on the differentiable 1.3M-triangle workload, the 256-mesh batch against
-                the plain per-mesh path (default and autotuned parameters), build times.
+                the plain per-mesh path (default and autotuned parameters), the build
+                (bytes: what a build must read and write, 100 B per triangle + the wide
+                nodes), the batch replay (its builds + its queries' visits).
 """
 import argparse
 import json
@@ -48,6 +50,26 @@ def roof(byts, ms, kernel):
     gbs = byts / (ms * 1e-3) / 1e9
     return {"kernel": kernel, "bound": "hbm", "bytes_per_launch": int(byts), "kernel_ms": ms,
             "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
+def cpq_bytes(n, nodes, tris, node_bytes):
+    """Algorithmic bytes of a closest-point launch: query I/O (12 in, 20 out, 12
+    barycentrics) + node_bytes per node visited + 36 per triangle tested."""
+    return n * (12 + 20 + 12) + nodes * node_bytes + tris * TRI_BYTES
+
+
+def ray_bytes(n, nodes, tris, node_bytes, any_hit=False):
+    """Algorithmic bytes of a ray launch: 32 in + 16 out (1 for any hit) per ray, plus
+    node_bytes per node visited and 36 per triangle tested."""
+    return n * (32 + (1 if any_hit else 16)) + nodes * node_bytes + tris * TRI_BYTES
+
+
+def build_bytes(info):
+    """The least a BVH build must move: read each triangle's indices and vertices (12 +
+    36 B), write its packed record and leaf-order entry (48 + 4 B) and every wide node in
+    each encoding kept (cpq_node_bytes, plus node_bytes when rays use 8-bit nodes)."""
+    per_node = info["cpq_node_bytes"] + (info["node_bytes"] if info["node_bytes"] != info["cpq_node_bytes"] else 0)
+    return info["n_tris"] * (12 + 36 + 48 + 4) + info["n_nodes"] * per_node
 
 
 def log(*a):
@@ -184,7 +206,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     bvh.trace_closest(rays, out=(prim, t, uv), stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger, stats=st)
     torch.cuda.synchronize()
     nodes, tris, spills, _ = st.cpu().tolist()
-    bytes_closest = n * (32 + 16) + nodes * info["node_bytes"] + tris * TRI_BYTES
+    bytes_closest = ray_bytes(n, nodes, tris, info["node_bytes"])
     st.zero_()
     bvh.trace_any(rays_any, out=occ, stack_depth=args.stack, waves_per_cu=args.waves, leaf_trigger=args.leaf_trigger, refill_trigger=args.refill_trigger, stats=st)
     torch.cuda.synchronize()
@@ -224,6 +246,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     rl["copy_achievable_GBps"] = copy_gbs
     rl["frac_of_copy"] = rl["achieved"] / copy_gbs
     rl["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
+    build_roof = roof(build_bytes(info), build_ms, "BVH build, all kernels (host-timed call)")
     res = {
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
@@ -246,9 +269,10 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                        "serial_rays_per_s": 2 * n / (ms_serial * 1e-3), "ms_per_step_serial": ms_serial,
                        "closest_rays_per_s": n / (ms_closest * 1e-3),
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
+                       "build_roofline": build_roof,
                        "hit_rate": hit_rate, "occluded_rate": occ_rate,
                        "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
-                       "any_roofline": roof(n * (32 + 1) + nodes_a * info["node_bytes"] + tris_a * TRI_BYTES,
+                       "any_roofline": roof(ray_bytes(n, nodes_a, tris_a, info["node_bytes"], any_hit=True),
                                             ms_any, roof_kernel(args, info, "any")),
                        "bvh": info, "parity_all_rays_vs_cpu_lbvh": parity, "fp64_check": fp64},
     }
@@ -275,7 +299,7 @@ def cpq_bench(name, w, args, torch, gpurt, dev, ctx, O=None, fp64_every=0, parit
     torch.cuda.synchronize()
     nodes, tris, _, _ = st.cpu().tolist()
     info = bvh.info()
-    byts = n * (12 + 20 + 12) + nodes * info["cpq_node_bytes"] + tris * TRI_BYTES
+    byts = cpq_bytes(n, nodes, tris, info["cpq_node_bytes"])
     prim = out[0].cpu().numpy()
     del bvh
     parity = None
@@ -348,11 +372,25 @@ def diff_bench(args, torch, gpurt, W, dev, ctx):
     ms_cb, ms_rb = bwd["auto"]["cpq_ms"], bwd["auto"]["ray_ms"]
     n_cq_atomics = 9 * int((dist > 0).sum())
     n_r_atomics = 9 * int((rp >= 0).sum())
+    # visits of the forward launches (counting variants, untimed) for their bytes
+    info = bvh.info()
+    st = torch.zeros(4, dtype=torch.int64, device=dev)
+    bvh.closest_point(q0, stats=st)
+    torch.cuda.synchronize()
+    cn, ct = st[0].item(), st[1].item()
+    st.zero_()
+    bvh.trace_closest(rays, stats=st)
+    torch.cuda.synchronize()
+    rn, rt_ = st[0].item(), st[1].item()
+    fwd_bytes = cpq_bytes(nq, cn, ct, info["cpq_node_bytes"]) + ray_bytes(nr, rn, rt_, info["node_bytes"])
     del bvh
     return {"workload": "diff", "n_tris": int(F.shape[0]), "n_queries": nq, "n_rays": nr,
             "fwd_units_per_s": (nq + nr) / (ms_f * 1e-3), "fwd_ms": ms_f,
             "fwd_bwd_units_per_s": (nq + nr) / (ms_fb * 1e-3), "fwd_bwd_ms": ms_fb,
             "fwd_bwd_over_fwd": ms_fb / ms_f,
+            "fwd_roofline": roof(fwd_bytes, ms_f, "forward: closest-point + closest-hit launches (autograd calls)"),
+            "fwd_bwd_roofline": roof(fwd_bytes + nq * CPQ_BWD_BYTES + nr * RAY_BWD_BYTES, ms_fb,
+                                     "forward + backward (autograd calls)"),
             "cpq_bwd_roofline": roof(nq * CPQ_BWD_BYTES, ms_cb, "closest-point backward (default vertex sums)"),
             "ray_bwd_roofline": roof(nr * RAY_BWD_BYTES, ms_rb, "hit-distance backward (default vertex sums)"),
             "bwd_ms_by_vertex_sums": bwd,
@@ -429,6 +467,25 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
                      and float((plain_g[i][2] - g["g_verts"]).abs().max()) <=
                      1e-5 * (float(plain_g[i][2].abs().max()) + 1e-6)
                      for i, g in enumerate(grads))
+    # algorithmic bytes of one replay: every mesh's build (build_bytes) and its queries'
+    # visits, counted by the counting variants on the same trees built the plain way
+    # (untimed)
+    replay_bytes = 0
+    st = torch.zeros(4, dtype=torch.int64, device=dev)
+    for m in meshes:
+        bvh = gpurt.Bvh(ctx, m["verts"], m["tris"], node_width=args.width, leaf_size=args.leaf,
+                        treelet_size=args.batch_treelet, node_format=args.format)
+        info = bvh.info()
+        st.zero_()
+        bvh.closest_point(m["queries"], stats=st)
+        torch.cuda.synchronize()
+        replay_bytes += cpq_bytes(m["queries"].shape[0], st[0].item(), st[1].item(), info["cpq_node_bytes"])
+        st.zero_()
+        bvh.trace_closest(m["rays"], stats=st)
+        torch.cuda.synchronize()
+        replay_bytes += ray_bytes(m["rays"].shape[0], st[0].item(), st[1].item(), info["node_bytes"])
+        replay_bytes += build_bytes(info)
+        bvh.close()
     # the batched backward with FP32 atomic vertex sums (the default picks the fixed order
     # for a batch this size)
     ctx.vertex_sums = "atomic"
@@ -468,7 +525,10 @@ def batched_bench(args, torch, gpurt, W, dev, ctx):
             "batched_units_per_s": (nq + nr) / (ms_b * 1e-3), "batched_ms": ms_b,
             "plain_units_per_s": (nq + nr) / (ms_p * 1e-3), "plain_ms": ms_p,
             "batched_over_plain": ms_p / ms_b,
+            "batched_roofline": roof(replay_bytes, ms_b, "batched replay: 256 builds + closest-point and closest-hit launches"),
             "fwd_bwd_batched_units_per_s": (nq + nr) / (ms_bfb * 1e-3), "fwd_bwd_batched_ms": ms_bfb,
+            "fwd_bwd_batched_roofline": roof(replay_bytes + nq * CPQ_BWD_BYTES + nr * RAY_BWD_BYTES, ms_bfb,
+                                             "batched forward + backward replays"),
             "fwd_bwd_plain_units_per_s": (nq + nr) / (ms_pfb * 1e-3), "fwd_bwd_plain_ms": ms_pfb,
             "fwd_bwd_batched_over_plain": ms_pfb / ms_bfb, "batch_create_s": create_s,
             "fwd_bwd_batched_atomic_sums_ms": ms_bfb_atomic, "parity_bwd_atomic_vs_fixed_sums": bool(parity_det),
@@ -624,7 +684,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4, choices=[2, 4, 8])
-    ap.add_argument("--stack", type=int, default=8, choices=[8, 16, 32, 64])
+    ap.add_argument("--stack", type=int, default=0, choices=[0, 8, 16, 32, 64],
+                    help="LDS stack entries per lane (0 = library default: 8 for rays, 16 for "
+                         "closest points)")
     ap.add_argument("--leaf", type=int, default=4, choices=range(1, 9), metavar="1..8",
                     help="max triangles per leaf")
     ap.add_argument("--waves", type=int, default=0, help="resident waves per CU (0 = library choice)")

@@ -411,7 +411,10 @@ int gpurt_batch_create(gpurt_ctx ctx, const gpurt_batch_mesh* meshes, size_t n_m
     int rc0 = parse_build_opts(bopts, bp);
     if (rc0) return rc0;
     const int W = bp.width, leaf = bp.leaf_size, tsize = bp.treelet_size, trounds = bp.treelet_rounds;
-    int S = (qopts && qopts->stack_depth) ? qopts->stack_depth : (W == 8 ? 16 : 8);
+    // 16 stack entries by default: the closest-point searches that dominate a batch spill
+    // with 8, and 16 waves per CU (the default below) fit 16-entry stacks in LDS (256-mesh
+    // batch: 26.8 -> 25.5 ms per replay; the autotuner's pick on that batch)
+    int S = (qopts && qopts->stack_depth) ? qopts->stack_depth : 16;
     if (S != 8 && S != 16 && S != 32 && S != 64)
         return set_error(GPURT_E_UNSUPPORTED, "stack_depth must be 0, 8, 16, 32 or 64");
     if (qopts && (qopts->waves_per_cu < 0 || qopts->waves_per_cu > 64))
