@@ -475,13 +475,26 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
 
 // Internal nodes bucketed by prefix length (a counting sort, order within a bucket
 // arbitrary): lvl_hist[256] counts, lvl_begin[257] prefix sums, lvl_order the nodes.
+// Each block counts a contiguous range of nodes in LDS and adds to the global counts
+// once per prefix length it saw: with one block per 256 nodes, the same-address device
+// atomics alone took over 1 ms per pass for a 25 M-node batch.  Within a block's range a
+// bucket keeps roughly node order, so the per-level passes read neighbouring nodes.
+constexpr unsigned kLevelBlocks = 1024;
+
+__device__ __forceinline__ void level_range(int n_int, uint32_t& lo, uint32_t& hi) {
+    const uint32_t per = (((uint32_t)n_int + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
+    lo = min((uint32_t)n_int, blockIdx.x * per);
+    hi = min((uint32_t)n_int, lo + per);
+}
+
 __global__ __launch_bounds__(kBlock) void k_level_hist(int n_int, const uint8_t* __restrict__ dlev,
                                                        uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n_int) atomicAdd(&h[dlev[i]], 1u);
+    uint32_t lo, hi;
+    level_range(n_int, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[dlev[i]], 1u);
     __syncthreads();
     if (h[threadIdx.x]) gatomic_add(&hist[threadIdx.x], h[threadIdx.x]);
 }
@@ -505,16 +518,18 @@ __global__ __launch_bounds__(kBlock) void k_level_scatter(int n_int, const uint8
     __shared__ uint32_t h[256], base[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t l = 0, rank = 0;
-    if (i < n_int) {
-        l = dlev[i];
-        rank = atomicAdd(&h[l], 1u);
+    uint32_t lo, hi;
+    level_range(n_int, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[dlev[i]], 1u);
+    __syncthreads();
+    // reserve this block's part of every bucket, then place its nodes there
+    base[threadIdx.x] = h[threadIdx.x] ? gatomic_add(&fill[threadIdx.x], h[threadIdx.x]) : 0u;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const uint32_t l = dlev[i];
+        order[base[l] + atomicAdd(&h[l], 1u)] = i;
     }
-    __syncthreads();
-    if (h[threadIdx.x]) base[threadIdx.x] = gatomic_add(&fill[threadIdx.x], h[threadIdx.x]);
-    __syncthreads();
-    if (i < n_int) order[base[l] + rank] = (uint32_t)i;
 }
 
 // Node boxes without atomics or fences: a child's common-prefix length (Karras' delta
@@ -609,9 +624,12 @@ __device__ uint32_t repack_leaf(const CollapseArgs& a, uint32_t r, uint32_t coun
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
+    // the graph of a batch launches every possible level: most are empty or small, so
+    // surplus blocks leave before anything else
+    const uint32_t cnt = a.lvl_cnt[a.level];
+    if (blockIdx.x * kBlock >= cnt) return;
     uint32_t begin = 0;
     for (int l = 0; l < a.level; l++) begin += a.lvl_cnt[l];
-    uint32_t cnt = a.lvl_cnt[a.level];
     WNode<W>* nodes = reinterpret_cast<WNode<W>*>(a.nodes);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
         uint32_t wi = begin + k;
@@ -888,9 +906,10 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
         while ((1u << nbits) < N) nbits++;
         const unsigned gI = cdiv(n_int, kBlock);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, b.lvl_hist, (size_t)256, 0u);
-        hipLaunchKernelGGL(k_level_hist, dim3(gI), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_hist);
+        const unsigned gL = std::min(gI, kLevelBlocks);
+        hipLaunchKernelGGL(k_level_hist, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_hist);
         hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, s, b.lvl_hist, b.lvl_begin, b.lvl_fill);
-        hipLaunchKernelGGL(k_level_scatter, dim3(gI), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_fill, b.lvl_order);
+        hipLaunchKernelGGL(k_level_scatter, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_fill, b.lvl_order);
         const unsigned lgrid = std::min(gI, 1024u);
         for (int level = 95; level >= 64 - key_bits; level--) {
             if (level >= 64 && level < 96 - nbits) continue;
