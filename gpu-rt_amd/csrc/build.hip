@@ -532,12 +532,93 @@ __global__ __launch_bounds__(kBlock) void k_level_scatter(int n_int, const uint8
     }
 }
 
-// Node boxes without atomics or fences: a child's common-prefix length (Karras' delta
-// of its range) is strictly greater than its parent's, so launching once per delta
-// value from the largest down makes every child box final before its parent reads it.
-// Each launch scans the delta byte of every internal node and merges the matching ones.
+// Node boxes without atomics or fences, in two parts.
+//
+// k_boxes_local: block c owns internal nodes [256c, 256c + 256).  A node whose leaf range
+// lies in one such chunk has its whole subtree there (a Karras node's index is an end of
+// its range, so the subtree's internal nodes lie inside the range), so the block computes
+// all of these bottom up by itself: rounds in which every node whose children are done
+// merges their boxes, with the chunk's boxes and costs kept in LDS (its own global writes
+// are not read back, so no cache can hand a lane a stale box) until no node is left.  A
+// round always finishes the deepest pending node, so the loop ends.  This covers ~90% of
+// the nodes in one launch.
+__device__ __forceinline__ bool box_in_chunk(uint2 r) { return (r.x >> 8) == (r.y >> 8); }
+static_assert(kBlock == 256, "k_boxes_local chunks are 256 nodes");
+
+__global__ __launch_bounds__(kBlock) void k_boxes_local(uint32_t n_int, const uint2* __restrict__ range,
+                                                        const uint32_t* __restrict__ left,
+                                                        const uint32_t* __restrict__ right,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const uint8_t* __restrict__ dlev,
+                                                        const float4* __restrict__ saabb,
+                                                        float4* __restrict__ nbox, float* __restrict__ cost,
+                                                        int leaf_size) {
+    __shared__ float4 blo[kBlock], bhi[kBlock];
+    __shared__ float bcost[kBlock];
+    __shared__ uint8_t done[kBlock];
+    const uint32_t c0 = blockIdx.x * kBlock;
+    const uint32_t i = c0 + threadIdx.x;
+    bool todo = false;
+    uint32_t L = 0, R = 0;
+    if (i < n_int && dlev[i] != 0xFF) {
+        todo = box_in_chunk(range[i]);
+        L = left[i];
+        R = right[i];
+    }
+    done[threadIdx.x] = 0;
+    __syncthreads();
+    while (__syncthreads_or(todo)) {
+        const bool ready = todo && ((L & kLeafBit) || done[L - c0]) && ((R & kLeafBit) || done[R - c0]);
+        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+        float c = 0.f;
+        if (ready) {
+            float4 al, ah, bl, bh;
+            float cl, cr;
+            if (L & kLeafBit) {
+                const size_t p = L & ~kLeafBit;
+                al = saabb[2 * p];
+                ah = saabb[2 * p + 1];
+                cl = kCt * box_area(al, ah);
+            } else {
+                al = blo[L - c0];
+                ah = bhi[L - c0];
+                cl = bcost[L - c0];
+            }
+            if (R & kLeafBit) {
+                const size_t p = R & ~kLeafBit;
+                bl = saabb[2 * p];
+                bh = saabb[2 * p + 1];
+                cr = kCt * box_area(bl, bh);
+            } else {
+                bl = blo[R - c0];
+                bh = bhi[R - c0];
+                cr = bcost[R - c0];
+            }
+            lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+            hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+            c = sah_node_cost(lo, hi, cnt[i], cl, cr, leaf_size);
+        }
+        __syncthreads();  // every read of this round before any write
+        if (ready) {
+            blo[threadIdx.x] = lo;
+            bhi[threadIdx.x] = hi;
+            bcost[threadIdx.x] = c;
+            done[threadIdx.x] = 1;
+            nbox[2 * (size_t)i] = lo;
+            nbox[2 * (size_t)i + 1] = hi;
+            cost[i] = c;
+            todo = false;
+        }
+    }
+}
+
+// k_boxes_level: the remaining nodes (ranges across chunks).  A child's common-prefix
+// length (Karras' delta of its range) is strictly greater than its parent's, so
+// launching once per delta value from the largest down makes every child box final
+// before its parent reads it.
 __global__ __launch_bounds__(kBlock) void k_boxes_level(int level, const uint32_t* __restrict__ lvl_begin,
                                                         const uint32_t* __restrict__ lvl_order,
+                                                        const uint2* __restrict__ range,
                                                         const uint32_t* __restrict__ left,
                                                         const uint32_t* __restrict__ right,
                                                         const uint32_t* __restrict__ cnt,
@@ -548,6 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_boxes_level(int level, const uint32_
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t k = beg + blockIdx.x * kBlock + threadIdx.x; k < end; k += stride) {
         const uint32_t i = lvl_order[k];
+        if (box_in_chunk(range[i])) continue;  // done by k_boxes_local
         float4 al, ah, bl, bh;
         const uint32_t L = left[i], R = right[i];
         ref_box(L, saabb, nbox, al, ah);
@@ -910,11 +992,13 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
         hipLaunchKernelGGL(k_level_hist, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_hist);
         hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, s, b.lvl_hist, b.lvl_begin, b.lvl_fill);
         hipLaunchKernelGGL(k_level_scatter, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_fill, b.lvl_order);
+        hipLaunchKernelGGL(k_boxes_local, dim3(gI), dim3(kBlock), 0, s, (uint32_t)n_int, b.range, b.left, b.right,
+                           b.cnt, b.dlev, b.saabb, b.nbox, b.cost, b.leaf_size);
         const unsigned lgrid = std::min(gI, 1024u);
         for (int level = 95; level >= 64 - key_bits; level--) {
             if (level >= 64 && level < 96 - nbits) continue;
             hipLaunchKernelGGL(k_boxes_level, dim3(lgrid), dim3(kBlock), 0, s, level, b.lvl_begin, b.lvl_order,
-                               b.left, b.right, b.cnt, b.saabb, b.nbox, b.cost, b.leaf_size);
+                               b.range, b.left, b.right, b.cnt, b.saabb, b.nbox, b.cost, b.leaf_size);
         }
         if (treelet_size > 0) {
             int trc = run_treelets(b, treelet_size, treelet_rounds, key_bits, nbits, s);
