@@ -679,10 +679,8 @@ __device__ __forceinline__ uint32_t ref_first(uint32_t r, const uint2* range) {
     return range[r].x;
 }
 
-// Copies the <= leaf_size triangles under binary ref r to fresh slots; returns the
-// first slot.
-__device__ uint32_t repack_leaf(const CollapseArgs& a, uint32_t r, uint32_t count) {
-    const uint32_t base = gatomic_add(&a.counters[2], count);
+// Copies the <= leaf_size triangles under binary ref r to the slots from `base`.
+__device__ void repack_leaf(const CollapseArgs& a, uint32_t r, uint32_t base) {
     uint32_t stack[8];
     int sp = 0;
     uint32_t out = base;
@@ -701,7 +699,25 @@ __device__ uint32_t repack_leaf(const CollapseArgs& a, uint32_t r, uint32_t coun
             stack[sp++] = a.left[x];
         }
     }
-    return base;
+}
+
+// Wave-aggregated allocation: each lane asks for n slots of *counter (any lane may ask
+// for 0; all 64 lanes take part), one device atomic per wave returns the wave's block
+// and the lane gets its part in lane order.  Many same-address device atomics, one per
+// node, serialise at the memory side.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, uint32_t n, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t x = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    total = __shfl(x, 63);
+    uint32_t base = 0;
+    if (lane == 63u && total) base = gatomic_add(counter, total);
+    base = __shfl(base, 63);
+    return base + x - n;
 }
 
 template <int W>
@@ -713,14 +729,20 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
     uint32_t begin = 0;
     for (int l = 0; l < a.level; l++) begin += a.lvl_cnt[l];
     WNode<W>* nodes = reinterpret_cast<WNode<W>*>(a.nodes);
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
-        uint32_t wi = begin + k;
+    // every lane runs every iteration (slot allocation is wave-wide); lanes past the
+    // level's end take part with nothing to allocate
+    for (uint32_t k0 = blockIdx.x * kBlock; k0 < cnt; k0 += gridDim.x * kBlock) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool act = k < cnt;
+        uint32_t wi = begin + (act ? k : 0u);
         uint32_t bref = a.wsrc[wi];
         uint32_t seg = a.wseg[wi];
         float pad = o2f(a.segstat[seg * kStat + 6]) * 0x1p-18f;
         uint32_t refs[W];
         int n = 0;
-        if (ref_count(bref, a.cnt) <= (uint32_t)a.leaf_size) {
+        if (!act) {
+            // nothing: n stays 0
+        } else if (ref_count(bref, a.cnt) <= (uint32_t)a.leaf_size) {
             refs[n++] = bref;
         } else {
             refs[n++] = a.left[bref];
@@ -745,7 +767,17 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
             }
         }
         WNode<W> out;
-        uint32_t n_internal = 0;
+        uint32_t n_internal = 0, n_repack = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            if (j < n) {
+                const uint32_t c = ref_count(refs[j], a.cnt);
+                if (c <= (uint32_t)a.leaf_size) n_repack += c;
+                else n_internal++;
+            }
+        }
+        uint32_t wave_total = 0;
+        uint32_t rp = a.repack ? wave_alloc(&a.counters[2], n_repack, wave_total) : 0u;
 #pragma unroll
         for (int j = 0; j < W; j++) {
             if (j < n) {
@@ -757,10 +789,15 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                 uint32_t c = ref_count(r, a.cnt);
                 out.aux[j] = c;
                 if (c <= (uint32_t)a.leaf_size) {
-                    const uint32_t first = a.repack ? repack_leaf(a, r, c) : ref_first(r, a.range);
+                    uint32_t first;
+                    if (a.repack) {
+                        repack_leaf(a, r, rp);
+                        first = rp;
+                        rp += c;
+                    } else {
+                        first = ref_first(r, a.range);
+                    }
                     out.child[j] = make_leaf(first, c);
-                } else {
-                    n_internal++;
                 }
             } else {
                 out.lox[j] = out.loy[j] = out.loz[j] = INFINITY;
@@ -769,9 +806,9 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                 out.aux[j] = 0;
             }
         }
+        const uint32_t base = wave_alloc(&a.counters[0], n_internal, wave_total);
+        if ((threadIdx.x & 63u) == 63u && wave_total) gatomic_add(&a.lvl_cnt[a.level + 1], wave_total);
         if (n_internal) {
-            uint32_t base = gatomic_add(&a.counters[0], n_internal);
-            gatomic_add(&a.lvl_cnt[a.level + 1], n_internal);
             uint32_t m = 0;
 #pragma unroll
             for (int j = 0; j < W; j++) {
@@ -788,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
                 }
             }
         }
-        nodes[wi] = out;
+        if (act) nodes[wi] = out;
     }
 }
 
