@@ -309,6 +309,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ a,
     }
 }
 
+// Scatter of one radix pass.  The block's tile (kSortTile pairs) is first ordered by
+// digit in LDS -- stably: rounds, waves and lanes in input order, a wave's lanes ranked
+// by an 8-ballot multisplit -- and then written out in that order, so each digit's run
+// of the tile (16 pairs on average) leaves as one contiguous stretch of the output
+// instead of one isolated 12-byte write per pair.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
                                                           uint64_t* __restrict__ kout,
@@ -316,41 +321,74 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
                                                           int shift,
                                                           const uint32_t* __restrict__ offs,
                                                           uint32_t nblocks) {
-    __shared__ uint32_t base[256];
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    __shared__ uint32_t gbase[256];   // this block's first output slot of digit d
+    __shared__ uint32_t lstart[256];  // the tile's first LDS slot of digit d
+    __shared__ uint32_t cur[256];     // pairs of digit d placed so far
     __shared__ uint32_t wcnt[4][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
+    const uint32_t tile = blockIdx.x * kSortTile;
+    const uint32_t m = min((uint32_t)kSortTile, n - tile);
+    gbase[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
+    cur[tid] = 0;
     for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t i = r * kBlock + tid;
+        key[r] = i < m ? kin[tile + i] : 0ull;
+        val[r] = i < m ? vin[tile + i] : 0u;
+        if (i < m) atomicAdd(&cur[(uint32_t)(key[r] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the tile's digit counts (Hillis-Steele over 256 entries)
+    uint32_t own = cur[tid], incl = own;
+    lstart[tid] = incl;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = tid >= off ? lstart[tid - off] : 0u;
+        __syncthreads();
+        incl += x;
+        lstart[tid] = incl;
+        __syncthreads();
+    }
+    lstart[tid] = incl - own;
+    cur[tid] = 0;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    uint32_t tile = blockIdx.x * kSortTile;
+#pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        uint32_t i = tile + r * kBlock + tid;
-        bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0ull;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const bool valid = r * kBlock + tid < (int)m;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            uint64_t bb = __ballot((d >> b) & 1u);
+            const uint64_t bb = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
-        uint32_t rank = __popcll(peers & lt_mask);
-        bool leader = valid && rank == 0;
-        if (leader) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        const uint32_t rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t off = base[d] + rank;
-            for (int w = 0; w < wave; w++) off += wcnt[w][d];
-            kout[off] = key;
-            vout[off] = val;
+            uint32_t pos = lstart[d] + cur[d] + rank;
+            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+            skey[pos] = key[r];
+            sval[pos] = val[r];
         }
         __syncthreads();
-        uint32_t tot = wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        base[tid] += tot;
+        cur[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
         for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
         __syncthreads();
+    }
+    for (uint32_t j = tid; j < m; j += kBlock) {
+        const uint64_t k = skey[j];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t o = gbase[d] + (j - lstart[d]);
+        kout[o] = k;
+        vout[o] = sval[j];
     }
 }
 
