@@ -1,7 +1,7 @@
 // treelet.hip -- SAH treelet restructuring of the binary LBVH (Karras & Aila, "Fast
 // Parallel Construction of High-Quality Bounding Volume Hierarchies", HPG 2013).
 //
-// For every internal node r with more than leaf_size triangles, bottom-up:
+// For every internal node r with more than 2 leaf_size triangles, bottom-up:
 //   1. form a treelet: start from r's two children and repeatedly open the treelet leaf
 //      (an internal node) with the largest surface area, up to N leaves (N = 5..7);
 //   2. find the binary topology over those leaves with the least SAH cost by dynamic
@@ -44,7 +44,12 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
     const uint32_t beg = lvl_begin[level], end = lvl_begin[level + 1];
     for (uint32_t k = beg + blockIdx.x; k < end; k += gridDim.x) {
         const int r = (int)lvl_order[k];
-        if (cnt[r] <= (uint32_t)leaf_size) continue;  // becomes one leaf: nothing to do
+        // a subtree of <= leaf_size triangles becomes one leaf: nothing to do; one of <=
+        // 2 leaf_size triangles becomes at most two leaves of one wide node, where a new
+        // topology gains nothing measurable (rays_scene: skipping them builds in 10.7
+        // instead of 13.4 ms with the same ray times; skipping up to 16 or 32 triangles
+        // builds faster still but costs 2-5% in ray time)
+        if (cnt[r] <= 2u * (uint32_t)leaf_size) continue;
         {
             if (lane == 0) {
                 // refresh r's cost from its (finished) children, then form the treelet
