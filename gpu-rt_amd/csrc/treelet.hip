@@ -23,6 +23,22 @@
 
 namespace gpurt_impl {
 
+// binomial coefficients C(n, k), n, k <= 7: subsets of each size among a treelet's leaves
+__constant__ uint8_t kBinom[8][8] = {
+    {1, 0, 0, 0, 0, 0, 0, 0},  {1, 1, 0, 0, 0, 0, 0, 0},   {1, 2, 1, 0, 0, 0, 0, 0},   {1, 3, 3, 1, 0, 0, 0, 0},
+    {1, 4, 6, 4, 1, 0, 0, 0},  {1, 5, 10, 10, 5, 1, 0, 0}, {1, 6, 15, 20, 15, 6, 1, 0}, {1, 7, 21, 35, 35, 21, 7, 1}};
+
+// the s-th submask of mask in increasing order (bit i of s selects the i-th set bit)
+__device__ __forceinline__ int nth_submask(int s, int mask) {
+    int out = 0;
+    for (int bit = 0; mask; bit++) {
+        const int lo = mask & -mask;
+        if (s & (1 << bit)) out |= lo;
+        mask ^= lo;
+    }
+    return out;
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __restrict__ lvl_begin,
                                                 const uint32_t* __restrict__ lvl_order, int leaf_size,
@@ -37,6 +53,8 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
     __shared__ uint32_t s_cnt[NS];
     __shared__ float s_copt[NS];
     __shared__ uint8_t s_part[NS];
+    __shared__ uint8_t s_list[NS];  // subsets ordered by size: size k at [s_kbase[k], + C(nl, k))
+    __shared__ int s_kbase[N + 2], s_kfill[N + 2];
     __shared__ int s_nl;
     __shared__ float s_cur;
     const int lane = threadIdx.x;
@@ -112,24 +130,75 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
                     s_part[sub] = 0;
                 }
             }
+            if (lane <= nl) {
+                int b = 0;
+                for (int j = 1; j < lane; j++) b += kBinom[nl][j];
+                s_kbase[lane] = b;
+                s_kfill[lane] = 0;
+            }
             __syncthreads();
+            for (int sub = lane + 1; sub <= full; sub += 64) {
+                const int k = __popc(sub);
+                s_list[s_kbase[k] + atomicAdd(&s_kfill[k], 1)] = (uint8_t)sub;
+            }
+            __syncthreads();
+            // Optimal costs by subset size.  A subset's splits keep its lowest leaf on one
+            // side (P holds it, P != sub).  Up to 4 leaves a lane takes a subset and tries
+            // its <= 7 splits; from 5 leaves a group of 2^(k-1) lanes takes one subset,
+            // a lane per split, and reduces.  Ties go to the larger P in both forms (the
+            // first found in a descending walk of the submasks), so the tree does not
+            // depend on which form ran.
             for (int k = 2; k <= nl; k++) {
-                for (int sub = lane + 1; sub <= full; sub += 64) {
-                    if (__popc(sub) != k) continue;
-                    const int low = sub & -sub;
-                    float best = INFINITY;
-                    int bp = low;
-                    for (int P = (sub - 1) & sub; P; P = (P - 1) & sub) {
-                        if (!(P & low)) continue;
-                        const float c = s_copt[P] + s_copt[sub ^ P];
-                        if (c < best) {
-                            best = c;
-                            bp = P;
+                const int nk = kBinom[nl][k];
+                const uint8_t* list = s_list + s_kbase[k];
+                if (k <= 4) {
+                    for (int idx = lane; idx < nk; idx += 64) {
+                        const int sub = list[idx];
+                        const int low = sub & -sub;
+                        float best = INFINITY;
+                        int bp = low;
+                        for (int P = (sub - 1) & sub; P; P = (P - 1) & sub) {
+                            if (!(P & low)) continue;
+                            const float c = s_copt[P] + s_copt[sub ^ P];
+                            if (c < best) {
+                                best = c;
+                                bp = P;
+                            }
+                        }
+                        const float a = box_area(s_lo[sub], s_hi[sub]);
+                        s_copt[sub] =
+                            (s_cnt[sub] <= (uint32_t)leaf_size) ? kCt * a * (float)s_cnt[sub] : kCi * a + best;
+                        s_part[sub] = (uint8_t)bp;
+                    }
+                } else {
+                    const int G = 1 << (k - 1);  // 16, 32 or 64 lanes per subset
+                    const int groups = 64 / G, g = lane / G, sl = lane % G;
+                    for (int base = 0; base < nk; base += groups) {
+                        const int idx = base + g;
+                        const bool ok = idx < nk;
+                        const int sub = ok ? list[idx] : 0;
+                        float c = INFINITY;
+                        int P = -1;
+                        if (ok && sl < G - 1) {
+                            const int low = sub & -sub;
+                            P = low | nth_submask(sl, sub ^ low);
+                            c = s_copt[P] + s_copt[sub ^ P];
+                        }
+                        for (int o = G / 2; o >= 1; o >>= 1) {
+                            const float oc = __shfl_xor(c, o);
+                            const int op = __shfl_xor(P, o);
+                            if (oc < c || (oc == c && op > P)) {
+                                c = oc;
+                                P = op;
+                            }
+                        }
+                        if (ok && sl == 0) {
+                            const float a = box_area(s_lo[sub], s_hi[sub]);
+                            s_copt[sub] =
+                                (s_cnt[sub] <= (uint32_t)leaf_size) ? kCt * a * (float)s_cnt[sub] : kCi * a + c;
+                            s_part[sub] = (uint8_t)P;
                         }
                     }
-                    const float a = box_area(s_lo[sub], s_hi[sub]);
-                    s_copt[sub] = (s_cnt[sub] <= (uint32_t)leaf_size) ? kCt * a * (float)s_cnt[sub] : kCi * a + best;
-                    s_part[sub] = (uint8_t)bp;
                 }
                 __syncthreads();
             }

@@ -227,3 +227,26 @@ def test_vertex_sum_modes_are_checked(dev):
     from gpurt import _lib
     assert _lib.load().gpurt_ctx_set_vertex_sums(ctx.handle, 7) == _lib.GPURT_E_INVALID
     ctx.close()
+
+
+def test_auto_vertex_sums_pick_the_fixed_order_for_large_calls(dev):
+    """From 2^21 terms (here 1,048,576 queries = 3,145,728 terms) the default mode uses
+    the fixed-order sums: its gradients equal the fixed mode's bit for bit, and match
+    the atomic mode's within rounding."""
+    V, F = _sphere(6, 11)
+    Q = W.surface_samples(V.astype(np.float64), F, np.random.default_rng(12), 1 << 20, 0.01).astype(np.float32)
+    out = {}
+    for mode in ("auto", "fixed", "atomic"):
+        ctx = gpurt.Context(0, vertex_sums=mode)
+        bvh = gpurt.Bvh(ctx, _t(V, dev), _t(F, dev))
+        tq = _t(Q, dev)
+        prim, pt, dist, bary = bvh.closest_point(tq)
+        gv = torch.zeros((V.shape[0], 3), device=dev)
+        bvh.closest_point_bwd(tq, prim, bary, pt, dist, torch.ones(len(Q), device=dev), None, gv)
+        torch.cuda.synchronize()
+        out[mode] = gv.cpu().numpy()
+        bvh.close()
+        ctx.close()
+    assert np.array_equal(out["auto"], out["fixed"])
+    scale = np.abs(out["fixed"]).max()
+    np.testing.assert_allclose(out["atomic"], out["fixed"], atol=1e-5 * scale, rtol=1e-4)
