@@ -375,17 +375,10 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
         ray_children<W>(in_inst ? bnodes : top_nodes, cur, r.inv, r.oix, r.oiy, r.oiz, tmin, best, slack, tn, c);
         if (!ANY) sort_children<W>(tn, c);
         have = false;
-        uint32_t pend = kEmpty;
-        float pend_t = 0.0f;
+        bool hit[W];
 #pragma unroll
-        for (int j = W - 1; j >= 0; j--) {
-            if (c[j] == kEmpty) continue;
-            if (pend != kEmpty) {
-                if (!st.push(pend, pend_t)) ok = false;
-            }
-            pend = c[j];
-            pend_t = tn[j];
-        }
+        for (int j = 0; j < W; j++) hit[j] = c[j] != kEmpty;
+        const uint32_t pend = push_children<!ANY>(st, c, tn, hit, ok);
         if (pend != kEmpty) {
             cur = pend;
             have = true;

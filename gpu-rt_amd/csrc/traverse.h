@@ -90,13 +90,63 @@ struct LaneStack {
     }
 };
 
+// Pushes the children of a node step that the lane visits later and returns the one it
+// continues with (kEmpty when no child is hit).  hit[j] marks the children to visit.
+// SORTED: the children are ordered near to far, the first hit one is next and the others
+// are pushed far to near; otherwise the first hit child in slot order is next and the
+// others are pushed in descending slot order.  When every push fits below the last LDS
+// slot, each child is written to the next free slot unconditionally and the slot
+// advances only past a pushed child, so a skipped child's entry is overwritten by the
+// next write or lies above the new stack top: a select per child instead of nested
+// branches per push (rays_scene closest hit 212-223 -> 198-205 us).
+template <bool SORTED, int W, int S, int BS>
+__device__ __forceinline__ uint32_t push_children(LaneStack<S, BS>& st, const uint32_t (&c)[W], const float (&key)[W],
+                                                  const bool (&hit)[W], bool& ok) {
+    uint32_t next = kEmpty;
+    if constexpr (SORTED) {
+        next = hit[0] ? c[0] : kEmpty;
+    } else {
+#pragma unroll
+        for (int j = W - 1; j >= 0; j--)
+            if (hit[j]) next = c[j];
+    }
+    constexpr int J0 = SORTED ? 1 : 0;  // slots that may be pushed: [J0, W)
+    bool push[W];
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = J0; j < W; j++) {
+        push[j] = hit[j] && (SORTED || c[j] != next);  // child words of hit children differ
+        n += push[j] ? 1u : 0u;
+    }
+    if ((uint32_t)st.sp + n < (uint32_t)S) {
+        // the writes reach slot sp + n at most, inside the lane's LDS column
+        uint32_t slot = (uint32_t)st.sp;
+#pragma unroll
+        for (int j = W - 1; j >= J0; j--) {
+            lds_stack[st.lds_index((int)slot)] = make_uint2(c[j], __float_as_uint(key[j]));
+            slot += push[j] ? 1u : 0u;
+        }
+        st.sp = (int)slot;
+    } else {
+#pragma unroll
+        for (int j = W - 1; j >= J0; j--)
+            if (push[j] && !st.push(c[j], key[j])) ok = false;
+    }
+    return next;
+}
+
 template <int W>
 __device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&c)[W]) {
+    // keys by min / max, children by selects: no branch (a branch per comparator cost
+    // the exchange's moves under divergent exec masks)
     auto cx = [&](int i, int j) {
-        if (d[j] < d[i]) {
-            float tf = d[i]; d[i] = d[j]; d[j] = tf;
-            uint32_t tc = c[i]; c[i] = c[j]; c[j] = tc;
-        }
+        const bool s = d[j] < d[i];
+        const float lo = fminf(d[i], d[j]), hi = fmaxf(d[i], d[j]);
+        const uint32_t ci = s ? c[j] : c[i], cj = s ? c[i] : c[j];
+        d[i] = lo;
+        d[j] = hi;
+        c[i] = ci;
+        c[j] = cj;
     };
     if constexpr (W == 2) {
         cx(0, 1);
@@ -204,7 +254,7 @@ __device__ __forceinline__ void ray_children(const WNode<W>* __restrict__ nodes,
         const float t0z = fmaf(nd.loz[j], inv.z, oiz), t1z = fmaf(nd.hiz[j], inv.z, oiz);
         const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-        const bool hb = (nd.child[j] != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+        const bool hb = (nd.child[j] != kEmpty) & (lo <= fmaf(fabsf(hi), slack, hi));
         tn[j] = hb ? lo : INFINITY;
         c[j] = hb ? nd.child[j] : kEmpty;
     }
@@ -269,7 +319,7 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
                                    fmaxf(fmaf(bt(nw[2][j / 4], j), az, bz), tmin));
             const float hi = fminf(fminf(fmaf(bt(fw[0][j / 4], j), ax, bx), fmaf(bt(fw[1][j / 4], j), ay, by)),
                                    fminf(fmaf(bt(fw[2][j / 4], j), az, bz), best));
-            const bool hb = (ch != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+            const bool hb = (ch != kEmpty) & (lo <= fmaf(fabsf(hi), slack, hi));
             tn[j] = hb ? lo : INFINITY;
             c[j] = hb ? ch : kEmpty;
         }
@@ -283,7 +333,7 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
         const float t0z = fmaf(qb(2, j), az, bz), t1z = fmaf(qb(5, j), az, bz);
         const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
-        const bool hb = (ch != kEmpty) & (lo <= hi + fabsf(hi) * slack);
+        const bool hb = (ch != kEmpty) & (lo <= fmaf(fabsf(hi), slack, hi));
         tn[j] = hb ? lo : INFINITY;
         c[j] = hb ? ch : kEmpty;
     }
@@ -293,12 +343,18 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
 // closest point
 // ------------------------------------------------------------------------------------
 
-// Resident waves per SIMD the register allocator targets: 8 (<= 64 VGPRs) for 2- and
-// 4-wide nodes; 8-wide nodes need more live registers for the child boxes and sort
-// keys (forcing 64 VGPRs makes them spill), so they get what they need (5-7 waves).
+// Resident waves per SIMD the register allocator targets: 7 (<= 72 VGPRs) for 2- and
+// 4-wide nodes.  At 8 (64 VGPRs) the 4-wide closest-hit kernel spilled 5 VGPRs to
+// scratch and 20 SGPRs to VGPR lanes, reloading them with v_readlane in the loop; at 7
+// it spills nothing (70 VGPRs), and every 4-wide kernel got faster with identical
+// answers: rays_scene closest hit 219 -> 213 us, diff closest point 875 -> 861 us,
+// cpq_large 13.7 -> 13.4 ms, batched replay 21.55 -> 21.33 ms.  Two such kernels at 16
+// waves per CU each (the two-stream ray step) still run side by side as fast as before.
+// 8-wide nodes need more live registers for the child boxes and sort keys, so they get
+// what they need (5-7 waves).
 template <int W, bool Q = false>
 struct WavesHint {
-    static constexpr int value = W == 8 ? 4 : 8;
+    static constexpr int value = W == 8 ? 4 : 7;
 };
 
 constexpr float kCullSlack = 1.0f + 0x1p-16f;  // relative slack on squared distances
@@ -772,19 +828,12 @@ __device__ __forceinline__ bool ray_stream(const NodeT* __restrict__ nodes, cons
             ray_children<W>(nodes, cur, inv, oix, oiy, oiz, tmin, best, kTSlack, tn, c);
             if (!ANY) sort_children<W>(tn, c);
             have = false;
-            // push all but the nearest hit child (far to near), then continue with it;
-            // the pending entry is carried in scalars so every index stays static
-            uint32_t pend = kEmpty;
-            float pend_t = 0.0f;
+            // closest hit: continue with the nearest hit child and push the others far
+            // to near; any hit: continue with the first in slot order
+            bool hit[W];
 #pragma unroll
-            for (int j = W - 1; j >= 0; j--) {
-                if (c[j] == kEmpty) continue;
-                if (pend != kEmpty) {
-                    if (!st.push(pend, pend_t)) ok = false;
-                }
-                pend = c[j];
-                pend_t = tn[j];
-            }
+            for (int j = 0; j < W; j++) hit[j] = c[j] != kEmpty;
+            const uint32_t pend = push_children<!ANY>(st, c, tn, hit, ok);
             if (pend != kEmpty) {
                 cur = pend;
                 have = true;
@@ -970,14 +1019,12 @@ __device__ __forceinline__ bool cpq_stream(const NodeT* __restrict__ nodes, cons
         }
         sort_children<W>(dd, c);
         have = false;
+        bool hit[W];
 #pragma unroll
-        for (int j = W - 1; j >= 1; j--) {
-            if (c[j] != kEmpty && dd[j] <= thr) {
-                if (!st.push(c[j], dd[j])) ok = false;
-            }
-        }
-        if (c[0] != kEmpty && dd[0] <= thr) {
-            cur = c[0];
+        for (int j = 0; j < W; j++) hit[j] = c[j] != kEmpty && dd[j] <= thr;
+        const uint32_t next = push_children<true>(st, c, dd, hit, ok);
+        if (next != kEmpty) {
+            cur = next;
             have = true;
         }
     }
