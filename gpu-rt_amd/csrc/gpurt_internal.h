@@ -61,16 +61,24 @@ static_assert(sizeof(WNode<8>) == 256, "node size");
 
 inline size_t node_bytes(int W) { return (size_t)32 * (size_t)W; }
 
-// Quantized wide node.  Child j's box on axis a decodes as
+// Quantized wide node, as 32-bit words.  Child j's box on axis a decodes as
 //   lo = fmaf((float)qlo[a][j], s_a, origin_a),  hi = fmaf((float)qhi[a][j], s_a, origin_a)
-// with s_a = 2^(e_a - 127) (e_a = byte a of `exps`, 1..254).
+// with s_a a power of two.  2- and 4-wide nodes store the three scales as floats (words
+// 3..5: the traversal multiplies them in directly); 8-wide nodes, whose 96 bytes have no
+// room for them, store the biased exponents e_a (s_a = 2^(e_a - 127), byte a of word 3).
+//   words 0..2         origin x, y, z (float)
+//   word 3 (.. 5)      scales (float) or exponent bytes
+//   kChild0 ..         child[W]
+//   kQ0 ..             q[6][W] bytes: qlo x, y, z, then qhi x, y, z; zero padding
+// An empty child slot has qlo = 255 and qhi = 0 on every axis (an inverted box).
 template <int W>
 struct alignas(16) QNode {
     static constexpr int kWidth = W;
-    float ox, oy, oz;
-    uint32_t exps;
-    uint32_t child[W];
-    uint8_t q[6][W];  // qlo x, y, z, then qhi x, y, z
+    static constexpr bool kFloatScale = W <= 4;
+    static constexpr int kChild0 = kFloatScale ? 6 : 4;
+    static constexpr int kQ0 = kChild0 + W;
+    static constexpr int kWords = (kQ0 + (6 * W + 3) / 4 + 3) / 4 * 4;
+    uint32_t w[kWords];
 };
 static_assert(sizeof(QNode<2>) == 48, "node size");
 static_assert(sizeof(QNode<4>) == 64, "node size");

@@ -7,7 +7,8 @@
 // FP32 expression the traversal decodes them with (fmaf(q, s, origin)), so every
 // decoded box contains the FP32 box it came from.  FP32 boxes are already inflated for
 // rounding, so the culls of the traversal stay conservative and query results do not
-// change -- only the node's size does (64 B instead of 128 B at W = 4).
+// change -- only the node's size does (64 B instead of 128 B at W = 4).  An empty child
+// slot is stored as the inverted box lo = 255, hi = 0.
 #include <algorithm>
 
 #include "gpurt_internal.h"
@@ -37,10 +38,13 @@ __global__ __launch_bounds__(kBlock) void k_quantize(const WNode<W>* __restrict_
         org[0] = org[1] = org[2] = 0.0f;
         top[0] = top[1] = top[2] = 0.0f;
     }
-    QNode<W> q;
-    q.ox = org[0];
-    q.oy = org[1];
-    q.oz = org[2];
+    using QN = QNode<W>;
+    QN q;
+#pragma unroll
+    for (int k = 0; k < QN::kWords; k++) q.w[k] = 0u;
+    q.w[0] = __float_as_uint(org[0]);
+    q.w[1] = __float_as_uint(org[1]);
+    q.w[2] = __float_as_uint(org[2]);
     uint32_t exps = 0;
     float sc[3];
 #pragma unroll
@@ -53,15 +57,24 @@ __global__ __launch_bounds__(kBlock) void k_quantize(const WNode<W>* __restrict_
         sc[a] = pow2_biased((uint32_t)e);
         exps |= (uint32_t)e << (8 * a);
     }
-    q.exps = exps;
+    if constexpr (QN::kFloatScale) {
+        q.w[3] = __float_as_uint(sc[0]);
+        q.w[4] = __float_as_uint(sc[1]);
+        q.w[5] = __float_as_uint(sc[2]);
+    } else {
+        q.w[3] = exps;
+    }
+    uint8_t* const qb = reinterpret_cast<uint8_t*>(q.w + QN::kQ0);  // q[6][W]
 #pragma unroll
     for (int j = 0; j < W; j++) {
-        q.child[j] = nd.child[j];
+        q.w[QN::kChild0 + j] = nd.child[j];
         const float lo[3] = {nd.lox[j], nd.loy[j], nd.loz[j]};
         const float hi[3] = {nd.hix[j], nd.hiy[j], nd.hiz[j]};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            uint32_t ql = 0, qh = 0;
+            // an empty slot gets the inverted box [255, 0] on every axis, which the
+            // traversal's box test rejects without looking at the child word
+            uint32_t ql = 255, qh = 0;
             if (nd.child[j] != kEmpty) {
                 // estimates, then exact checks with the decoder's expression; q = 0
                 // decodes to origin (<= lo) and q = 255 to >= top (>= hi), so the
@@ -77,8 +90,8 @@ __global__ __launch_bounds__(kBlock) void k_quantize(const WNode<W>* __restrict_
                 for (int it = 0; it < 4 && qh < 255 && fmaf((float)qh, sc[a], org[a]) < hi[a]; it++) qh++;
                 if (fmaf((float)qh, sc[a], org[a]) < hi[a]) qh = 255;
             }
-            q.q[a][j] = (uint8_t)ql;
-            q.q[3 + a][j] = (uint8_t)qh;
+            qb[a * W + j] = (uint8_t)ql;
+            qb[(3 + a) * W + j] = (uint8_t)qh;
         }
     }
     out[i] = q;

@@ -200,10 +200,21 @@ __device__ __forceinline__ void fetch_node(const WNode<W>* __restrict__ nodes, u
 
 // Quantized node: sizeof(QNode<W>) / 16 dwordx4 loads, then every box plane decodes as
 // fmaf(q, 2^(e-127), origin) -- the expression the builder checked it with.
+// the per-axis plane scales of a quantized node (its words w)
+template <int W>
+__device__ __forceinline__ V3 qnode_scales(const uint32_t* w) {
+    if constexpr (QNode<W>::kFloatScale)
+        return v3(__uint_as_float(w[3]), __uint_as_float(w[4]), __uint_as_float(w[5]));
+    else
+        return v3(__uint_as_float((w[3] & 0xFFu) << 23), __uint_as_float(((w[3] >> 8) & 0xFFu) << 23),
+                  __uint_as_float(((w[3] >> 16) & 0xFFu) << 23));
+}
+
 template <int W>
 __device__ __forceinline__ void fetch_node(const QNode<W>* __restrict__ nodes, uint32_t i, ChildBoxes<W>& b) {
-    constexpr int NW = (int)sizeof(QNode<W>) / 4;
-    constexpr int Q0 = 4 + W;  // first word of the 8-bit planes
+    using QN = QNode<W>;
+    constexpr int NW = QN::kWords;
+    constexpr int Q0 = QN::kQ0;  // first word of the 8-bit planes
     const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
     uint32_t w[NW];
 #pragma unroll
@@ -215,16 +226,15 @@ __device__ __forceinline__ void fetch_node(const QNode<W>* __restrict__ nodes, u
         w[4 * k + 3] = v.w;
     }
     const float ox = __uint_as_float(w[0]), oy = __uint_as_float(w[1]), oz = __uint_as_float(w[2]);
-    const float sx = __uint_as_float((w[3] & 0xFFu) << 23);
-    const float sy = __uint_as_float(((w[3] >> 8) & 0xFFu) << 23);
-    const float sz = __uint_as_float(((w[3] >> 16) & 0xFFu) << 23);
+    const V3 sc = qnode_scales<W>(w);
+    const float sx = sc.x, sy = sc.y, sz = sc.z;
     auto qb = [&](int row, int j) -> float {
         const int byte = row * W + j;
         return (float)((w[Q0 + byte / 4] >> (8 * (byte % 4))) & 0xFFu);
     };
 #pragma unroll
     for (int j = 0; j < W; j++) {
-        b.child[j] = w[4 + j];
+        b.child[j] = w[QN::kChild0 + j];
         b.lox[j] = fmaf(qb(0, j), sx, ox);
         b.loy[j] = fmaf(qb(1, j), sy, oy);
         b.loz[j] = fmaf(qb(2, j), sz, oz);
@@ -270,8 +280,9 @@ template <int W>
 __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes, uint32_t i, V3 inv, float oix,
                                              float oiy, float oiz, float tmin, float best, float slack, float* tn,
                                              uint32_t* c) {
-    constexpr int NW = (int)sizeof(QNode<W>) / 4;
-    constexpr int Q0 = 4 + W;  // first word of the 8-bit planes
+    using QN = QNode<W>;
+    constexpr int NW = QN::kWords;
+    constexpr int Q0 = QN::kQ0;  // first word of the 8-bit planes
     const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
     uint32_t w[NW];
 #pragma unroll
@@ -282,9 +293,10 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
         w[4 * k + 2] = v.z;
         w[4 * k + 3] = v.w;
     }
-    const float ax = __uint_as_float((w[3] & 0xFFu) << 23) * inv.x;
-    const float ay = __uint_as_float(((w[3] >> 8) & 0xFFu) << 23) * inv.y;
-    const float az = __uint_as_float(((w[3] >> 16) & 0xFFu) << 23) * inv.z;
+    const V3 sc = qnode_scales<W>(w);
+    const float ax = sc.x * inv.x;
+    const float ay = sc.y * inv.y;
+    const float az = sc.z * inv.z;
     const float bx = fmaf(__uint_as_float(w[0]), inv.x, oix);
     const float by = fmaf(__uint_as_float(w[1]), inv.y, oiy);
     const float bz = fmaf(__uint_as_float(w[2]), inv.z, oiz);
@@ -298,7 +310,11 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
         // is non-decreasing for a >= 0 and non-increasing for a <= 0 (a has inv's sign),
         // so the row picked is exactly the min / max of the two plane distances.  With
         // W % 4 == 0 each row is whole words, so the choice is a word select per row word
-        // instead of a min and a max per child and axis.
+        // instead of a min and a max per child and axis.  An empty slot holds the
+        // inverted box lo = 255, hi = 0 on every axis: for either sign its near plane
+        // lies 255 steps beyond its far one, so the test rejects it without a compare of
+        // the child word; a degenerate ray that still passes it (a scale step below 2^-20
+        // of the plane distances) gets its child word, kEmpty, and never visits it.
         constexpr int RW = W / 4;  // words per byte row
         const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
         uint32_t nw[3][RW], fw[3][RW];
@@ -314,25 +330,27 @@ __device__ __forceinline__ void ray_children(const QNode<W>* __restrict__ nodes,
         auto bt = [](uint32_t v, int j) -> float { return (float)((v >> (8 * (j % 4))) & 0xFFu); };
 #pragma unroll
         for (int j = 0; j < W; j++) {
-            const uint32_t ch = w[4 + j];
+            const uint32_t ch = w[QN::kChild0 + j];
             const float lo = fmaxf(fmaxf(fmaf(bt(nw[0][j / 4], j), ax, bx), fmaf(bt(nw[1][j / 4], j), ay, by)),
                                    fmaxf(fmaf(bt(nw[2][j / 4], j), az, bz), tmin));
             const float hi = fminf(fminf(fmaf(bt(fw[0][j / 4], j), ax, bx), fmaf(bt(fw[1][j / 4], j), ay, by)),
                                    fminf(fmaf(bt(fw[2][j / 4], j), az, bz), best));
-            const bool hb = (ch != kEmpty) & (lo <= fmaf(fabsf(hi), slack, hi));
+            const bool hb = lo <= fmaf(fabsf(hi), slack, hi);
             tn[j] = hb ? lo : INFINITY;
-            c[j] = hb ? ch : kEmpty;
+            c[j] = hb ? ch : kEmpty;  // an empty slot stays kEmpty either way
         }
         return;
     }
 #pragma unroll
     for (int j = 0; j < W; j++) {
-        const uint32_t ch = w[4 + j];
+        const uint32_t ch = w[QN::kChild0 + j];
         const float t0x = fmaf(qb(0, j), ax, bx), t1x = fmaf(qb(3, j), ax, bx);
         const float t0y = fmaf(qb(1, j), ay, by), t1y = fmaf(qb(4, j), ay, by);
         const float t0z = fmaf(qb(2, j), az, bz), t1z = fmaf(qb(5, j), az, bz);
         const float lo = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         const float hi = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best));
+        // (the per-axis min / max turns the inverted box of an empty slot into a proper
+        // one, so this form tests the child word)
         const bool hb = (ch != kEmpty) & (lo <= fmaf(fabsf(hi), slack, hi));
         tn[j] = hb ? lo : INFINITY;
         c[j] = hb ? ch : kEmpty;

@@ -199,9 +199,10 @@ typedef struct gpurt_bvh_arrays {
  *                            uint32 child[W]; uint32 count[W] (triangles under the child)
  *   child word: internal node index (< 2^31); leaf: 0x80000000 | (count-1) << 28 | first
  *               packed triangle (count 1..8); empty slot: 0xFFFFFFFF
- *   8-bit node: float origin[3]; uint32 exponents (byte a = e_a, scale 2^(e_a - 127));
- *               uint32 child[W]; uint8 q[6][W] (lo x, y, z, hi x, y, z), zero-padded; child
- *               j's box is fmaf(q, scale, origin) per plane
+ *   8-bit node: float origin[3]; W = 2, 4: float scale[3]; W = 8: uint32 exponents (byte
+ *               a = e_a, scale 2^(e_a - 127)); uint32 child[W]; uint8 q[6][W] (lo x, y, z,
+ *               hi x, y, z), zero-padded; child j's box is fmaf(q, scale, origin) per
+ *               plane (scales are powers of two); an empty slot has lo = 255, hi = 0
  *   packed triangle: float4 (a.xyz, id bits), float4 (b.xyz, 0), float4 (c.xyz, 0) -- a, b,
  *               c the mesh's vertices in index order, id the triangle index
  * Child boxes are inflated by 2^-18 of the largest coordinate magnitude, and every 8-bit

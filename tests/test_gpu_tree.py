@@ -119,11 +119,23 @@ def _check_q8(ex, W, lo, hi, child):
     q = ex["qnodes"]
     assert q.shape[1] == QBYTES[W]
     origin = q[:, 0:12].copy().view(np.float32)                      # (n, 3)
-    exps = q[:, 12:15].astype(np.int32)                               # (n, 3)
-    qchild = q[:, 16:16 + 4 * W].copy().view(np.uint32)
+    if W <= 4:  # float scales, then the child words
+        scale = q[:, 12:24].copy().view(np.float32)                   # (n, 3)
+        c0 = 24
+    else:       # biased exponent bytes
+        exps = q[:, 12:15].astype(np.int32)                           # (n, 3)
+        scale = np.ldexp(np.float32(1), exps - 127).astype(np.float32)
+        c0 = 16
+    # every scale is a power of two
+    m, _ = np.frexp(scale.astype(np.float64))
+    assert (m == 0.5).all()
+    qchild = q[:, c0:c0 + 4 * W].copy().view(np.uint32)
     assert np.array_equal(qchild, child)
-    planes = q[:, 16 + 4 * W:16 + 10 * W].reshape(-1, 6, W).astype(np.float32)
-    scale = np.ldexp(np.float32(1), exps - 127).astype(np.float32)   # (n, 3)
+    planes = q[:, c0 + 4 * W:c0 + 10 * W].reshape(-1, 6, W).astype(np.float32)
+    # empty slots hold the inverted box lo = 255, hi = 0
+    em = child == EMPTY
+    assert (planes[:, 0:3].transpose(0, 2, 1)[em] == 255).all()
+    assert (planes[:, 3:6].transpose(0, 2, 1)[em] == 0).all()
     # fmaf(q, scale, origin): q * scale is exact in FP32, so this is one rounding as fmaf
     dec_lo = (planes[:, 0:3].transpose(0, 2, 1) * scale[:, None, :]).astype(np.float32) + origin[:, None, :]
     dec_hi = (planes[:, 3:6].transpose(0, 2, 1) * scale[:, None, :]).astype(np.float32) + origin[:, None, :]
