@@ -511,13 +511,21 @@ __global__ __launch_bounds__(kBlock) void k_karras(int N, const uint64_t* __rest
 // internal node boxes and SAH costs
 // ------------------------------------------------------------------------------------
 
-// Internal nodes bucketed by prefix length (a counting sort, order within a bucket
-// arbitrary): lvl_hist[256] counts, lvl_begin[257] prefix sums, lvl_order the nodes.
-// Each block counts a contiguous range of nodes in LDS and adds to the global counts
-// once per prefix length it saw: with one block per 256 nodes, the same-address device
+// Internal nodes bucketed by prefix length l (a counting sort, order within a bucket
+// arbitrary), each length split in two: bucket 2l holds the nodes with more than
+// 2 leaf_size triangles, which are the treelet roots, and 2l + 1 the others, so the box
+// pass reads buckets 2l and 2l + 1 and the treelet pass only 2l (it used to load and
+// skip the small roots: two dependent loads per node, which made up most of a launch on
+// the deep levels).  lvl_hist[512] counts, lvl_begin[513] prefix sums, lvl_order the
+// nodes.  Each block counts a contiguous range of nodes in LDS and adds to the global
+// counts once per bucket it saw: with one block per 256 nodes, the same-address device
 // atomics alone took over 1 ms per pass for a 25 M-node batch.  Within a block's range a
 // bucket keeps roughly node order, so the per-level passes read neighbouring nodes.
 constexpr unsigned kLevelBlocks = 1024;
+
+__device__ __forceinline__ uint32_t level_bucket(uint8_t dlev, uint32_t cnt, uint32_t small_max) {
+    return 2u * dlev + (cnt <= small_max ? 1u : 0u);
+}
 
 __device__ __forceinline__ void level_range(int n_int, uint32_t& lo, uint32_t& hi) {
     const uint32_t per = (((uint32_t)n_int + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
@@ -525,47 +533,56 @@ __device__ __forceinline__ void level_range(int n_int, uint32_t& lo, uint32_t& h
     hi = min((uint32_t)n_int, lo + per);
 }
 
+static_assert(kLevelBuckets == 2 * kBlock, "two buckets per thread in the level kernels");
+
 __global__ __launch_bounds__(kBlock) void k_level_hist(int n_int, const uint8_t* __restrict__ dlev,
+                                                       const uint32_t* __restrict__ cnt, uint32_t small_max,
                                                        uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
+    __shared__ uint32_t h[kLevelBuckets];
     h[threadIdx.x] = 0;
+    h[threadIdx.x + kBlock] = 0;
     __syncthreads();
     uint32_t lo, hi;
     level_range(n_int, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[dlev[i]], 1u);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[level_bucket(dlev[i], cnt[i], small_max)], 1u);
     __syncthreads();
-    if (h[threadIdx.x]) gatomic_add(&hist[threadIdx.x], h[threadIdx.x]);
+    for (uint32_t k = threadIdx.x; k < kLevelBuckets; k += kBlock)
+        if (h[k]) gatomic_add(&hist[k], h[k]);
 }
 
 __global__ void k_level_scan(const uint32_t* __restrict__ hist, uint32_t* __restrict__ begin,
                              uint32_t* __restrict__ fill) {
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (int l = 0; l < 256; l++) {
-            begin[l] = run;
-            fill[l] = run;
-            run += hist[l];
+        for (uint32_t k = 0; k < kLevelBuckets; k++) {
+            begin[k] = run;
+            fill[k] = run;
+            run += hist[k];
         }
-        begin[256] = run;
+        begin[kLevelBuckets] = run;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_scatter(int n_int, const uint8_t* __restrict__ dlev,
+                                                          const uint32_t* __restrict__ cnt, uint32_t small_max,
                                                           uint32_t* __restrict__ fill,
                                                           uint32_t* __restrict__ order) {
-    __shared__ uint32_t h[256], base[256];
+    __shared__ uint32_t h[kLevelBuckets], base[kLevelBuckets];
     h[threadIdx.x] = 0;
+    h[threadIdx.x + kBlock] = 0;
     __syncthreads();
     uint32_t lo, hi;
     level_range(n_int, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[dlev[i]], 1u);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[level_bucket(dlev[i], cnt[i], small_max)], 1u);
     __syncthreads();
     // reserve this block's part of every bucket, then place its nodes there
-    base[threadIdx.x] = h[threadIdx.x] ? gatomic_add(&fill[threadIdx.x], h[threadIdx.x]) : 0u;
-    h[threadIdx.x] = 0;
+    for (uint32_t k = threadIdx.x; k < kLevelBuckets; k += kBlock) {
+        base[k] = h[k] ? gatomic_add(&fill[k], h[k]) : 0u;
+        h[k] = 0;
+    }
     __syncthreads();
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const uint32_t l = dlev[i];
+        const uint32_t l = level_bucket(dlev[i], cnt[i], small_max);
         order[base[l] + atomicAdd(&h[l], 1u)] = i;
     }
 }
@@ -663,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_boxes_level(int level, const uint32_
                                                         const float4* __restrict__ saabb,
                                                         float4* __restrict__ nbox, float* __restrict__ cost,
                                                         int leaf_size) {
-    const uint32_t beg = lvl_begin[level], end = lvl_begin[level + 1];
+    const uint32_t beg = lvl_begin[2 * level], end = lvl_begin[2 * level + 2];  // both size buckets
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t k = beg + blockIdx.x * kBlock + threadIdx.x; k < end; k += stride) {
         const uint32_t i = lvl_order[k];
@@ -953,9 +970,9 @@ static void carve_all(BuildBufs& b, Carver& c, size_t NI) {
     c.take(b.cnt, NI);
     c.take(b.cost, NI);
     c.take(b.lvl_order, NI);
-    c.take(b.lvl_hist, 256);
-    c.take(b.lvl_begin, 257);
-    c.take(b.lvl_fill, 256);
+    c.take(b.lvl_hist, kLevelBuckets);
+    c.take(b.lvl_begin, kLevelBuckets + 1);
+    c.take(b.lvl_fill, kLevelBuckets);
     c.take(b.nbox, 2 * NI);
     c.take(b.wsrc, b.capacity);
     c.take(b.wseg, b.capacity);
@@ -1062,11 +1079,13 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
         int nbits = 0;
         while ((1u << nbits) < N) nbits++;
         const unsigned gI = cdiv(n_int, kBlock);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, b.lvl_hist, (size_t)256, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, b.lvl_hist, (size_t)kLevelBuckets, 0u);
         const unsigned gL = std::min(gI, kLevelBlocks);
-        hipLaunchKernelGGL(k_level_hist, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_hist);
+        const uint32_t small_max = 2u * (uint32_t)b.leaf_size;  // subtrees no treelet restructures
+        hipLaunchKernelGGL(k_level_hist, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.cnt, small_max, b.lvl_hist);
         hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, s, b.lvl_hist, b.lvl_begin, b.lvl_fill);
-        hipLaunchKernelGGL(k_level_scatter, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.lvl_fill, b.lvl_order);
+        hipLaunchKernelGGL(k_level_scatter, dim3(gL), dim3(kBlock), 0, s, n_int, b.dlev, b.cnt, small_max, b.lvl_fill,
+                           b.lvl_order);
         hipLaunchKernelGGL(k_boxes_local, dim3(gI), dim3(kBlock), 0, s, (uint32_t)n_int, b.range, b.left, b.right,
                            b.cnt, b.dlev, b.saabb, b.nbox, b.cost, b.leaf_size);
         const unsigned lgrid = std::min(gI, 1024u);

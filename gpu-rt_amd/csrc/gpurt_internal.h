@@ -39,6 +39,8 @@ constexpr uint32_t kMaxLeafFirst = 0x0FFFFFFFu;  // 2^28 - 1 packed triangles ma
 // at least one binary level, so 64 launches always finish the tree.
 constexpr int kMaxLevels = 64;
 constexpr int kBlock = 256;
+// buckets of the per-prefix-length build passes: two per length (0..255)
+constexpr uint32_t kLevelBuckets = 512;
 
 __host__ __device__ inline uint32_t make_leaf(uint32_t first, uint32_t count) {
     return kLeafBit | ((count - 1u) << 28) | first;
@@ -235,9 +237,11 @@ struct BuildBufs {
     uint8_t* dlev = nullptr;  // common-prefix length (Karras delta) of each internal node
     uint32_t* cnt = nullptr;  // triangles under each internal node
     float* cost = nullptr;    // SAH cost of each internal node's subtree
-    uint32_t* lvl_order = nullptr;  // internal nodes bucketed by prefix length
-    uint32_t* lvl_hist = nullptr;
-    uint32_t* lvl_begin = nullptr;  // bucket l = lvl_order[lvl_begin[l] .. lvl_begin[l+1])
+    // internal nodes bucketed by prefix length l and size: bucket 2l holds those with
+    // more than 2 leaf_size triangles (the treelet roots), 2l + 1 the others
+    uint32_t* lvl_order = nullptr;
+    uint32_t* lvl_hist = nullptr;   // kLevelBuckets counts
+    uint32_t* lvl_begin = nullptr;  // bucket k = lvl_order[lvl_begin[k] .. lvl_begin[k+1])
     uint32_t* lvl_fill = nullptr;
     float4* nbox = nullptr;   // binary node boxes (lo, hi)
     uint32_t *wsrc = nullptr, *wseg = nullptr, *lvl = nullptr, *counters = nullptr;

@@ -3,7 +3,8 @@
 //
 // For every internal node r with more than 2 leaf_size triangles, bottom-up:
 //   1. form a treelet: start from r's two children and repeatedly open the treelet leaf
-//      (an internal node) with the largest surface area, up to N leaves (N = 5..7);
+//      (an internal node) with the largest surface area, up to N leaves (N = 5..7), one
+//      candidate per lane;
 //   2. find the binary topology over those leaves with the least SAH cost by dynamic
 //      programming over all 2^N - 1 leaf subsets (one 64-lane wave per treelet: lanes
 //      own subsets; each subset tries every split that keeps its lowest leaf on one
@@ -58,57 +59,85 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
     __shared__ int s_nl;
     __shared__ float s_cur;
     const int lane = threadIdx.x;
-    // one wave per treelet root of this prefix length (bucket from the box pass)
-    const uint32_t beg = lvl_begin[level], end = lvl_begin[level + 1];
+    // one wave per treelet root of this prefix length: the roots with more than
+    // 2 leaf_size triangles (bucket 2 * level; the others become at most two leaves of one
+    // wide node, where a new topology gains nothing measurable -- rays_scene: skipping them
+    // builds in 10.7 instead of 13.4 ms with the same ray times; skipping up to 16 or 32
+    // triangles builds faster still but costs 2-5% in ray time)
+    const uint32_t beg = lvl_begin[2 * level], end = lvl_begin[2 * level + 1];
     for (uint32_t k = beg + blockIdx.x; k < end; k += gridDim.x) {
         const int r = (int)lvl_order[k];
-        // a subtree of <= leaf_size triangles becomes one leaf: nothing to do; one of <=
-        // 2 leaf_size triangles becomes at most two leaves of one wide node, where a new
-        // topology gains nothing measurable (rays_scene: skipping them builds in 10.7
-        // instead of 13.4 ms with the same ray times; skipping up to 16 or 32 triangles
-        // builds faster still but costs 2-5% in ray time)
-        if (cnt[r] <= 2u * (uint32_t)leaf_size) continue;
         {
-            if (lane == 0) {
-                // refresh r's cost from its (finished) children, then form the treelet
-                float4 lo = nbox[2 * (size_t)r], hi = nbox[2 * (size_t)r + 1];
-                s_cur = sah_node_cost(lo, hi, cnt[r], ref_cost(left[r], saabb, cost),
-                                      ref_cost(right[r], saabb, cost), leaf_size);
-                cost[r] = s_cur;
-                s_leaf[0] = left[r];
-                s_leaf[1] = right[r];
-                s_int[0] = (uint32_t)r;
-                int nl = 2, ni = 1;
-                while (nl < N) {
-                    int best = -1;
-                    float best_a = -1.0f;
-                    for (int j = 0; j < nl; j++) {
-                        const uint32_t x = s_leaf[j];
-                        if (x & kLeafBit) continue;
-                        const float a = box_area(nbox[2 * (size_t)x], nbox[2 * (size_t)x + 1]);
-                        if (a > best_a) {
-                            best_a = a;
-                            best = j;
-                        }
-                    }
-                    if (best < 0) break;
-                    const uint32_t x = s_leaf[best];
-                    s_int[ni++] = x;
-                    s_leaf[best] = left[x];
-                    s_leaf[nl++] = right[x];
+            // Form the treelet with one candidate (treelet leaf) per lane: a lane that
+            // takes a candidate loads its box, triangle count, cost and, for an internal
+            // node, its children, all at once, so opening a node costs one round of
+            // loads (by the two lanes that receive its children) instead of a serial
+            // scan of every candidate's box.  The node opened is the internal candidate
+            // with the largest surface area, the lowest slot on ties.
+            uint32_t x = kEmpty, xl = 0, xr = 0, xc = 0;
+            float4 xlo = make_float4(0.f, 0.f, 0.f, 0.f), xhi = xlo;
+            float xcost = 0.0f, xarea = -1.0f;
+            auto take = [&](uint32_t ref) {
+                x = ref;
+                ref_box(ref, saabb, nbox, xlo, xhi);
+                if (ref & kLeafBit) {
+                    xc = 1u;
+                    xcost = kCt * box_area(xlo, xhi);
+                    xarea = -1.0f;  // not openable
+                } else {
+                    xc = cnt[ref];
+                    xcost = cost[ref];
+                    xl = left[ref];
+                    xr = right[ref];
+                    const float a = box_area(xlo, xhi);
+                    xarea = a > -1.0f ? a : -1.0f;  // (a NaN area is never opened)
                 }
-                s_nl = nl;
+            };
+            float4 rlo = xlo, rhi = xhi;
+            uint32_t rc = 0;
+            if (lane == 0) {
+                rlo = nbox[2 * (size_t)r];
+                rhi = nbox[2 * (size_t)r + 1];
+                rc = cnt[r];
             }
-            __syncthreads();
-            const int nl = s_nl;
+            if (lane < 2) take(lane ? right[r] : left[r]);
+            // the root's cost, refreshed from its (finished) children, the first two
+            // candidates
+            const float c_right = __shfl(xcost, 1);
+            if (lane == 0) {
+                s_cur = sah_node_cost(rlo, rhi, rc, xcost, c_right, leaf_size);
+                cost[r] = s_cur;
+                s_int[0] = (uint32_t)r;
+            }
+            int nl = 2, ni = 1;
+            while (nl < N) {
+                float a = lane < nl ? xarea : -1.0f;
+                int j = lane;
+#pragma unroll
+                for (int o = 4; o >= 1; o >>= 1) {  // N <= 7: lanes 0..7 hold every candidate
+                    const float oa = __shfl_xor(a, o);
+                    const int oj = __shfl_xor(j, o);
+                    if (oa > a || (oa == a && oj < j)) {
+                        a = oa;
+                        j = oj;
+                    }
+                }
+                if (__shfl(a, 0) < 0.0f) break;  // no internal candidate left
+                const int jb = __shfl(j, 0);
+                const uint32_t ob = __shfl(x, jb), lb = __shfl(xl, jb), rb = __shfl(xr, jb);
+                if (lane == 0) s_int[ni] = ob;
+                ni++;
+                if (lane == jb) take(lb);
+                if (lane == nl) take(rb);
+                nl++;
+            }
+            if (lane == 0) s_nl = nl;
             if (lane < nl) {
-                const uint32_t x = s_leaf[lane];
-                float4 lo, hi;
-                ref_box(x, saabb, nbox, lo, hi);
-                s_llo[lane] = lo;
-                s_lhi[lane] = hi;
-                s_lcnt[lane] = (x & kLeafBit) ? 1u : cnt[x];
-                s_lcost[lane] = ref_cost(x, saabb, cost);
+                s_leaf[lane] = x;
+                s_llo[lane] = xlo;
+                s_lhi[lane] = xhi;
+                s_lcnt[lane] = xc;
+                s_lcost[lane] = xcost;
             }
             __syncthreads();
             const int full = (1 << nl) - 1;
