@@ -114,7 +114,6 @@ __global__ void k_reduce_stats(int n_seg, const uint32_t* __restrict__ segstat_r
 __global__ __launch_bounds__(kBlock) void k_prep(const SegDesc* __restrict__ segs, int n_seg,
                                                  uint32_t N, float4* __restrict__ aabb_lo,
                                                  float4* __restrict__ aabb_hi,
-                                                 float4* __restrict__ tri_v,
                                                  uint32_t* __restrict__ segstat,
                                                  uint32_t* __restrict__ counters) {
     uint32_t g = blockIdx.x * kBlock + threadIdx.x;
@@ -155,9 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_prep(const SegDesc* __restrict__ seg
         }
         aabb_lo[g] = make_float4(lo[0], lo[1], lo[2], __uint_as_float((uint32_t)s));
         aabb_hi[g] = make_float4(hi[0], hi[1], hi[2], 0.0f);
-        tri_v[3 * (size_t)g + 0] = make_float4(v[0][0], v[0][1], v[0][2], __uint_as_float(t));
-        tri_v[3 * (size_t)g + 1] = make_float4(v[1][0], v[1][1], v[1][2], 0.0f);
-        tri_v[3 * (size_t)g + 2] = make_float4(v[2][0], v[2][1], v[2][2], 0.0f);
     }
     // reduce over the wave when it covers one mesh, else per lane; this block's replica
     segstat += (size_t)(blockIdx.x % kRep) * n_seg * kStat;
@@ -425,21 +421,45 @@ bool radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
 // k_pack: gather triangles and AABBs into sorted order
 // ------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_pack(uint32_t N, const uint32_t* __restrict__ vals,
-                                                 const float4* __restrict__ aabb_lo,
-                                                 const float4* __restrict__ aabb_hi,
-                                                 const float4* __restrict__ tri_v,
+// Triangles in sorted order: sorted primitive p is triangle t = vals[p] - tri_offset of
+// mesh s = keys[p] >> 30.  Its vertices are read from the mesh itself (neighbours in
+// Morton order share vertices, so these reads mostly hit the caches) rather than
+// gathered from a per-triangle copy in input order (two box rows and three vertex rows
+// at a random index each), and its box is rebuilt from them exactly as k_prep built it.
+__global__ __launch_bounds__(kBlock) void k_pack(uint32_t N, const uint64_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ vals,
+                                                 const SegDesc* __restrict__ segs,
                                                  float4* __restrict__ saabb,
                                                  float4* __restrict__ tris,
                                                  uint32_t* __restrict__ leaf_order) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= N) return;
-    uint32_t g = vals[p];
-    saabb[2 * (size_t)p] = aabb_lo[g];
-    saabb[2 * (size_t)p + 1] = aabb_hi[g];
-    tris[3 * (size_t)p + 0] = tri_v[3 * (size_t)g + 0];
-    tris[3 * (size_t)p + 1] = tri_v[3 * (size_t)g + 1];
-    tris[3 * (size_t)p + 2] = tri_v[3 * (size_t)g + 2];
+    const uint32_t g = vals[p];
+    const uint32_t s = (uint32_t)(keys[p] >> 30);
+    const SegDesc sd = segs[s];
+    const uint32_t t = g - sd.tri_offset;
+    float v[3][3];
+    bool finite = true;
+    for (int k = 0; k < 3; k++) {
+        uint32_t vi = sd.idx[3 * (size_t)t + k];
+        if (vi >= sd.n_verts) vi = 0;  // reported by k_prep (the build fails)
+        const float* q = sd.verts + (size_t)vi * sd.vstride;
+        v[k][0] = q[0]; v[k][1] = q[1]; v[k][2] = q[2];
+        finite = finite && isfinite(v[k][0]) && isfinite(v[k][1]) && isfinite(v[k][2]);
+    }
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < 3; k++) {
+            const float x = finite ? v[k][a] : 0.0f;
+            lo[a] = fminf(lo[a], x);
+            hi[a] = fmaxf(hi[a], x);
+        }
+    }
+    saabb[2 * (size_t)p] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(s));
+    saabb[2 * (size_t)p + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    tris[3 * (size_t)p + 0] = make_float4(v[0][0], v[0][1], v[0][2], __uint_as_float(t));
+    tris[3 * (size_t)p + 1] = make_float4(v[1][0], v[1][1], v[1][2], 0.0f);
+    tris[3 * (size_t)p + 2] = make_float4(v[2][0], v[2][1], v[2][2], 0.0f);
     if (leaf_order) leaf_order[p] = g;
 }
 
@@ -950,7 +970,6 @@ static void carve_all(BuildBufs& b, Carver& c, size_t NI) {
     c.take(b.segs, n_seg);
     c.take(b.aabb_lo, N);
     c.take(b.aabb_hi, N);
-    c.take(b.tri_v, 3 * (size_t)N);
     c.take(b.segstat_rep, (size_t)kStat * n_seg * kRep);
     c.take(b.segstat, (size_t)kStat * n_seg);
     c.take(b.keys0, N);
@@ -1051,7 +1070,7 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(cdiv(NI, kBlock), 4096u)), dim3(kBlock), 0, s,
                        b.nparent, NI, kEmpty);
     hipLaunchKernelGGL(k_prep, dim3(gN), dim3(kBlock), 0, s, b.segs, n_seg, N, b.aabb_lo, b.aabb_hi,
-                       b.tri_v, b.segstat_rep, b.counters);
+                       b.segstat_rep, b.counters);
     hipLaunchKernelGGL(k_reduce_stats, dim3(cdiv((size_t)n_seg * kStat, kBlock)), dim3(kBlock), 0, s, n_seg,
                        b.segstat_rep, b.segstat);
     hipLaunchKernelGGL(k_morton, dim3(gN), dim3(kBlock), 0, s, N, b.aabb_lo, b.aabb_hi, b.segstat,
@@ -1066,8 +1085,7 @@ int build_enqueue_binary(BuildBufs& b, int treelet_size, int treelet_rounds, hip
         kin = b.keys1;
         vin = b.vals1;
     }
-    hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, vin, b.aabb_lo, b.aabb_hi, b.tri_v,
-                       b.saabb, b.tris, b.leaf_order);
+    hipLaunchKernelGGL(k_pack, dim3(gN), dim3(kBlock), 0, s, N, kin, vin, b.segs, b.saabb, b.tris, b.leaf_order);
     if (N > 1) {
         const int n_int = (int)(N - 1);
         hipLaunchKernelGGL(k_karras, dim3(cdiv(N - 1, kBlock)), dim3(kBlock), 0, s, (int)N, kin, b.segs,

@@ -224,7 +224,7 @@ struct BuildBufs {
     uint32_t nblocks_sort = 0;
     uint32_t capacity = 0;  // wide nodes the node array can hold
     SegDesc* segs = nullptr;
-    float4 *aabb_lo = nullptr, *aabb_hi = nullptr, *tri_v = nullptr;
+    float4 *aabb_lo = nullptr, *aabb_hi = nullptr;
     uint32_t* segstat_rep = nullptr;  // per-mesh statistics, replicated for atomics
     uint32_t* segstat = nullptr;      // per-mesh statistics, reduced
     uint64_t *keys0 = nullptr, *keys1 = nullptr;

@@ -56,7 +56,6 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
     __shared__ uint8_t s_part[NS];
     __shared__ uint8_t s_list[NS];  // subsets ordered by size: size k at [s_kbase[k], + C(nl, k))
     __shared__ int s_kbase[N + 2], s_kfill[N + 2];
-    __shared__ int s_nl;
     __shared__ float s_cur;
     const int lane = threadIdx.x;
     // one wave per treelet root of this prefix length: the roots with more than
@@ -131,7 +130,6 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
                 if (lane == nl) take(rb);
                 nl++;
             }
-            if (lane == 0) s_nl = nl;
             if (lane < nl) {
                 s_leaf[lane] = x;
                 s_llo[lane] = xlo;
