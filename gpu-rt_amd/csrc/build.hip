@@ -306,10 +306,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ a,
 }
 
 // Scatter of one radix pass.  The block's tile (kSortTile pairs) is first ordered by
-// digit in LDS -- stably: rounds, waves and lanes in input order, a wave's lanes ranked
-// by an 8-ballot multisplit -- and then written out in that order, so each digit's run
-// of the tile (16 pairs on average) leaves as one contiguous stretch of the output
-// instead of one isolated 12-byte write per pair.
+// digit in LDS, stably, and then written out in that order, so each digit's run of the
+// tile (16 pairs on average) leaves as one contiguous stretch of the output instead of
+// one isolated 12-byte write per pair.  Each wave ranks its own 1024 consecutive pairs:
+// per round of 64, an 8-ballot multisplit ranks the lanes of equal digit, and a
+// wave-private LDS counter per digit (read by all lanes, advanced by each group's first
+// lane; a wave's LDS operations complete in order) carries the rank across rounds, so the
+// rounds need no block barrier.  One scan over (digit, wave) counts then gives every
+// wave's first slot per digit.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
                                                           uint64_t* __restrict__ kout,
@@ -317,47 +321,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
                                                           int shift,
                                                           const uint32_t* __restrict__ offs,
                                                           uint32_t nblocks) {
+    constexpr int kWaves = kBlock / 64;
+    constexpr uint32_t kWaveItems = kSortTile / kWaves;  // consecutive pairs per wave
+    static_assert(kWaveItems == 64 * kSortItems, "one pair per lane and round");
     __shared__ uint64_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
-    __shared__ uint32_t gbase[256];   // this block's first output slot of digit d
-    __shared__ uint32_t lstart[256];  // the tile's first LDS slot of digit d
-    __shared__ uint32_t cur[256];     // pairs of digit d placed so far
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t gbase[256];          // this block's first output slot of digit d
+    __shared__ uint32_t lstart[256];         // the tile's first LDS slot of digit d
+    __shared__ uint32_t wcnt[kWaves][256];   // a wave's pairs of digit d, then its first slot
+    __shared__ uint32_t wsum[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = blockIdx.x * kSortTile;
     const uint32_t m = min((uint32_t)kSortTile, n - tile);
     gbase[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
-    cur[tid] = 0;
-    for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
-    uint64_t key[kSortItems];
-    uint32_t val[kSortItems];
+    for (int w = 0; w < kWaves; w++) wcnt[w][tid] = 0;
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortItems; r++) {
-        const uint32_t i = r * kBlock + tid;
-        key[r] = i < m ? kin[tile + i] : 0ull;
-        val[r] = i < m ? vin[tile + i] : 0u;
-        if (i < m) atomicAdd(&cur[(uint32_t)(key[r] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    // exclusive scan of the tile's digit counts (Hillis-Steele over 256 entries)
-    uint32_t own = cur[tid], incl = own;
-    lstart[tid] = incl;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t x = tid >= off ? lstart[tid - off] : 0u;
-        __syncthreads();
-        incl += x;
-        lstart[tid] = incl;
-        __syncthreads();
-    }
-    lstart[tid] = incl - own;
-    cur[tid] = 0;
-    __syncthreads();
+    uint32_t* const mycnt = wcnt[wave];
+    const uint32_t wbase = (uint32_t)wave * kWaveItems;
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        const bool valid = r * kBlock + tid < (int)m;
+        const uint32_t i = wbase + (uint32_t)r * 64u + (uint32_t)lane;
+        const bool valid = i < m;
+        key[r] = valid ? kin[tile + i] : 0ull;
+        val[r] = valid ? vin[tile + i] : 0u;
         const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -365,20 +354,47 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
             const uint64_t bb = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
-        const uint32_t rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = lstart[d] + cur[d] + rank;
-            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+        const uint32_t rk = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t before = mycnt[d];
+        rank[r] = before + rk;
+        if (valid && rk == 0) mycnt[d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // digit tid: the tile's count and every wave's offset inside the digit's run
+    uint32_t c[kWaves], tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        c[w] = wcnt[w][tid];
+        tot += c[w];
+    }
+    // exclusive scan of the 256 digit totals: within each wave by shuffles, then across
+    uint32_t incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= off) incl += x;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t start = incl - tot;
+    for (int w = 0; w < wave; w++) start += wsum[w];
+    lstart[tid] = start;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        wcnt[w][tid] = start;
+        start += c[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        if (wbase + (uint32_t)r * 64u + (uint32_t)lane < m) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+            const uint32_t pos = mycnt[d] + rank[r];
             skey[pos] = key[r];
             sval[pos] = val[r];
         }
-        __syncthreads();
-        cur[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
-        __syncthreads();
     }
+    __syncthreads();
     for (uint32_t j = tid; j < m; j += kBlock) {
         const uint64_t k = skey[j];
         const uint32_t d = (uint32_t)(k >> shift) & 255u;
