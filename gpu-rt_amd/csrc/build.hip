@@ -829,32 +829,68 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
         uint32_t bref = a.wsrc[wi];
         uint32_t seg = a.wseg[wi];
         float pad = o2f(a.segstat[seg * kStat + 6]) * 0x1p-18f;
-        uint32_t refs[W];
+        // The wide node's children, kept in registers with their triangle counts and
+        // boxes: a child is loaded once, together with its own children when it is an
+        // internal node, so opening it (the child with the largest area among those with
+        // more than leaf_size triangles, the lowest slot on ties) costs one round of
+        // loads instead of re-reading every child's count and box.  Slots are addressed
+        // with compile-time indices only, so nothing spills to scratch.
+        uint32_t refs[W], rc[W], rl[W], rr[W];
+        float4 rlo[W], rhi[W];
+        float rar[W];
+        auto load_child = [&](uint32_t r, uint32_t& ref, uint32_t& c, uint32_t& l, uint32_t& rgt, float4& lo,
+                              float4& hi, float& ar) {
+            ref = r;
+            c = ref_count(r, a.cnt);
+            ref_box(r, a.saabb, a.nbox, lo, hi);
+            const bool internal = !(r & kLeafBit);
+            l = internal ? a.left[r] : 0u;
+            rgt = internal ? a.right[r] : 0u;
+            ar = c > (uint32_t)a.leaf_size ? box_area(lo, hi) : -1.0f;
+        };
         int n = 0;
         if (!act) {
             // nothing: n stays 0
-        } else if (ref_count(bref, a.cnt) <= (uint32_t)a.leaf_size) {
-            refs[n++] = bref;
         } else {
-            refs[n++] = a.left[bref];
-            refs[n++] = a.right[bref];
-            while (n < W) {
-                int best = -1;
-                float best_area = -1.0f;
+            uint32_t bc, bl, br;
+            float4 blo, bhi;
+            float bar;
+            load_child(bref, refs[0], bc, bl, br, blo, bhi, bar);
+            if (bc <= (uint32_t)a.leaf_size) {
+                rc[0] = bc;
+                rlo[0] = blo;
+                rhi[0] = bhi;
+                n = 1;
+            } else {
+                load_child(bl, refs[0], rc[0], rl[0], rr[0], rlo[0], rhi[0], rar[0]);
+                load_child(br, refs[1], rc[1], rl[1], rr[1], rlo[1], rhi[1], rar[1]);
+                n = 2;
+                while (n < W) {
+                    int best = -1;
+                    float best_area = -1.0f;
 #pragma unroll
-                for (int j = 0; j < W; j++) {
-                    if (j >= n) break;
-                    uint32_t r = refs[j];
-                    if (ref_count(r, a.cnt) <= (uint32_t)a.leaf_size) continue;
-                    float4 lo, hi;
-                    ref_box(r, a.saabb, a.nbox, lo, hi);
-                    float ar = box_area(lo, hi);
-                    if (ar > best_area) { best_area = ar; best = j; }
+                    for (int j = 0; j < W; j++) {
+                        if (j < n && rar[j] > best_area) {
+                            best_area = rar[j];
+                            best = j;
+                        }
+                    }
+                    if (best < 0) break;
+                    uint32_t ol = 0, orr = 0;
+#pragma unroll
+                    for (int j = 0; j < W; j++) {
+                        if (j == best) {
+                            ol = rl[j];
+                            orr = rr[j];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < W; j++) {
+                        if (j == best) load_child(ol, refs[j], rc[j], rl[j], rr[j], rlo[j], rhi[j], rar[j]);
+                        if (j == n) load_child(orr, refs[j], rc[j], rl[j], rr[j], rlo[j], rhi[j], rar[j]);
+                    }
+                    n++;
                 }
-                if (best < 0) break;
-                uint32_t r = refs[best];
-                refs[best] = a.left[r];
-                refs[n++] = a.right[r];
             }
         }
         WNode<W> out;
@@ -862,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 #pragma unroll
         for (int j = 0; j < W; j++) {
             if (j < n) {
-                const uint32_t c = ref_count(refs[j], a.cnt);
+                const uint32_t c = rc[j];
                 if (c <= (uint32_t)a.leaf_size) n_repack += c;
                 else n_internal++;
             }
@@ -872,12 +908,11 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 #pragma unroll
         for (int j = 0; j < W; j++) {
             if (j < n) {
-                uint32_t r = refs[j];
-                float4 lo, hi;
-                ref_box(r, a.saabb, a.nbox, lo, hi);
+                const uint32_t r = refs[j];
+                const float4 lo = rlo[j], hi = rhi[j];
                 out.lox[j] = lo.x - pad; out.loy[j] = lo.y - pad; out.loz[j] = lo.z - pad;
                 out.hix[j] = hi.x + pad; out.hiy[j] = hi.y + pad; out.hiz[j] = hi.z + pad;
-                uint32_t c = ref_count(r, a.cnt);
+                const uint32_t c = rc[j];
                 out.aux[j] = c;
                 if (c <= (uint32_t)a.leaf_size) {
                     uint32_t first;
