@@ -24,11 +24,6 @@
 
 namespace gpurt_impl {
 
-// binomial coefficients C(n, k), n, k <= 7: subsets of each size among a treelet's leaves
-__constant__ uint8_t kBinom[8][8] = {
-    {1, 0, 0, 0, 0, 0, 0, 0},  {1, 1, 0, 0, 0, 0, 0, 0},   {1, 2, 1, 0, 0, 0, 0, 0},   {1, 3, 3, 1, 0, 0, 0, 0},
-    {1, 4, 6, 4, 1, 0, 0, 0},  {1, 5, 10, 10, 5, 1, 0, 0}, {1, 6, 15, 20, 15, 6, 1, 0}, {1, 7, 21, 35, 35, 21, 7, 1}};
-
 // the s-th submask of mask in increasing order (bit i of s selects the i-th set bit)
 __device__ __forceinline__ int nth_submask(int s, int mask) {
     int out = 0;
@@ -130,6 +125,9 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
                 if (lane == nl) take(rb);
                 nl++;
             }
+            // every lane ran the same loop: nl is wave-uniform, so the table lookups below
+            // are scalar loads rather than one dependent vector load per lane
+            nl = __builtin_amdgcn_readfirstlane(nl);
             if (lane < nl) {
                 s_leaf[lane] = x;
                 s_llo[lane] = xlo;
@@ -158,8 +156,13 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
                 }
             }
             if (lane <= nl) {
-                int b = 0;
-                for (int j = 1; j < lane; j++) b += kBinom[nl][j];
+                // subsets of fewer than `lane` leaves (of at least one): the sum of C(nl, j)
+                // for 1 <= j < lane, with C(nl, j) = C(nl, j - 1) (nl - j + 1) / j
+                int b = 0, c = 1;
+                for (int j = 1; j < lane; j++) {
+                    c = c * (nl - j + 1) / j;
+                    b += c;
+                }
                 s_kbase[lane] = b;
                 s_kfill[lane] = 0;
             }
@@ -175,8 +178,9 @@ __global__ __launch_bounds__(64) void k_treelet(int level, const uint32_t* __res
             // a lane per split, and reduces.  Ties go to the larger P in both forms (the
             // first found in a descending walk of the submasks), so the tree does not
             // depend on which form ran.
+            int nk = nl;  // C(nl, k), updated at the top of each step
             for (int k = 2; k <= nl; k++) {
-                const int nk = kBinom[nl][k];
+                nk = nk * (nl - k + 1) / k;
                 const uint8_t* list = s_list + s_kbase[k];
                 if (k <= 4) {
                     for (int idx = lane; idx < nk; idx += 64) {
