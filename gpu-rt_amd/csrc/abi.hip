@@ -117,6 +117,88 @@ int compact_verts(const float* verts, size_t stride_floats, size_t n, float* out
     return GPURT_OK;
 }
 
+int ctx_scratch(gpurt_ctx_s* c, hipStream_t s, size_t bytes, CallScratch& out) {
+    out = CallScratch{};
+    out.s = s;
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    const bool capturing = hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone;
+    if (capturing) (void)hipGetLastError();
+    if (!capturing) {
+        std::lock_guard<std::mutex> lk(c->scratch_mu);
+        int k = -1;
+        for (size_t i = 0; i < c->scratch.size(); i++)
+            if (c->scratch[i].s == s) {
+                k = (int)i;
+                break;
+            }
+        if (k < 0) {
+            gpurt_ctx_s::StreamBuf nb;
+            nb.s = s;
+            if (hipEventCreateWithFlags(&nb.last, hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                nb.last = nullptr;
+            }
+            if (nb.last) {
+                c->scratch.push_back(nb);
+                k = (int)c->scratch.size() - 1;
+            }
+        }
+        if (k >= 0 && !c->scratch[k].busy) {
+            gpurt_ctx_s::StreamBuf& e = c->scratch[k];
+            // a recycled stream handle: the buffer's previous user may be another stream
+            if (e.mem && hipStreamWaitEvent(s, e.last, 0) != hipSuccess) return hip_error(hipGetLastError(), "hipStreamWaitEvent");
+            if (e.bytes < bytes) {
+                // Outside the stream-ordered pool, so the builds' own stream-ordered
+                // allocations keep reusing the pool's memory.  Growing waits for the
+                // device (hipFree), once per new largest call on the stream.
+                if (e.mem) (void)hipFree(e.mem);
+                e.mem = nullptr;
+                e.bytes = 0;
+                const size_t want = bytes + bytes / 4;  // room for somewhat larger calls
+                if (hipMalloc(&e.mem, want) != hipSuccess) {
+                    (void)hipGetLastError();
+                    e.mem = nullptr;
+                    return set_error(GPURT_E_OOM, "query scratch allocation failed");
+                }
+                e.bytes = want;
+            }
+            e.busy = true;
+            out.mem = e.mem;
+            out.slot = k;
+            return GPURT_OK;
+        }
+    }
+    // a call recorded into a graph, or the stream's buffer is held by another call
+    if (hipMallocAsync(&out.mem, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        out.mem = nullptr;
+        return set_error(GPURT_E_OOM, "query scratch allocation failed");
+    }
+    return GPURT_OK;
+}
+
+void ctx_scratch_done(gpurt_ctx_s* c, CallScratch& cs) {
+    if (!cs.mem) return;
+    if (cs.slot < 0) {
+        (void)hipFreeAsync(cs.mem, cs.s);
+    } else {
+        std::lock_guard<std::mutex> lk(c->scratch_mu);
+        gpurt_ctx_s::StreamBuf& e = c->scratch[cs.slot];
+        if (hipEventRecord(e.last, cs.s) != hipSuccess) (void)hipGetLastError();
+        e.busy = false;
+    }
+    cs = CallScratch{};
+}
+
+void ctx_scratch_release(gpurt_ctx_s* c) {
+    std::lock_guard<std::mutex> lk(c->scratch_mu);
+    for (auto& e : c->scratch) {
+        if (e.mem) (void)hipFree(e.mem);
+        if (e.last) (void)hipEventDestroy(e.last);
+    }
+    c->scratch.clear();
+}
+
 // node format used when gpurt_build_opts::node_format is 0
 constexpr int kDefaultFormat = kFormatMixed;
 
@@ -218,6 +300,7 @@ void gpurt_ctx_destroy(gpurt_ctx ctx) {
         }
         ctx->bvhs.clear();
     }
+    ctx_scratch_release(ctx);
     if (ctx->release) (void)hipStreamSynchronize(ctx->release);
     (void)hipGetLastError();
     if (ctx->release) (void)hipStreamDestroy(ctx->release);

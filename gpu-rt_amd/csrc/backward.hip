@@ -97,22 +97,21 @@ namespace {
 // scratch of the vertex-gradient sum over n items (3 terms each)
 struct TermScratch {
     void* mem = nullptr;
+    CallScratch cs;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     uint32_t *v0 = nullptr, *v1 = nullptr, *sort = nullptr;
     float *terms = nullptr, *partial = nullptr;
 };
 
-int alloc_terms(size_t n, hipStream_t s, TermScratch& t) {
+int alloc_terms(gpurt_ctx_s* c, size_t n, hipStream_t s, TermScratch& t) {
     if (3 * n > 0xFFFFFFFFull - kBlock) return set_error(GPURT_E_INVALID, "too many queries for one backward call");
     const size_t m = 3 * n;
     const size_t words = radix_sort_scratch_words((uint32_t)m);
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t bytes = 2 * up(8 * m) + 2 * up(4 * m) + 2 * up(12 * m) + up(4 * words) + 256;
-    if (hipMallocAsync(&t.mem, bytes, s) != hipSuccess) {
-        (void)hipGetLastError();
-        t.mem = nullptr;
-        return set_error(GPURT_E_OOM, "backward scratch allocation failed");
-    }
+    const int rc = ctx_scratch(c, s, bytes, t.cs);
+    if (rc) return rc;
+    t.mem = t.cs.mem;
     char* p = static_cast<char*>(t.mem);
     t.k0 = reinterpret_cast<uint64_t*>(p);
     p += up(8 * m);
@@ -162,13 +161,13 @@ int launch_cpq_bwd(gpurt_bvh_s* b, const float* q, size_t n, const int32_t* prim
         return GPURT_OK;
     }
     TermScratch t;
-    int rc = alloc_terms(n, s, t);
+    int rc = alloc_terms(b->ctx, n, s, t);
     if (rc) return rc;
     hipLaunchKernelGGL(k_cpq_bwd_terms, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n, q, prim, g,
                        b->tri_idx, (uint32_t)b->n_tris, b->verts, (uint32_t)b->n_verts, g_q, t.k0, t.v0, t.terms);
     sum_terms(t, n, (uint32_t)b->n_verts, g_verts, s);
     const hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(t.mem, s);
+    ctx_scratch_done(b->ctx, t.cs);
     if (e != hipSuccess) return hip_error(e, "closest-point backward launch");
     return GPURT_OK;
 }
@@ -186,14 +185,14 @@ int launch_ray_bwd(gpurt_bvh_s* b, const gpurt_ray* rays, size_t n, const int32_
         return GPURT_OK;
     }
     TermScratch ts;
-    int rc = alloc_terms(n, s, ts);
+    int rc = alloc_terms(b->ctx, n, s, ts);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ray_bwd_terms, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (uint32_t)n,
                        reinterpret_cast<const float4*>(rays), prim, t, uv, g, b->tri_idx, (uint32_t)b->n_tris,
                        b->verts, (uint32_t)b->n_verts, g_o, g_d, ts.k0, ts.v0, ts.terms);
     sum_terms(ts, n, (uint32_t)b->n_verts, g_verts, s);
     const hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(ts.mem, s);
+    ctx_scratch_done(b->ctx, ts.cs);
     if (e != hipSuccess) return hip_error(e, "ray backward launch");
     return GPURT_OK;
 }

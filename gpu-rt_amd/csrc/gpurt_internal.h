@@ -145,6 +145,16 @@ struct gpurt_ctx_s {
     std::mutex live_mu;
     std::set<gpurt_bvh_s*> bvhs;
     std::set<gpurt_tlas_s*> tlases;
+    // device scratch of the query and backward calls, one buffer per stream (ctx_scratch)
+    struct StreamBuf {
+        hipStream_t s = nullptr;
+        void* mem = nullptr;
+        size_t bytes = 0;
+        hipEvent_t last = nullptr;  // recorded after the buffer's last use
+        bool busy = false;          // a call is between ctx_scratch and ctx_scratch_done
+    };
+    std::mutex scratch_mu;
+    std::vector<StreamBuf> scratch;
 };
 
 namespace gpurt_impl {
@@ -298,10 +308,29 @@ struct TravPlan {
     unsigned grid = 1;
     uint32_t total = 256, spill_slots = 0, leaf_trigger = 32;
 };
+// Device scratch of one query or backward call.  Outside graph capture it is the
+// context's buffer for the call's stream: grown when a call needs more (hipMalloc, after
+// a device wait to free the old one), reused by the next call on that stream without an
+// allocation, and released with the context; the stream first waits for the buffer's last recorded use,
+// so a stream handle that the runtime recycles for another stream cannot overlap it.
+// While the stream is being captured into a graph, or while another call on the same
+// stream holds the buffer (two host threads), the call gets an allocation of its own,
+// freed in stream order when it is done.  (A stream-ordered allocation per call made
+// the host wait for the device on every large call: ~230 us for 1 M rays.)
+struct CallScratch {
+    void* mem = nullptr;
+    hipStream_t s = nullptr;
+    int slot = -1;  // the context's buffer index, or -1 for a call-owned allocation
+};
+int ctx_scratch(gpurt_ctx_s* c, hipStream_t s, size_t bytes, CallScratch& out);
+void ctx_scratch_done(gpurt_ctx_s* c, CallScratch& cs);
+void ctx_scratch_release(gpurt_ctx_s* c);  // after the device is idle (context destroy)
+
 struct TravScratch {
     void* mem = nullptr;
     uint32_t* counters = nullptr;
     uint2* spill = nullptr;
+    CallScratch cs;
 };
 // LDS per lane beyond the stack slots of the single-level kernels: the 64-bit result key
 // and the member count of work donation (traverse.h)
@@ -310,8 +339,8 @@ constexpr int kGroupKeyBytes = 12;
 // lds_extra: LDS bytes per lane beyond the stack slots
 int plan_traversal(gpurt_ctx ctx, int width, bool q8, QKind kind, int stack_need, const gpurt_query_opts* opts,
                    size_t n, TravPlan& p, int waves_per_simd = 0, int lds_extra = kGroupKeyBytes);
-int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t);
-void free_traversal_scratch(TravScratch& t, hipStream_t s);
+int alloc_traversal_scratch(gpurt_ctx_s* c, const TravPlan& p, hipStream_t s, TravScratch& t);
+void free_traversal_scratch(gpurt_ctx_s* c, TravScratch& t);
 // the same scratch in memory the caller holds
 size_t traversal_scratch_bytes(const TravPlan& p);
 void carve_traversal_scratch(const TravPlan& p, void* mem, TravScratch& t);

@@ -193,21 +193,21 @@ void clear_traversal_counters(const TravScratch& t, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kQueues), 0, s, t.counters);
 }
 
-int alloc_traversal_scratch(const TravPlan& p, hipStream_t s, TravScratch& t) {
-    // one stream-ordered allocation owned by the call
-    void* mem = nullptr;
-    if (hipMallocAsync(&mem, traversal_scratch_bytes(p), s) != hipSuccess) {
-        (void)hipGetLastError();
+int alloc_traversal_scratch(gpurt_ctx_s* c, const TravPlan& p, hipStream_t s, TravScratch& t) {
+    CallScratch cs;
+    const int rc = ctx_scratch(c, s, traversal_scratch_bytes(p), cs);
+    if (rc) {
         t.mem = nullptr;
-        return set_error(GPURT_E_OOM, "traversal scratch allocation failed");
+        return rc;
     }
-    carve_traversal_scratch(p, mem, t);
+    carve_traversal_scratch(p, cs.mem, t);
+    t.cs = cs;
     clear_traversal_counters(t, s);
     return GPURT_OK;
 }
 
-void free_traversal_scratch(TravScratch& t, hipStream_t s) {
-    if (t.mem) (void)hipFreeAsync(t.mem, s);
+void free_traversal_scratch(gpurt_ctx_s* c, TravScratch& t) {
+    ctx_scratch_done(c, t.cs);
     t.mem = nullptr;
 }
 
@@ -275,7 +275,7 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
         carve_traversal_scratch(p, fixed_scratch, t);
         clear_traversal_counters(t, s);
         t.mem = nullptr;  // not freed below
-    } else if ((rc = alloc_traversal_scratch(p, s, t))) {
+    } else if ((rc = alloc_traversal_scratch(b->ctx, p, s, t))) {
         return rc;
     }
     a.tris = b->tris;
@@ -306,7 +306,7 @@ static int run_query(gpurt_bvh_s* b, QKind kind, TravArgs a, size_t n, const gpu
         default: break;
     }
     hipError_t le = hipGetLastError();
-    free_traversal_scratch(t, s);
+    free_traversal_scratch(b->ctx, t);
     if (bs == 0) return set_error(GPURT_E_UNSUPPORTED, "node width");
     if (le != hipSuccess) return hip_error(le, "traversal launch");
     return GPURT_OK;

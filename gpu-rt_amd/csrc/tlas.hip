@@ -823,7 +823,7 @@ int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t*
             return set_error(GPURT_E_INVALID, "traversal scratch buffer too small");
         carve_traversal_scratch(p, scratch, sc);
         clear_traversal_counters(sc, s);
-    } else if ((rc = alloc_traversal_scratch(p, s, sc))) {
+    } else if ((rc = alloc_traversal_scratch(T->ctx, p, s, sc))) {
         return rc;
     }
     TlasArgs a{};
@@ -863,7 +863,7 @@ int tlas_trace_ex(gpurt_tlas T, const gpurt_ray* rays, size_t n, const uint32_t*
         default: break;
     }
     hipError_t le = hipGetLastError();
-    if (!scratch) free_traversal_scratch(sc, s);
+    if (!scratch) free_traversal_scratch(T->ctx, sc);
     if (!launched) return set_error(GPURT_E_UNSUPPORTED, "node width / stack depth");
     if (le != hipSuccess) return hip_error(le, "TLAS traversal launch");
     return GPURT_OK;

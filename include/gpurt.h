@@ -121,7 +121,14 @@ typedef struct gpurt_bvh_info {
 /* ---- context --------------------------------------------------------------------- */
 
 /* Create a context on HIP device `device`.  Replaces the ray-tracing part of
- * VK::Manager::init / init_rt (src/vk/vulkan.cpp:1136-1160, 1326-1385). */
+ * VK::Manager::init / init_rt (src/vk/vulkan.cpp:1136-1160, 1326-1385).
+ * Query and backward calls need device scratch (work-queue counters, the traversal
+ * stacks' spill area, the fixed-order sums' term buffers).  The context keeps one scratch
+ * buffer per stream that calls use in turn, grown to the largest call on that stream
+ * (growing waits for the device once) and released by gpurt_ctx_destroy, so a call does
+ * not allocate and returns without waiting for the device.  A call on a stream being
+ * captured into a HIP graph, or on a stream whose buffer another host thread's call
+ * holds, takes a stream-ordered allocation of its own instead. */
 GPURT_API int gpurt_ctx_create(int device, gpurt_ctx* out);
 /* Waits for the device, then frees the context (which must outlive its handles). */
 GPURT_API void gpurt_ctx_destroy(gpurt_ctx ctx);
@@ -145,7 +152,7 @@ GPURT_API int gpurt_ctx_status(gpurt_ctx ctx, uint32_t* flags, int clear);
  *   GPURT_SUMS_FIXED_ORDER  the terms are sorted by vertex (a stable radix sort) and each
  *                           vertex's terms added in a fixed order (a tree over query
  *                           order): the same bits on every run; 144 bytes of
- *                           stream-ordered scratch per query
+ *                           scratch per query (the context's per-stream buffer)
  *   GPURT_SUMS_ATOMIC       FP32 atomic adds, one pass; the summation order, and so the
  *                           last bits, can vary from run to run
  *   GPURT_SUMS_AUTO         (the default) whichever is faster for the call: the fixed

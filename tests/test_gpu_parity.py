@@ -95,6 +95,28 @@ def test_rays_small_scene_bitexact(ctx, dev, width, fmt):
     assert 0.2 < (prim >= 0).mean() < 0.9
 
 
+@pytest.mark.parametrize("kind", ["rays", "cpq"])
+def test_stack_spills_keep_answers_exact(ctx, dev, kind):
+    """8-wide nodes with the smallest LDS stack (8 entries): a node visit pushes up to 7
+    children, so pushes often do not fit below the last LDS slot; they then take the
+    per-push path, which sends the entries beyond the LDS slots to the device spill buffer.
+    The counting kernel (same tree, same options) confirms that entries spilled, and the
+    answers still equal the brute force."""
+    st = torch.zeros(4, dtype=torch.int64, device=dev)
+    if kind == "rays":
+        w = W.rays_scene(n_rays=8192, grid=101, n_spheres=6)
+        _ray_check(ctx, dev, w["verts"], w["tris"], w["rays"], w["rays_any"], node_width=8, stack_depth=8)
+        bvh = gpurt.Bvh(ctx, _t(w["verts"], dev), _t(w["tris"], dev), node_width=8)
+        bvh.trace_closest(_t(w["rays"], dev), stack_depth=8, stats=st)
+    else:
+        w = W.cpq_small(n_queries=4096)
+        bvh = _cpq_check(ctx, dev, w["verts"], w["tris"], w["queries"], node_width=8, stack_depth=8)
+        bvh.closest_point(_t(w["queries"], dev), stack_depth=8, stats=st)
+    nodes, tris, spills, _ = st.cpu().tolist()
+    print(kind, "nodes", nodes, "tris", tris, "spills", spills, flush=True)
+    assert spills > 0
+
+
 @pytest.mark.parametrize("refill", [1, 13, 64])
 @pytest.mark.parametrize("leaf", [1, 64])
 def test_wave_scheduling_options_keep_answers(ctx, dev, leaf, refill):
