@@ -146,7 +146,8 @@ int ctx_scratch(gpurt_ctx_s* c, hipStream_t s, size_t bytes, CallScratch& out) {
         if (k >= 0 && !c->scratch[k].busy) {
             gpurt_ctx_s::StreamBuf& e = c->scratch[k];
             // a recycled stream handle: the buffer's previous user may be another stream
-            if (e.mem && hipStreamWaitEvent(s, e.last, 0) != hipSuccess) return hip_error(hipGetLastError(), "hipStreamWaitEvent");
+            if (e.mem && hipStreamWaitEvent(s, e.last, 0) != hipSuccess)
+                return hip_error(hipGetLastError(), "hipStreamWaitEvent");
             if (e.bytes < bytes) {
                 // Outside the stream-ordered pool, so the builds' own stream-ordered
                 // allocations keep reusing the pool's memory.  Growing waits for the
