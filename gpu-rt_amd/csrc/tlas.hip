@@ -252,7 +252,7 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
             const bool idle = !has;
             const bool donor = has && !wait && !done && (!in_inst || st.sp > sp_base) &&
                                st.sp >= ((have || pleaf != kEmpty) ? 1 : 2);
-            if (__ballot(idle) != 0 && __ballot(donor) != 0) {
+            if ((uint32_t)__popcll(__ballot(idle)) >= kRayDonateIdle && __ballot(donor) != 0) {
                 bool take, give;
                 const uint32_t from = pair_donors(idle, donor, take, give);
                 uint2 e = make_uint2(0u, 0u);
@@ -368,20 +368,35 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
             continue;
         }
         if (!node_ready) continue;
-        if (STATS) n_nodes++;
+        // Node step: up to kRayNodeVisits visits in a row within one level, as in the
+        // one-level kernel; the stack pop stays above the instance's stack base, and a
+        // leaf child (a triangle leaf, or an instance at the top level) ends the step.
         const float slack = in_inst ? kTSlack : kTSlackTop;
-        float tn[W];
-        uint32_t c[W];
-        ray_children<W>(in_inst ? bnodes : top_nodes, cur, r.inv, r.oix, r.oiy, r.oiz, tmin, best, slack, tn, c);
-        if (!ANY) sort_children<W>(tn, c);
-        have = false;
-        bool hit[W];
+        const NodeT* const lnodes = in_inst ? bnodes : top_nodes;
+        const int floor_sp = in_inst ? sp_base : 0;
+#pragma unroll 1
+        for (int visit = 0; visit < kRayNodeVisits; visit++) {
+            if (STATS) n_nodes++;
+            float tn[W];
+            uint32_t c[W];
+            ray_children<W>(lnodes, cur, r.inv, r.oix, r.oiy, r.oiz, tmin, best, slack, tn, c);
+            if (!ANY) sort_children<W>(tn, c);
+            have = false;
+            bool hit[W];
 #pragma unroll
-        for (int j = 0; j < W; j++) hit[j] = c[j] != kEmpty;
-        const uint32_t pend = push_children<!ANY>(st, c, tn, hit, ok);
-        if (pend != kEmpty) {
-            cur = pend;
-            have = true;
+            for (int j = 0; j < W; j++) hit[j] = c[j] != kEmpty;
+            const uint32_t pend = push_children<!ANY>(st, c, tn, hit, ok);
+            if (pend != kEmpty) {
+                cur = pend;
+                have = true;
+            } else if (st.sp > floor_sp) {
+                const uint2 e = st.pop();
+                if (__uint_as_float(e.y) <= best) {
+                    cur = e.x;
+                    have = true;
+                }
+            }
+            if (!have || child_is_leaf(cur)) break;
         }
     }
     return ok;
@@ -389,11 +404,12 @@ __device__ __forceinline__ bool tlas_ray_stream(const NodeT* __restrict__ top_no
 
 // Resident waves per SIMD the two-level kernels are compiled for: the second level's
 // state (instance, stack base, BLAS pointers) and the work-donation state do not fit 64
-// VGPRs without spilling.  4-wide closest hit gets up to 85 (6 waves: at 72 the LDS stack
-// address spilled to scratch and every pop waited for it), 4-wide any hit fits 72 (7
-// waves, 8% faster than 6); 8-wide nodes take what they need.
-template <int W, bool ANY>
-constexpr int kTlasWaves = W == 8 ? 4 : (ANY ? 7 : 6);
+// VGPRs without spilling.  4-wide nodes get up to 85 (6 waves): at 72 the closest-hit
+// kernel spilled the LDS stack address to scratch and every pop waited for it, and with
+// two node visits per step the any-hit kernel spilled 8 VGPRs at 72 and none at 6 waves
+// (65-instance rays workload: any hit 240 -> 236 us).  8-wide nodes take what they need.
+template <int W>
+constexpr int kTlasWaves = W == 8 ? 4 : 6;
 
 struct TlasArgs {
     const void* top_nodes;
@@ -417,7 +433,7 @@ struct TlasArgs {
 };
 
 template <int W, bool Q, int S, int BS, bool ANY, bool STATS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W, ANY>))) void k_tlas_ray(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kTlasWaves<W>))) void k_tlas_ray(
     TlasArgs a) {
     const uint32_t total = gridDim.x * BS;
     const uint32_t gid = blockIdx.x * BS + threadIdx.x;
@@ -758,13 +774,11 @@ namespace gpurt_impl {
 // against 16 and 1).
 constexpr int kTlasDefaultTrigger = 32;
 
-static int tlas_waves(int W, bool any) {
-    return W == 8 ? (any ? kTlasWaves<8, true> : kTlasWaves<8, false>) : (any ? kTlasWaves<4, true> : kTlasWaves<4, false>);
-}
+static int tlas_waves(int W) { return W == 8 ? kTlasWaves<8> : kTlasWaves<4>; }
 
 static gpurt_query_opts tlas_opts(const gpurt_query_opts* opts, int W, bool any) {
     gpurt_query_opts o = opts ? *opts : gpurt_query_opts{};
-    if (o.waves_per_cu == 0) o.waves_per_cu = 4 * tlas_waves(W, any);
+    if (o.waves_per_cu == 0) o.waves_per_cu = 4 * tlas_waves(W);
     if (o.leaf_trigger == 0) o.leaf_trigger = kTlasDefaultTrigger;
     if (o.refill_trigger == 0) o.refill_trigger = any ? 8 : 24;
     return o;
@@ -777,7 +791,7 @@ static int tlas_plan(gpurt_tlas T, size_t n, bool any, const gpurt_query_opts* o
     // worst-case stack: the top levels' entries plus one BLAS's
     const int need = (W - 1) * (T->top->depth + T->max_blas_depth) + 2;
     return plan_traversal(T->ctx, W, T->q8, any ? QKind::Any : QKind::Closest, need, opts, n, p,
-                          tlas_waves(W, any), kTlasGroupBytes);
+                          tlas_waves(W), kTlasGroupBytes);
 }
 
 size_t tlas_scratch_bytes(gpurt_tlas T, size_t n) {
