@@ -340,6 +340,7 @@ typedef struct gpurt_batch_grad {
 GPURT_API int gpurt_batch_set_grads(gpurt_batch batch, const gpurt_batch_grad* grads);
 /* Run the recorded backward graph on `stream` (after gpurt_batch_run on the same stream). */
 GPURT_API int gpurt_batch_run_backward(gpurt_batch batch, void* stream);
+/* Waits for the device (every stream), then frees the batch and its recorded graphs. */
 GPURT_API void gpurt_batch_destroy(gpurt_batch batch);
 
 /* ---- autotuner ------------------------------------------------------------------------ */
@@ -430,6 +431,8 @@ typedef struct gpurt_instance {
  * TLAS is complete. */
 GPURT_API int gpurt_tlas_build(gpurt_ctx ctx, const gpurt_instance* instances, size_t n,
                                const gpurt_build_opts* opts, void* stream, gpurt_tlas* out);
+/* Free a TLAS: like gpurt_bvh_destroy, its memory is released in stream order after the
+ * work of every call that used it, without a host wait. */
 GPURT_API void gpurt_tlas_destroy(gpurt_tlas tlas);
 
 /* Closest hit through the TLAS: per ray obj (custom_index of the hit instance, -1 on a
@@ -473,6 +476,7 @@ typedef struct gpurt_object {
  * scene.  Returns when the scene is complete. */
 GPURT_API int gpurt_scene_create(gpurt_tlas tlas, const gpurt_object* objects, size_t n_objects,
                                  void* stream, gpurt_scene* out);
+/* Waits for the device (every stream), then frees the scene. */
 GPURT_API void gpurt_scene_destroy(gpurt_scene scene);
 
 /* Per-frame constants: RTPipe_Constants (src/vk/rt.h:85-102) and the camera's inverse
