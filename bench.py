@@ -159,13 +159,21 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     rays = torch.from_numpy(w["rays"]).to(dev)
     rays_any = torch.from_numpy(w["rays_any"]).to(dev)
     n = rays.shape[0]
+    # The process's first build also loads the build kernels' code objects (one-time
+    # cost); the build leg reports the second, warm build and the first one beside it.
+    build_kw = dict(node_width=args.width, treelet_size=args.treelet, node_format=args.format,
+                    leaf_size=args.leaf)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bvh = gpurt.Bvh(ctx, tv, tf, node_width=args.width, treelet_size=args.treelet,
-                    node_format=args.format, leaf_size=args.leaf)
+    bvh = gpurt.Bvh(ctx, tv, tf, **build_kw)
+    build_cold_ms = (time.perf_counter() - t0) * 1e3
+    bvh.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bvh = gpurt.Bvh(ctx, tv, tf, **build_kw)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
-    log(f"[bench] GPU build {build_ms:.1f} ms, {info}")
+    log(f"[bench] GPU build {build_ms:.1f} ms (first build of the process {build_cold_ms:.1f} ms), {info}")
     prim = torch.empty(n, dtype=torch.int32, device=dev)
     t = torch.empty(n, dtype=torch.float32, device=dev)
     uv = torch.empty((n, 2), dtype=torch.float32, device=dev)
@@ -246,7 +254,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
     rl["copy_achievable_GBps"] = copy_gbs
     rl["frac_of_copy"] = rl["achieved"] / copy_gbs
     rl["traffic"] = measured_traffic(args, int(F.shape[0]), n, info["node_format"])
-    build_roof = roof(build_bytes(info), build_ms, "BVH build, all kernels (host-timed call)")
+    build_roof = roof(build_bytes(info), build_ms, "BVH build, all kernels (host-timed warm call)")
     res = {
         "value": 2 * n / (ms * 1e-3),
         "ms_per_step": ms,
@@ -269,6 +277,7 @@ def ray_bench(args, torch, gpurt, W, O, dev, ctx):
                        "serial_rays_per_s": 2 * n / (ms_serial * 1e-3), "ms_per_step_serial": ms_serial,
                        "closest_rays_per_s": n / (ms_closest * 1e-3),
                        "any_rays_per_s": n / (ms_any * 1e-3), "gpu_build_ms": build_ms,
+                       "gpu_build_first_call_ms": build_cold_ms,
                        "build_roofline": build_roof,
                        "hit_rate": hit_rate, "occluded_rate": occ_rate,
                        "any_nodes_per_ray": nodes_a / n, "any_tris_per_ray": tris_a / n,
