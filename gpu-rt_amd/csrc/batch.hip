@@ -209,6 +209,7 @@ struct BatchGradDev {
     float* g_origin;
     float* g_dir;
     float* g_verts;
+    uint64_t voff;  // the mesh's first vertex in the batch-wide numbering of the fixed-order sums
 };
 
 constexpr uint32_t kZeroSeg = 4096;  // floats cleared per block of the zeroing kernel
@@ -240,14 +241,14 @@ __global__ __launch_bounds__(256) void k_batch_cpq_finish(const BatchMeshDev* __
 // Batched backward, one wave per 64-item chunk of the forward chunk list (closest-point
 // chunks, then ray chunks).  Query and ray gradients are written here.  Vertex terms are
 // scatter-added with FP32 atomics when `keys` is null; otherwise item (chunk c, lane l)
-// writes them to term slots 3 (64 c + l) .. + 2, keyed by (mesh << vbits | vertex), or
-// `none` when it adds nothing (no such item, no vertex gradient requested), for the
-// fixed-order sum.
+// writes them to term slots 3 (64 c + l) .. + 2, keyed by the vertex's batch-wide index
+// (the mesh's voff + its vertex), or `none` when it adds nothing (no such item, no vertex
+// gradient requested), for the fixed-order sum.
 template <bool RAYS>
 __global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restrict__ meshes,
                                                    const BatchGradDev* __restrict__ grads,
                                                    const uint2* __restrict__ chunks, uint32_t n_chunks,
-                                                   uint32_t first_chunk, int vbits, uint64_t none,
+                                                   uint32_t first_chunk, uint64_t none,
                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                    float* __restrict__ terms) {
     const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restric
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const size_t k = k0 + j;
-        keys[k] = (t.v[j] == kNoVertex || !gd.g_verts) ? none : ((uint64_t)ch.x << vbits | t.v[j]);
+        keys[k] = (t.v[j] == kNoVertex || !gd.g_verts) ? none : gd.voff + t.v[j];
         vals[k] = (uint32_t)k;
         terms[3 * k] = t.c[j][0];
         terms[3 * k + 1] = t.c[j][1];
@@ -281,14 +282,23 @@ __global__ __launch_bounds__(256) void k_batch_bwd(const BatchMeshDev* __restric
     }
 }
 
-// adds a (mesh << vbits | vertex) total to that mesh's vertex gradient (cleared by
-// k_batch_zero_vgrads; each key has one run)
+// adds a batch-wide vertex's total to its mesh's vertex gradient (cleared by
+// k_batch_zero_vgrads; each key has one run).  voff[0..n_meshes] are the meshes' first
+// batch-wide vertices (a mesh without a vertex gradient adds none), so the mesh is the
+// last m with voff[m] <= key.
 struct MeshVertexSink {
     const BatchGradDev* grads;
-    int vbits;
+    const uint64_t* voff;
+    uint32_t n_meshes;
     __device__ void operator()(uint64_t key, float x, float y, float z) const {
-        float* g = grads[key >> vbits].g_verts;
-        const size_t v = key & ((1ull << vbits) - 1);
+        uint32_t lo = 0, hi = n_meshes;  // voff[lo] <= key < voff[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (voff[mid] <= key) lo = mid;
+            else hi = mid;
+        }
+        float* g = grads[lo].g_verts;
+        const size_t v = key - voff[lo];
         g[3 * v] += x;
         g[3 * v + 1] += y;
         g[3 * v + 2] += z;
@@ -579,6 +589,7 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
     const size_t n_meshes = b->host_meshes.size();
     std::vector<BatchGradDev> gd(n_meshes);
     std::vector<uint2> segs;
+    std::vector<uint64_t> voff(n_meshes + 1, 0);  // batch-wide vertex numbering
     for (size_t m = 0; m < n_meshes; m++) {
         const gpurt_batch_mesh& hm = b->host_meshes[m];
         const gpurt_batch_grad& g = grads[m];
@@ -589,7 +600,8 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
             return set_error(GPURT_E_INVALID, "too many vertices for a batched vertex gradient" + at);
         gd[m] = BatchGradDev{hm.verts, hm.vstride_bytes / 4, hm.tri_idx, (uint32_t)hm.n_tris, (uint32_t)hm.n_verts,
                              hm.n_q ? g.g_dist : nullptr, g.g_q, hm.n_r ? g.g_t : nullptr, g.g_origin, g.g_dir,
-                             g.g_verts};
+                             g.g_verts, voff[m]};
+        voff[m + 1] = voff[m] + (g.g_verts ? (uint64_t)hm.n_verts : 0);
         if (g.g_verts)
             for (uint64_t f = 0; f < 3 * (uint64_t)hm.n_verts; f += kZeroSeg) segs.push_back(make_uint2((uint32_t)m, (uint32_t)f));
     }
@@ -600,16 +612,13 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
     const uint64_t m64 = fixed ? 3ull * 64 * n_chunks : 0;
     if (m64 > 0xFFFFFFFFull - kBlock) return set_error(GPURT_E_INVALID, "too many queries for a batched backward");
     const uint32_t m = (uint32_t)m64;
-    uint32_t max_verts = 1;
-    for (const gpurt_batch_mesh& hm : b->host_meshes) max_verts = std::max<uint32_t>(max_verts, (uint32_t)hm.n_verts);
-    int vbits = 1, mbits = 1;
-    while (vbits < 32 && (max_verts - 1) >> vbits) vbits++;
-    while (mbits < 32 && (n_meshes >> mbits)) mbits++;  // n_meshes itself marks "no vertex"
-    if (vbits + mbits > 63) return set_error(GPURT_E_INVALID, "batch too large for a batched backward");
-    const uint64_t none = (uint64_t)n_meshes << vbits;
+    // sort keys: batch-wide vertex indices, and their count itself for "no vertex"
+    const uint64_t none = voff[n_meshes];
+    int kbits = 1;
+    while (kbits < 64 && (none >> kbits)) kbits++;
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t tbytes = 2 * up(8 * (size_t)m) + 2 * up(4 * (size_t)m) + 2 * up(12 * (size_t)m) +
-                          up(4 * radix_sort_scratch_words(std::max<uint32_t>(m, 1)));
+                          up(4 * radix_sort_scratch_words(std::max<uint32_t>(m, 1))) + up(8 * (n_meshes + 1));
     DevGuard dg(b->ctx->device);
     if (!dg.ok) return set_error(GPURT_E_DEVICE, "hipSetDevice failed");
     b->drop_backward();
@@ -623,6 +632,7 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
     uint64_t *k0 = nullptr, *k1 = nullptr;
     uint32_t *v0 = nullptr, *v1 = nullptr, *sort_scratch = nullptr;
     float *terms = nullptr, *partial = nullptr;
+    uint64_t* voff_d = nullptr;
     if (m) {
         char* tp = static_cast<char*>(b->terms_mem);
         const size_t m8 = up(8 * (size_t)m), m4 = up(4 * (size_t)m), m12 = up(12 * (size_t)m);
@@ -633,10 +643,13 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
         terms = reinterpret_cast<float*>(tp + 2 * m8 + 2 * m4);
         partial = reinterpret_cast<float*>(tp + 2 * m8 + 2 * m4 + m12);
         sort_scratch = reinterpret_cast<uint32_t*>(tp + 2 * m8 + 2 * m4 + 2 * m12);
+        voff_d = reinterpret_cast<uint64_t*>(tp + 2 * m8 + 2 * m4 + 2 * m12 +
+                                             up(4 * radix_sort_scratch_words(std::max<uint32_t>(m, 1))));
     }
     if (hipMemcpy(b->grads_d, gd.data(), sizeof(BatchGradDev) * n_meshes, hipMemcpyHostToDevice) != hipSuccess ||
         (!segs.empty() &&
-         hipMemcpy(b->zero_segs_d, segs.data(), sizeof(uint2) * segs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+         hipMemcpy(b->zero_segs_d, segs.data(), sizeof(uint2) * segs.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        (voff_d && hipMemcpy(voff_d, voff.data(), sizeof(uint64_t) * (n_meshes + 1), hipMemcpyHostToDevice) != hipSuccess)) {
         int rc = hip_error(hipGetLastError(), "batch gradient setup");
         b->drop_backward();
         return rc;
@@ -651,18 +664,18 @@ int gpurt_batch_set_grads(gpurt_batch b, const gpurt_batch_grad* grads) {
                            b->zero_segs_d);
     if (b->n_cpq_chunks)
         hipLaunchKernelGGL(k_batch_bwd<false>, dim3(cdiv(b->n_cpq_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
-                           b->grads_d, b->chunks_d, b->n_cpq_chunks, 0u, vbits, none, k0, v0, terms);
+                           b->grads_d, b->chunks_d, b->n_cpq_chunks, 0u, none, k0, v0, terms);
     if (b->n_ray_chunks)
         hipLaunchKernelGGL(k_batch_bwd<true>, dim3(cdiv(b->n_ray_chunks, 4)), dim3(256), 0, b->cap, b->meshes_d,
-                           b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks, b->n_cpq_chunks, vbits, none,
+                           b->grads_d, b->chunks_d + b->n_cpq_chunks, b->n_ray_chunks, b->n_cpq_chunks, none,
                            k0, v0, terms);
     if (m) {
         // each (mesh, vertex)'s terms in chunk order, summed in a fixed order
-        const bool in1 = radix_sort_pairs(k0, v0, k1, v1, m, vbits + mbits, sort_scratch, b->cap);
+        const bool in1 = radix_sort_pairs(k0, v0, k1, v1, m, kbits, sort_scratch, b->cap);
         hipLaunchKernelGGL(k_wave_segsum<kBlock>, dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m, in1 ? k1 : k0,
                            in1 ? v1 : v0, terms, partial);
         hipLaunchKernelGGL((k_run_sum<kBlock, MeshVertexSink>), dim3(cdiv(m, kBlock)), dim3(kBlock), 0, b->cap, m,
-                           in1 ? k1 : k0, partial, none, MeshVertexSink{b->grads_d, vbits});
+                           in1 ? k1 : k0, partial, none, MeshVertexSink{b->grads_d, voff_d, (uint32_t)n_meshes});
     }
     hipError_t el = hipGetLastError();
     hipGraph_t g = nullptr;
